@@ -1,0 +1,163 @@
+"""Benchmark of the Yukawa pair-force + velocity-Verlet hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload: the 27,648-ion crust mixture (0.6 Z=8 A=16, 0.3 Z=26 A=56,
+0.1 Z=34 A=80; n = 7.18e-5 fm^-3, kappa = 0.7, Gamma(Z=8) = 100, cutoff
+min(L/2, 12 lambda), dt = 0.05/omega_p), positions uniform random from seed
+1001, state resident in HBM before timing.  A step is one full velocity-Verlet
+step (drift + all-pairs force + kick), with energies/pressure measured every
+10th step.  Both precision forms run; the headline ``value`` is the FP64
+drop-in form, the mixed form is reported beside it.
+
+Prints ONE JSON line.  ``value`` = pair interactions/s = N(N-1)/2 x steps /
+wall seconds.  ``roofline`` is the FP64 pair kernel against the FP64 vector
+peak, with the kernel's average duration from HIP events recorded on the
+stream it runs on.  ``cpu_baseline`` times the reference's own Fortran OpenMP
+build (oracle/_ref) on this host's cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector
+PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector (packed)
+FLOPS_PER_PAIR_FP64 = 90     # directed pair, force-only step, special functions costed
+FLOPS_PER_PAIR_MEAS = 109    # measurement step
+FM_C_SECONDS = 3.33564e-24
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nind", type=int, default=10, help="measure every nind steps")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-calls", type=int, default=3)
+    ap.add_argument("--forms", default="fp64,mixed")
+    return ap.parse_args()
+
+
+def run_form(w, precision, steps, warmup, nind):
+    from yukawamd_amd.system import from_workload
+    s = from_workload(w, precision=precision, timing=True)
+    try:
+        s.accel(True)
+        if warmup:
+            s.run(warmup, nind)
+        s.reset_timing()
+        t0 = time.perf_counter()
+        recs = s.run(steps, nind)      # returns after the device has finished
+        t = time.perf_counter() - t0
+        tim = s.timing()
+        return {"seconds": t, "timing": tim, "records": recs}
+    finally:
+        s.close()
+
+
+def cpu_baseline(calls):
+    env = dict(os.environ)
+    threads = len(os.sched_getaffinity(0))
+    threads = min(threads, 16)
+    env["OMP_NUM_THREADS"] = str(threads)
+    env.setdefault("OMP_PROC_BIND", "close")
+    cmd = [sys.executable, "-m", "oracle.cpu_bench", "--workload", "ion_mixture_27648",
+           "--calls", str(calls)]
+    out = subprocess.run(cmd, cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
+    if out.returncode != 0:
+        return {"error": out.stderr[-400:]}
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(independent systems shard one per GPU; see the sharding design)",
+              file=sys.stderr)
+        return 2
+    from yukawamd_amd import workloads as W
+    w = W.ion_mixture()
+    n = w.n
+    pairs = n * (n - 1) // 2
+    forms = {}
+    for prec in args.forms.split(","):
+        r = run_form(w, prec, args.steps, args.warmup, args.nind)
+        t = r["seconds"]
+        tim = r["timing"]
+        avg_ms = tim["force_ms"] / max(tim["force_launches"], 1)
+        nmeas = args.steps // args.nind if args.nind else 0
+        flops = (args.steps - nmeas) * FLOPS_PER_PAIR_FP64 + nmeas * FLOPS_PER_PAIR_MEAS
+        directed = n * (n - 1)
+        achieved = directed * flops / args.steps / (avg_ms * 1e-3) / 1e12
+        forms[prec] = {
+            "pair_interactions_per_s": pairs * args.steps / t,
+            "ms_per_step": 1e3 * t / args.steps,
+            "steps_per_s": args.steps / t,
+            "ns_per_day": w.dt * FM_C_SECONDS * 1e9 * (args.steps / t) * 86400,
+            "omega_p_t_per_day": 0.05 * (args.steps / t) * 86400,
+            "force_kernel_avg_ms": avg_ms,
+            "force_kernel_tflops": achieved,
+            "last_record": r["records"][-1] if r["records"] else None,
+        }
+    f64 = forms.get("fp64") or next(iter(forms.values()))
+    line = {
+        "metric": "pair interactions/s at N=27,648 (FP64 drop-in; mixed beside it)",
+        "value": f64["pair_interactions_per_s"],
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": f64["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded uniform-random positions, Maxwell-Boltzmann velocities)",
+        "config": {"workload": "ion_mixture_27648", "n": n, "species": "0.6 Z8A16/0.3 Z26A56/"
+                   "0.1 Z34A80", "kappa": 0.7, "gamma_z8": 100, "rccut_fm": w.params.rccut,
+                   "box_fm": w.params.xl[0], "dt_fm_per_c": w.dt, "nind": args.nind,
+                   "integrator": "vel_verlet_2", "parallelism": "single-gpu"},
+        "ns_per_day": f64["ns_per_day"],
+        "forms": forms,
+    }
+    if "mixed" in forms and "fp64" in forms:
+        line["mixed_over_fp64"] = (forms["mixed"]["pair_interactions_per_s"]
+                                   / forms["fp64"]["pair_interactions_per_s"])
+    line["roofline"] = {
+        "bound": "valu",
+        "kernel": "k_pairs<MIX,FP64>",
+        "achieved": f64["force_kernel_tflops"],
+        "peak": PEAK_FP64_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": f64["force_kernel_tflops"] / PEAK_FP64_TFLOPS,
+        "traffic": None,
+        "flops_per_directed_pair": FLOPS_PER_PAIR_FP64,
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(args.cpu_calls)
+        if "error" in cb:
+            line["cpu_baseline"] = {"value": None, "error": cb["error"]}
+        else:
+            line["cpu_baseline"] = {
+                "value": cb["pair_interactions_per_s"], "unit": "pair interactions/s",
+                "cores": cb["threads"], "kind": cb["kind"],
+                "sample": f"{cb['calls']} force-only accel calls of ion_mixture_27648 "
+                          f"({cb['seconds']:.1f} s, {cb['cpu']})",
+            }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

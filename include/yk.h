@@ -1,0 +1,179 @@
+/*
+ * yk.h -- C ABI of the MI355X Yukawa pair-force / velocity-Verlet library
+ *         (libyukawa_hip.so, built from yukawamd_amd/csrc).
+ *
+ * The reference drives its hot path through two implicit-interface Fortran
+ * subroutines that read and write module globals (src/md_globals.f:153-172):
+ *
+ *   accel(do_measurements)   src/int_control.f:52-81
+ *       -> accel_ion_pure    src/int_ion_pure.f:98-254
+ *       -> accel_ion_mix     src/int_ion_mix.f:105-295
+ *       -> accel_nn          src/int_nn.f:130-345
+ *   newton(do_measurements)  src/vel_verlet_2.f:18-120
+ *
+ * This ABI replaces both.  A handle owns one particle system resident in HBM
+ * (structure-of-arrays); the host hands arrays over in the reference's
+ * Fortran (3,n) column-major layout and gets them back the same way.  A
+ * Fortran host binds every function below with iso_c_binding
+ * (yukawamd_amd/fortran/yk_bind.f90).
+ *
+ * Conventions
+ *   - Plain C types only; every function returns YK_OK (0) or a yk_status.
+ *     The library never exits the process (the reference's md_exit,
+ *     src/md_utils.f:158-183, is the caller's business); yk_last_error()
+ *     returns a message for the calling thread's last failure.
+ *   - Logicals are int32_t (0 = .false.).
+ *   - Host arrays are borrowed for the duration of the call only.
+ *   - One host thread per handle; each handle launches on its own HIP stream
+ *     on the device that was current at yk_create.  Calls return after the
+ *     work they describe is complete, except where noted.
+ */
+#ifndef YK_H
+#define YK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YK_ABI_VERSION 1
+
+typedef struct yk_system yk_system;
+
+/* sim_type, src/int_control.f:60-79 ('B-field' sets a = 0, :71-72). */
+enum yk_sim_type {
+    YK_PURE_ION = 0,     /* 'pure-ion'    */
+    YK_ION_MIXTURE = 1,  /* 'ion-mixture' */
+    YK_NUCLEON = 2,      /* 'nucleon'     */
+    YK_BFIELD_TEST = 3   /* 'B-field'     */
+};
+
+enum yk_precision {
+    YK_FP64 = 0,   /* FP64 throughout: the drop-in form                        */
+    YK_MIXED = 1   /* FP64 minimum image and accumulation, FP32 pair functions */
+};
+
+enum yk_flags {
+    /* Keep the reference mixture kernel's last-particle behaviour: particle
+     * n-1 never receives its zii factor (src/int_ion_mix.f:158, :201, :250). */
+    YK_FLAG_REF_F1 = 1u << 0,
+    /* Kick-drift-kick (src/vel_verlet_1.f:16-89) instead of the linked
+     * predictor/corrector form (src/vel_verlet_2.f:33-79). */
+    YK_FLAG_KDK = 1u << 1,
+    /* Record HIP-event timings of every force launch (yk_get_timing). */
+    YK_FLAG_TIMING = 1u << 2
+};
+
+enum yk_status {
+    YK_OK = 0,
+    YK_ERR_ARG = 1,         /* invalid argument                              */
+    YK_ERR_HIP = 2,         /* HIP runtime error                             */
+    YK_ERR_NOMEM = 3,       /* device or host allocation failed              */
+    YK_ERR_STATE = 4,       /* call not valid in the handle's current state  */
+    YK_ERR_NONFINITE = 5,   /* NaN/Inf in an input array                     */
+    YK_ERR_UNSUPPORTED = 6, /* combination not implemented                   */
+    YK_ERR_CAPACITY = 7     /* admission / capacity limit reached            */
+};
+
+/* Run parameters: the module globals accel/newton read
+ * (src/md_globals.f:36-140). */
+typedef struct yk_params {
+    int32_t sim_type;    /* enum yk_sim_type                                  */
+    int32_t precision;   /* enum yk_precision                                 */
+    uint32_t flags;      /* enum yk_flags                                     */
+    int32_t coulomb_on;  /* coulomb .eq. 'screened-coulomb' (int_nn.f:167)    */
+    int32_t nuclear_on;  /* nuclear .eq. 'HPP'              (int_nn.f:168)    */
+    int32_t reserved0;
+    double xl[3];        /* box edge lengths (fm)                             */
+    double xmuc;         /* inverse screening length (1/fm)                   */
+    double rccut;        /* Coulomb cutoff radius (fm)                        */
+    double frp;          /* ion form factor                                   */
+    double vc;           /* Coulomb coupling ALPHA*HBARC (MeV fm)             */
+    double xmass;        /* nucleon mass (MeV)                                */
+    double rho;          /* number density (1/fm^3), pressure only            */
+    double kT;           /* temperature (MeV), pressure only                  */
+    double zi, ai;       /* pure-ion Z and A                                  */
+    double aa, bb, cc;   /* HPP strengths (MeV)                               */
+    double xpacket;      /* HPP Lambda (fm^2)                                 */
+    double rncut;        /* nuclear cutoff radius (fm)                        */
+    double bfield;       /* uniform B along z (Gauss), vel_verlet_2.f:61-69   */
+} yk_params;
+
+/* Scalars accel/newton leave in md_globals (ev, ek, px, pp). */
+typedef struct yk_meas {
+    double ev;     /* potential energy per particle (MeV)                    */
+    double ek;     /* kinetic energy per particle (MeV); newton only         */
+    double px;     /* pressure rho*(kT + vir/(3n)) (MeV/fm^3)                */
+    double vir;    /* virial sum (MeV)                                       */
+    double pp[6];  /* pressure tensor xx xy xz yy yz zz: accel returns the   */
+                   /* virial part (MeV), newton the full tensor / volume     */
+} yk_meas;
+
+/* Accumulated device timings, filled when YK_FLAG_TIMING is set. */
+typedef struct yk_timing {
+    double force_ms;        /* sum of force-kernel durations (HIP events)  */
+    int64_t force_launches; /* number of force-kernel launches timed       */
+    double step_ms;         /* sum of whole-step durations (HIP events)    */
+    int64_t steps;          /* number of steps timed                       */
+    int64_t pairs_swept;    /* directed pairs evaluated by timed launches  */
+} yk_timing;
+
+/* Library -------------------------------------------------------------- */
+int32_t yk_abi_version(void);
+const char *yk_last_error(void);
+
+/* Create a system of n particles on the current HIP device.  zii/aii are the
+ * per-particle charge and mass numbers (src/md_globals.f:158-159); they may be
+ * NULL for YK_PURE_ION, which uses params->zi/ai.  Replaces the allocation
+ * and parameter setup of input1/initconfig (src/input1.F:394-399,
+ * src/initconfig.f:395-399) for the hot path. */
+int yk_create(const yk_params *params, int32_t n, const double *zii,
+              const double *aii, yk_system **out);
+void yk_destroy(yk_system *sys);
+
+/* Replace the run parameters (box strain, screening, cutoff, flags); n,
+ * sim_type and precision stay fixed.  Replaces direct writes of md_globals
+ * such as strain's xl update (src/strain.f:19-82). */
+int yk_set_params(yk_system *sys, const yk_params *params);
+
+/* Upload positions (and velocities; v may be NULL for zero) in (3,n)
+ * layout.  Positions are used as given; the integrator wraps them into
+ * [0, xl] like src/vel_verlet_2.f:39-40.  Returns YK_ERR_NONFINITE if any
+ * coordinate is NaN/Inf. */
+int yk_put_state(yk_system *sys, const double *x, const double *v);
+
+/* Download positions, velocities, accelerations in (3,n) layout; any
+ * pointer may be NULL.  Host readers of particle state (ttot, cancel_vcom,
+ * checkpoint: src/gen_utils.f, src/md_utils.f) call this first. */
+int yk_get_state(yk_system *sys, double *x, double *v, double *a);
+
+/* accel(do_measurements): accelerations of the current positions, and when
+ * do_measurements != 0 also ev, vir, px and the virial pressure tensor.
+ * out may be NULL.  src/int_control.f:52-81. */
+int yk_accel(yk_system *sys, int32_t do_measurements, yk_meas *out);
+
+/* newton(do_measurements): one velocity-Verlet step of length dt.  Needs
+ * accelerations of the current positions (a previous yk_accel/yk_newton).
+ * src/vel_verlet_2.f:18-120 (src/vel_verlet_1.f with YK_FLAG_KDK). */
+int yk_newton(yk_system *sys, double dt, int32_t do_measurements, yk_meas *out);
+
+/* nsteps newton steps with state kept on the device.  Step s (1-based)
+ * measures when nind > 0 and s % nind == 0 -- the md.F cadence
+ * do_measurements = mdmod(istep+1,nind).eq.0 (src/md.F:129) -- and writes
+ * its record to records[s/nind - 1] (records may be NULL; it must hold
+ * nsteps/nind entries otherwise).  One host synchronisation at the end. */
+int yk_run(yk_system *sys, double dt, int32_t nsteps, int32_t nind,
+           yk_meas *records);
+
+/* Ttot: kinetic energy per particle of the current velocities,
+ * (0.5*xmass*sum A v^2)/n, src/gen_utils.f:40-55. */
+int yk_ttot(yk_system *sys, double *eka);
+
+int yk_get_timing(yk_system *sys, yk_timing *out);
+int yk_reset_timing(yk_system *sys);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YK_H */

@@ -1,0 +1,233 @@
+"""Oracle for the yukawamd hot path -- TEST INFRASTRUCTURE ONLY.
+
+Two checkers, both CPU:
+  * ``liboracle_yk.so`` (oracle/yk_oracle.c): a C restatement of accel_ion_pure,
+    accel_ion_mix, accel_nn and newton (vel_verlet_2) with the reference's loop
+    nest and evaluation order, plus a long-double "truth" evaluation;
+  * ``_ref/libykref.so``: the reference's own Fortran compiled from its sources
+    with oracle/ref_harness.f90 (built only where the reference is present).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+package.  The product (yukawamd_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_SO = HERE / "_build" / "liboracle_yk.so"
+REF_SO = HERE / "_ref" / "libykref.so"
+
+KINDS = {"pure-ion": 0, "ion-mixture": 1, "nucleon": 2, "B-field": 3}
+NMEAS = 10
+MEAS_KEYS = ("ev", "vir", "px", "pp11", "pp12", "pp13", "pp22", "pp23", "pp33", "ek")
+
+
+class YkoParams(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int), ("coulomb_on", C.c_int), ("nuclear_on", C.c_int),
+        ("ref_f1", C.c_int), ("xl", C.c_double * 3),
+        ("xmuc", C.c_double), ("rccut", C.c_double), ("frp", C.c_double),
+        ("vc", C.c_double), ("xmass", C.c_double), ("rho", C.c_double), ("kT", C.c_double),
+        ("zi", C.c_double), ("ai", C.c_double), ("aa", C.c_double), ("bb", C.c_double),
+        ("cc", C.c_double), ("xpacket", C.c_double), ("rncut", C.c_double),
+        ("bfield", C.c_double),
+    ]
+
+
+_D = C.POINTER(C.c_double)
+_lib = None
+_ref = None
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not ORACLE_SO.exists():
+            raise FileNotFoundError(f"{ORACLE_SO} not built (make -C oracle)")
+        lib = C.CDLL(str(ORACLE_SO))
+        P = C.POINTER(YkoParams)
+        lib.yko_accel.argtypes = [P, C.c_int, _D, _D, _D, C.c_int, _D, _D]
+        lib.yko_newton.argtypes = [P, C.c_int, C.c_double, _D, _D, _D, _D, _D, _D, C.c_int, _D]
+        lib.yko_newton_kdk.argtypes = [P, C.c_int, C.c_double, _D, _D, _D, _D, _D, C.c_int, _D]
+        lib.yko_ttot.argtypes = [P, C.c_int, _D, _D]
+        lib.yko_ttot.restype = C.c_double
+        lib.yko_accel_truth.argtypes = [P, C.c_int, _D, _D, _D, _D]
+        lib.yko_accel_omp.argtypes = [P, C.c_int, _D, _D, _D, _D, C.c_int]
+        _lib = lib
+    return _lib
+
+
+def ref_available() -> bool:
+    return REF_SO.exists()
+
+
+def load_ref():
+    global _ref
+    if _ref is None:
+        if not REF_SO.exists():
+            raise FileNotFoundError(f"{REF_SO} not built (make -C oracle ref)")
+        lib = C.CDLL(str(REF_SO))
+        lib.ykref_setup.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _D, _D, _D, _D, _D]
+        lib.ykref_setup.restype = C.c_int
+        lib.ykref_accel.argtypes = [C.c_int, _D, _D]
+        lib.ykref_newton.argtypes = [C.c_int, C.c_double, C.c_int, _D]
+        lib.ykref_set_accel.argtypes = [_D]
+        lib.ykref_get.argtypes = [_D, _D, _D]
+        lib.ykref_ttot.restype = C.c_double
+        lib.ykref_timers.argtypes = [_D]
+        _ref = lib
+    return _ref
+
+
+def params_struct(p, ref_f1=None) -> YkoParams:
+    s = YkoParams()
+    s.kind = KINDS[p.sim_type]
+    s.coulomb_on = 1 if p.coulomb_on else 0
+    s.nuclear_on = 1 if p.nuclear_on else 0
+    s.ref_f1 = int(p.ref_compat_f1 if ref_f1 is None else ref_f1)
+    for k in range(3):
+        s.xl[k] = p.xl[k]
+    for name in ("xmuc", "rccut", "frp", "vc", "xmass", "rho", "kT", "zi", "ai", "aa", "bb",
+                 "cc", "xpacket", "rncut", "bfield"):
+        setattr(s, name, float(getattr(p, name)))
+    return s
+
+
+def _arr(a, shape):
+    out = np.ascontiguousarray(a, dtype=np.float64)
+    assert out.shape == shape, (out.shape, shape)
+    return out
+
+
+def _meas(m):
+    return dict(zip(MEAS_KEYS, (float(v) for v in m)))
+
+
+def accel(p, x, Z, A, do_meas=True, ref_f1=None):
+    """accel(do_measurements) of the restatement: (a (n,3), meas dict)."""
+    n = x.shape[0]
+    x = _arr(x, (n, 3))
+    Z = _arr(Z, (n,))
+    A = _arr(A, (n,))
+    a = np.zeros((n + 3, 3))
+    m = np.zeros(NMEAS)
+    ps = params_struct(p, ref_f1)
+    rc = load().yko_accel(C.byref(ps), n, _ptr(x), _ptr(Z), _ptr(A), int(do_meas), _ptr(a),
+                          _ptr(m))
+    if rc != 0:
+        raise RuntimeError(f"yko_accel failed ({rc})")
+    return a[:n].copy(), _meas(m)
+
+
+def newton(p, dt, x, v, vold, a, Z, A, do_meas=False, ref_f1=None, kdk=False):
+    """One step on copies of the state; returns (x, v, vold, a, meas)."""
+    n = x.shape[0]
+    x = _arr(x, (n, 3)).copy()
+    v = _arr(v, (n, 3)).copy()
+    vold = _arr(vold, (n, 3)).copy()
+    ab = np.zeros((n + 3, 3))
+    ab[:n] = a
+    Z = _arr(Z, (n,))
+    A = _arr(A, (n,))
+    m = np.zeros(NMEAS)
+    ps = params_struct(p, ref_f1)
+    lib = load()
+    if kdk:
+        rc = lib.yko_newton_kdk(C.byref(ps), n, dt, _ptr(x), _ptr(v), _ptr(ab), _ptr(Z), _ptr(A),
+                                int(do_meas), _ptr(m))
+    else:
+        rc = lib.yko_newton(C.byref(ps), n, dt, _ptr(x), _ptr(v), _ptr(vold), _ptr(ab), _ptr(Z),
+                            _ptr(A), int(do_meas), _ptr(m))
+    if rc != 0:
+        raise RuntimeError(f"yko_newton failed ({rc})")
+    return x, v, vold, ab[:n].copy(), _meas(m)
+
+
+def ttot(p, v, A):
+    n = v.shape[0]
+    ps = params_struct(p)
+    return load().yko_ttot(C.byref(ps), n, _ptr(_arr(v, (n, 3))), _ptr(_arr(A, (n,))))
+
+
+def truth(p, x, Z):
+    """Long-double full-matrix evaluation: (force (n,3) MeV/fm, ev_total, vir, pp[6])."""
+    n = x.shape[0]
+    x = _arr(x, (n, 3))
+    Z = _arr(Z, (n,))
+    f = np.zeros((n, 3))
+    out = np.zeros(8)
+    ps = params_struct(p, 0)
+    load().yko_accel_truth(C.byref(ps), n, _ptr(x), _ptr(Z), _ptr(f), _ptr(out))
+    return f, float(out[0]), float(out[1]), out[2:].copy()
+
+
+def accel_omp(p, x, Z, A, nthreads=0):
+    """OpenMP restatement of the force-only branch (CPU timing); returns (a, threads)."""
+    n = x.shape[0]
+    a = np.zeros((n, 3))
+    ps = params_struct(p, 0)
+    used = load().yko_accel_omp(C.byref(ps), n, _ptr(_arr(x, (n, 3))), _ptr(_arr(Z, (n,))),
+                                _ptr(_arr(A, (n,))), _ptr(a), int(nthreads))
+    return a, used
+
+
+# ---------------------------------------------------------------------------
+# The reference's own Fortran (oracle/_ref/libykref.so)
+def _ref_prm(p):
+    return np.array([*p.xl, p.xmuc, p.rccut, p.frp, p.xmass, p.rho, p.kT, p.zi, p.ai,
+                     p.aa, p.bb, p.cc, p.xpacket, p.rncut, p.bfield], dtype=np.float64)
+
+
+def ref_setup(p, x, v, Z, A):
+    n = x.shape[0]
+    prm = _ref_prm(p)
+    x = _arr(x, (n, 3))
+    v = _arr(v, (n, 3)) if v is not None else np.zeros((n, 3))
+    rc = load_ref().ykref_setup(KINDS[p.sim_type], n, int(p.coulomb_on), int(p.nuclear_on),
+                                _ptr(prm), _ptr(x), _ptr(v), _ptr(_arr(Z, (n,))),
+                                _ptr(_arr(A, (n,))))
+    if rc != 0:
+        raise RuntimeError("ykref_setup failed")
+
+
+def ref_accel(p, x, Z, A, do_meas=True):
+    """The reference's accel(do_measurements) on these positions."""
+    n = x.shape[0]
+    ref_setup(p, x, None, Z, A)
+    a = np.zeros((n + 3, 3))
+    m = np.zeros(NMEAS)
+    load_ref().ykref_accel(int(do_meas), _ptr(a), _ptr(m))
+    return a[:n].copy(), _meas(m)
+
+
+def ref_newton(p, dt, x, v, a, Z, A, nsteps=1, meas_last=True):
+    """nsteps of the reference's newton (vel_verlet_2) from (x, v, a)."""
+    n = x.shape[0]
+    ref_setup(p, x, v, Z, A)
+    lib = load_ref()
+    a = _arr(a, (n, 3))
+    lib.ykref_set_accel(_ptr(a))
+    m = np.zeros(NMEAS)
+    lib.ykref_newton(int(nsteps), float(dt), int(meas_last), _ptr(m))
+    xo, vo, ao = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+    lib.ykref_get(_ptr(xo), _ptr(vo), _ptr(ao))
+    return xo, vo, ao, _meas(m)
+
+
+def ref_timers():
+    t = np.zeros(6)
+    load_ref().ykref_timers(_ptr(t))
+    return {"t_calc_a": t[0], "n_calc_a": int(t[1]), "t_accel": t[2], "n_accel": int(t[3]),
+            "t_newton": t[4], "n_newton": int(t[5])}
+
+
+def ref_reset_timers():
+    load_ref().ykref_reset_timers()

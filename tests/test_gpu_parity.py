@@ -1,0 +1,207 @@
+"""HIP path against the oracle and the reference's golden vectors (needs a GPU).
+
+Bars (stated per test):
+  FP64 forces    condition-aware |dF_i| / sum_j |F_ij| <= 1e-12, and the plain
+                 per-particle |dF_i|/|F_i| is reported and bounded by 1e-10
+                 (summation order differs from the reference, so a particle
+                 whose force nearly cancels loses relative digits in both);
+  FP64 energy, pressure, pressure-tensor diagonal: 1e-13 relative;
+  mixed form     max |dF_i|/|F_i| and drift reported; bounded in the test.
+All calls go through the C ABI (libyukawa_hip.so).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from _golden import NAMES, load
+from yukawamd_amd import workloads as W
+from yukawamd_amd.system import YukawaSystem
+
+pytestmark = pytest.mark.gpu
+
+
+def masses(p, A):
+    return (A if p.sim_type != "nucleon" else np.ones_like(A)) * p.xmass
+
+
+def abs_pair_force_sum(p, x, Z, A):
+    """sum_j |F_ij| per particle (numpy, FP64), the conditioning of F_i."""
+    n = x.shape[0]
+    L = np.array(p.xl)
+    out = np.zeros(n)
+    for i0 in range(0, n, 256):
+        xi = x[i0:i0 + 256, None, :]
+        d = xi - x[None, :, :]
+        d = np.where(d > 0.5 * L, d - L, d)
+        d = np.where(d < -0.5 * L, d + L, d)
+        r2 = (d * d).sum(-1)
+        idx = np.arange(i0, min(i0 + 256, n))
+        r2[np.arange(len(idx)), idx] = np.inf
+        r = np.sqrt(r2)
+        if p.sim_type == "nucleon":
+            g = np.exp(-r2 / (2 * p.xpacket))
+            same = Z[i0:i0 + 256, None] == Z[None, :]
+            bc = np.where(same, p.bb + p.cc, p.bb - p.cc)
+            s = np.where(r2 <= p.rncut ** 2, np.abs((2 * p.aa * g + bc) * g / p.xpacket), 0)
+            pp = (Z[i0:i0 + 256, None] == 1) & (Z[None, :] == 1) & (r2 <= p.rccut ** 2)
+            s += np.where(pp, p.vc * (1 + p.xmuc * r) * np.exp(-p.xmuc * r) / (r * r2), 0)
+        else:
+            zz = Z[i0:i0 + 256, None] * Z[None, :] if p.sim_type == "ion-mixture" else p.zi ** 2
+            s = np.where(r2 <= p.rccut ** 2,
+                         p.frp * p.vc * zz * (1 + p.xmuc * r) * np.exp(-p.xmuc * r) / (r * r2), 0)
+        out[i0:i0 + 256] = (s * r).sum(-1)
+    return out
+
+
+def gpu_accel(p, x, Z, A, do_meas=True):
+    with YukawaSystem(p, Z=Z, A=A) as s:
+        s.put_state(x)
+        m = s.accel(do_meas)
+        return s.a, m
+
+
+def force_errors(p, a_gpu, a_ref, x, Z, A):
+    m = masses(p, A)[:, None]
+    dF = np.linalg.norm((a_gpu - a_ref) * m, axis=1)
+    F = np.linalg.norm(a_ref * m, axis=1)
+    cond = abs_pair_force_sum(p, x, Z, A)
+    ok = cond > 0
+    return (dF[ok] / F[ok]).max(), (dF[ok] / cond[ok]).max()
+
+
+def check_meas(m_gpu, m_ref, rtol=1e-13):
+    assert m_gpu["ev"] == pytest.approx(m_ref["ev"], rel=rtol)
+    assert m_gpu["px"] == pytest.approx(m_ref["px"], rel=rtol)
+    diag = max(abs(m_ref["pp11"]), abs(m_ref["pp22"]), abs(m_ref["pp33"]))
+    for k_gpu, k_ref in zip(range(6), ("pp11", "pp12", "pp13", "pp22", "pp23", "pp33")):
+        tol = rtol * abs(m_ref[k_ref]) if k_ref in ("pp11", "pp22", "pp33") else rtol * diag
+        assert abs(m_gpu["pp"][k_gpu] - m_ref[k_ref]) <= tol, k_ref
+
+
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("name", NAMES)
+def test_golden_fp64_accel(name):
+    p, g = load(name)
+    p = p.replace(ref_compat_f1=True)
+    for dm in (1, 0):
+        a, m = gpu_accel(p, g["x"], g["Z"], g["A"], bool(dm))
+        rel, cond = force_errors(p, a, g[f"a_meas{dm}"], g["x"], g["Z"], g["A"])
+        print(f"{name} meas={dm}: max rel {rel:.2e} cond {cond:.2e}")
+        assert cond <= 1e-12 and rel <= 1e-10
+        if dm:
+            check_meas(m, dict(zip(oracle.MEAS_KEYS, g["m_meas1"])))
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_golden_fp64_three_steps(name):
+    p, g = load(name)
+    p = p.replace(ref_compat_f1=True)
+    with YukawaSystem(p, Z=g["Z"], A=g["A"], dt=float(g["dt"])) as s:
+        s.put_state(g["x"], g["v"])
+        s.accel(True)
+        s.newton(False)
+        s.newton(False)
+        m = s.newton(True)
+        x, v, a = s.get_state()
+    L = p.xl[0]
+    dx = np.abs(x - g["x3"])
+    dx = np.minimum(dx, L - dx)
+    assert dx.max() <= 1e-12 * L
+    assert np.abs(v - g["v3"]).max() <= 1e-11 * np.abs(g["v3"]).max()
+    mref = dict(zip(oracle.MEAS_KEYS, g["m3"]))
+    assert m["ek"] == pytest.approx(mref["ek"], rel=1e-12)
+    assert m["ev"] == pytest.approx(mref["ev"], rel=1e-12)
+
+
+@pytest.mark.parametrize("make", [
+    lambda: W.pure_ion(n=3456, seed=1000),          # the pure-ion workload at full size
+    lambda: W.ion_mixture(n=2000, seed=51),
+    lambda: W.nucleon(n=4000, seed=52),
+    lambda: W.ion_mixture(n=1024, seed=53, kappa=2.0),
+])
+def test_fp64_against_oracle(make):
+    w = make()
+    p = w.params
+    for dm in (True, False):
+        a_o, m_o = oracle.accel(p, w.x, w.Z, w.A, dm, ref_f1=0)
+        a_g, m_g = gpu_accel(p, w.x, w.Z, w.A, dm)
+        rel, cond = force_errors(p, a_g, a_o, w.x, w.Z, w.A)
+        print(f"{w.name} meas={dm}: max rel {rel:.2e} cond {cond:.2e}")
+        assert cond <= 1e-12 and rel <= 1e-10
+        if dm:
+            check_meas(m_g, m_o)
+
+
+@pytest.mark.parametrize("make", [lambda: W.ion_mixture(n=3000, seed=61),
+                                  lambda: W.pure_ion(n=3456, seed=1000),
+                                  lambda: W.nucleon(n=6000, seed=62)])
+def test_mixed_precision_error(make):
+    w = make()
+    p = w.params
+    a_o, m_o = oracle.accel(p, w.x, w.Z, w.A, True, ref_f1=0)
+    a_g, m_g = gpu_accel(p.replace(precision="mixed"), w.x, w.Z, w.A, True)
+    rel, cond = force_errors(p, a_g, a_o, w.x, w.Z, w.A)
+    print(f"{w.name} mixed: max rel {rel:.2e} cond {cond:.2e} "
+          f"ev rel {abs(m_g['ev'] / m_o['ev'] - 1):.2e}")
+    assert cond <= 1e-6 and rel <= 1e-4
+    assert m_g["ev"] == pytest.approx(m_o["ev"], rel=1e-6)
+
+
+def test_ion_mixture_27648_properties():
+    """Full-size mixture: Newton's 3rd law, determinism, run == repeated newton."""
+    w = W.ion_mixture()
+    p = w.params
+    with YukawaSystem(p, Z=w.Z, A=w.A, dt=w.dt) as s:
+        s.put_state(w.x, w.v)
+        m1 = s.accel(True)
+        a1 = s.a
+        m2 = s.accel(True)
+        a2 = s.a
+        assert np.array_equal(a1, a2) and m1 == m2            # bitwise deterministic
+        F = a1 * masses(p, w.A)[:, None]
+        assert np.abs(F.sum(0)).max() <= 1e-12 * np.abs(F).sum()
+        for _ in range(4):
+            s.newton(False)
+        xa, va, aa = s.get_state()
+    with YukawaSystem(p, Z=w.Z, A=w.A, dt=w.dt) as s:
+        s.put_state(w.x, w.v)
+        s.accel(True)
+        recs = s.run(4, nind=2)
+        xb, vb, ab = s.get_state()
+    assert len(recs) == 2
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb) and np.array_equal(aa, ab)
+
+
+def test_kdk_against_oracle():
+    w = W.ion_mixture(n=700, seed=71)
+    p = w.params.replace(integrator="kdk")
+    a, _ = oracle.accel(p, w.x, w.Z, w.A, True, ref_f1=0)
+    x, v, vold = w.x, w.v, np.zeros_like(w.v)
+    for st in range(3):
+        x, v, vold, a, m = oracle.newton(p, w.dt, x, v, vold, a, w.Z, w.A, st == 2, kdk=True)
+    with YukawaSystem(p, Z=w.Z, A=w.A, dt=w.dt) as s:
+        s.put_state(w.x, w.v)
+        s.accel(True)
+        s.newton(False)
+        s.newton(False)
+        mg = s.newton(True)
+        xg, vg, ag = s.get_state()
+    assert np.abs(vg - v).max() <= 1e-11 * np.abs(v).max()
+    assert mg["ek"] == pytest.approx(m["ek"], rel=1e-12)
+
+
+def test_error_paths():
+    from yukawamd_amd._lib import YkError
+    w = W.ion_mixture(n=64, seed=81)
+    with YukawaSystem(w.params, Z=w.Z, A=w.A, dt=w.dt) as s:
+        bad = w.x.copy()
+        bad[3, 1] = np.nan
+        with pytest.raises(YkError) as e:
+            s.put_state(bad)
+        assert e.value.code == 5
+        s.put_state(w.x, w.v)
+        with pytest.raises(YkError) as e:
+            s.newton(False)
+        assert e.value.code == 4
+        s.accel(False)
+        s.newton(False)

@@ -1,0 +1,65 @@
+"""Build recipe for the native parts of yukawamd_amd (in-tree, gfx950).
+
+    python -m yukawamd_amd.build          # library only
+    python -m yukawamd_amd.build --all    # library, Fortran binding, oracle
+
+The HIP library is compiled with hipcc for gfx950 only.  Packed FP32 VALU ops
+are disabled (-packed-fp32-ops) and FP contraction is off: the kernels write
+every fused multiply-add they want explicitly, and the integrator keeps the
+reference's separate multiply and add roundings.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+LIB = PKG / "libyukawa_hip.so"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(str(c) for c in cmd), flush=True)
+    subprocess.run([str(c) for c in cmd], check=True, cwd=cwd)
+
+
+def build_library(verbose: bool = True) -> Path:
+    srcs = [CSRC / "yk_lib.hip"]
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, *HIP_FLAGS, "-o", tmp, *srcs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> None:
+    _run(["make", "-s", "-C", ROOT / "oracle"])
+    if Path("/root/reference/src").is_dir():
+        _run(["make", "-s", "-C", ROOT / "oracle", "ref"])
+
+
+def main(argv=None) -> int:
+    argv = sys.argv[1:] if argv is None else argv
+    build_library()
+    if "--all" in argv:
+        build_oracle()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -1,0 +1,672 @@
+// yk_lib.hip -- C-ABI host side of libyukawa_hip.so (declared in include/yk.h).
+//
+// One yk_system owns one particle system resident in HBM.  yk_accel replaces
+// accel(do_measurements) (src/int_control.f:52-81) and yk_newton replaces
+// newton(do_measurements) (src/vel_verlet_2.f:18-120).  The host-side
+// transposes between the reference's Fortran (3,n) arrays and the device
+// structure-of-arrays happen only in yk_put_state / yk_get_state.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/yk.h"
+#include "yk_kernels.hpp"
+
+using namespace yk;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define YK_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(YK_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                           \
+    } while (0)
+
+}  // namespace
+
+struct yk_system {
+    yk_params p{};
+    int n = 0;
+    int device = 0;
+    std::vector<double> zhost;     // charge numbers, kept for position uploads
+    hipStream_t stream = nullptr;
+    // device state
+    double4 *pos = nullptr;
+    double *vel = nullptr, *vold = nullptr, *acc = nullptr;
+    double *aii = nullptr, *den = nullptr;
+    double *part = nullptr;        // [nchunks][11][n]
+    double *blockpart = nullptr;   // [nblocks][NQ]
+    MeasOut *d_meas = nullptr;     // device scalar records
+    int meas_cap = 0;
+    // host staging (pinned)
+    double *h_stage = nullptr;     // 4n doubles
+    MeasOut *h_meas = nullptr;
+    int h_meas_cap = 0;
+    // decomposition
+    int nblocks = 0, ntiles = 0, nchunks = 0, tiles_per_chunk = 0;
+    bool have_accel = false;
+    // timing
+    std::vector<hipEvent_t> ev;    // pairs of (start, stop) for force launches
+    size_t ev_used = 0;
+    hipEvent_t step0 = nullptr, step1 = nullptr;
+    yk_timing timing{};
+};
+
+namespace {
+
+bool finite3(const double *x, int n)
+{
+    for (size_t k = 0; k < (size_t)3 * n; ++k)
+        if (!std::isfinite(x[k])) return false;
+    return true;
+}
+
+int check_params(const yk_params *p)
+{
+    if (!p) return fail(YK_ERR_ARG, "params is NULL");
+    if (p->sim_type < YK_PURE_ION || p->sim_type > YK_BFIELD_TEST)
+        return fail(YK_ERR_ARG, "sim_type=%d: program not compiled to do this simulation type",
+                    p->sim_type);
+    if (p->precision != YK_FP64 && p->precision != YK_MIXED)
+        return fail(YK_ERR_ARG, "precision=%d is not YK_FP64 or YK_MIXED", p->precision);
+    for (int k = 0; k < 3; ++k)
+        if (!(p->xl[k] > 0.0) || !std::isfinite(p->xl[k]))
+            return fail(YK_ERR_ARG, "box edge xl(%d)=%g must be positive", k + 1, p->xl[k]);
+    if (!(p->rccut > 0.0)) return fail(YK_ERR_ARG, "rccut=%g must be positive", p->rccut);
+    if (p->xmuc < 0.0 || !std::isfinite(p->xmuc))
+        return fail(YK_ERR_ARG, "xmuc=%g must be finite and >= 0", p->xmuc);
+    if (!(p->xmass > 0.0)) return fail(YK_ERR_ARG, "xmass=%g must be positive", p->xmass);
+    if (p->sim_type == YK_PURE_ION && !(p->ai > 0.0))
+        return fail(YK_ERR_ARG, "pure-ion needs ai > 0 (got %g)", p->ai);
+    if (p->sim_type == YK_NUCLEON && p->nuclear_on && !(p->xpacket > 0.0))
+        return fail(YK_ERR_ARG, "xpacket=%g must be positive", p->xpacket);
+    return YK_OK;
+}
+
+// evcut switch: src/int_ion_mix.f:140-145, src/int_nn.f:176-181.
+double evcut_of(const yk_params &p)
+{
+    double hd = std::sqrt(p.xl[0] * p.xl[0] + p.xl[1] * p.xl[1] + p.xl[2] * p.xl[2]) / 2.0;
+    if (!(p.rccut <= hd)) return 0.0;
+    double e = std::exp(-(p.xmuc * p.rccut)) / p.rccut;
+    return (p.sim_type == YK_NUCLEON) ? p.vc * std::exp(-(p.xmuc * p.rccut)) / p.rccut : e;
+}
+
+ForceConsts force_consts(const yk_system *s)
+{
+    const yk_params &p = s->p;
+    ForceConsts c{};
+    for (int k = 0; k < 3; ++k) {
+        c.L[k] = p.xl[k];
+        c.invL[k] = 1.0 / p.xl[k];
+    }
+    c.rc2 = p.rccut * p.rccut;
+    c.rn2 = p.rncut * p.rncut;
+    c.mu = p.xmuc;
+    c.evcut = evcut_of(p);
+    c.xp2i = (p.xpacket > 0) ? 1.0 / (2.0 * p.xpacket) : 0.0;
+    c.xpi = (p.xpacket > 0) ? 1.0 / p.xpacket : 0.0;
+    c.aa = p.aa;
+    c.aa2 = 2.0 * p.aa;
+    c.bpc = p.bb + p.cc;
+    c.bmc = p.bb - p.cc;
+    c.vc = p.vc;
+    const double LOG2E = 1.4426950408889634074;
+    c.muf = (float)p.xmuc;
+    c.mul2ef = (float)(p.xmuc * LOG2E);
+    c.xp2il2ef = (float)(c.xp2i * LOG2E);
+    c.xpif = (float)c.xpi;
+    c.aaf = (float)c.aa;
+    c.aa2f = (float)c.aa2;
+    c.bpcf = (float)c.bpc;
+    c.bmcf = (float)c.bmc;
+    c.vcf = (float)p.vc;
+    c.n = s->n;
+    c.coul_on = p.coulomb_on ? 1 : 0;
+    c.nuc_on = p.nuclear_on ? 1 : 0;
+    return c;
+}
+
+StepConsts step_consts(const yk_system *s, double dt)
+{
+    const yk_params &p = s->p;
+    StepConsts c{};
+    const bool kdk = (p.flags & YK_FLAG_KDK) != 0;
+    c.dt = dt;
+    c.afac = kdk ? 0.5 * dt * dt : 0.5 * (dt * dt);   // vel_verlet_1.f:28 / vel_verlet_2.f:33
+    c.hdt = 0.5 * dt;
+    c.bfield_c = 3.218404e-20 * p.bfield;
+    c.has_bfield = (p.bfield != 0.0) ? 1 : 0;
+    for (int k = 0; k < 3; ++k) c.L[k] = p.xl[k];
+    switch (p.sim_type) {
+    case YK_PURE_ION: c.cnum = p.frp * p.zi * p.zi * p.vc; break;   // int_ion_pure.f:229
+    case YK_ION_MIXTURE: c.cnum = p.frp * p.vc; break;              // int_ion_mix.f:275
+    default: c.cnum = 1.0; break;                                    // int_nn.f:320
+    }
+    c.n = s->n;
+    c.ref_f1 = (p.flags & YK_FLAG_REF_F1) ? 1 : 0;
+    c.kind = p.sim_type;
+    return c;
+}
+
+MeasConsts meas_consts(const yk_system *s, int mode)
+{
+    const yk_params &p = s->p;
+    StepConsts sc = step_consts(s, 0.0);
+    MeasConsts m{};
+    m.cnum = sc.cnum;
+    m.rho = p.rho;
+    m.kT = p.kT;
+    m.xmass = p.xmass;
+    m.vol = p.xl[0] * p.xl[1] * p.xl[2];
+    m.three_n = 3.0f * (float)s->n;
+    m.n_f = (float)s->n;
+    m.mode = mode;
+    return m;
+}
+
+void choose_decomposition(yk_system *s)
+{
+    const int n = s->n;
+    s->nblocks = (n + BLOCK - 1) / BLOCK;
+    s->ntiles = (n + TILE - 1) / TILE;
+    // enough workgroups to fill 256 CUs several times over
+    int want = (2048 + s->nblocks - 1) / s->nblocks;
+    if (want < 1) want = 1;
+    if (want > s->ntiles) want = s->ntiles;
+    s->tiles_per_chunk = (s->ntiles + want - 1) / want;
+    s->nchunks = (s->ntiles + s->tiles_per_chunk - 1) / s->tiles_per_chunk;
+}
+
+int ensure_meas_cap(yk_system *s, int cap)
+{
+    if (cap <= s->meas_cap) return YK_OK;
+    if (s->d_meas) YK_HIP(hipFree(s->d_meas));
+    s->d_meas = nullptr;
+    YK_HIP(hipMalloc(&s->d_meas, sizeof(MeasOut) * (size_t)cap));
+    s->meas_cap = cap;
+    if (cap > s->h_meas_cap) {
+        if (s->h_meas) YK_HIP(hipHostFree(s->h_meas));
+        s->h_meas = nullptr;
+        YK_HIP(hipHostMalloc(&s->h_meas, sizeof(MeasOut) * (size_t)cap, hipHostMallocDefault));
+        s->h_meas_cap = cap;
+    }
+    return YK_OK;
+}
+
+int timing_begin(yk_system *s, hipEvent_t *stop_out)
+{
+    *stop_out = nullptr;
+    if (!(s->p.flags & YK_FLAG_TIMING)) return YK_OK;
+    if (s->ev_used + 2 > s->ev.size()) {
+        for (int k = 0; k < 64; ++k) {
+            hipEvent_t e;
+            YK_HIP(hipEventCreate(&e));
+            s->ev.push_back(e);
+        }
+    }
+    hipEvent_t a = s->ev[s->ev_used], b = s->ev[s->ev_used + 1];
+    s->ev_used += 2;
+    YK_HIP(hipEventRecord(a, s->stream));
+    *stop_out = b;
+    return YK_OK;
+}
+
+int timing_collect(yk_system *s)
+{
+    if (s->ev_used == 0) return YK_OK;
+    YK_HIP(hipEventSynchronize(s->ev[s->ev_used - 1]));
+    for (size_t k = 0; k < s->ev_used; k += 2) {
+        float ms = 0.f;
+        YK_HIP(hipEventElapsedTime(&ms, s->ev[k], s->ev[k + 1]));
+        s->timing.force_ms += ms;
+        s->timing.force_launches += 1;
+        s->timing.pairs_swept += (int64_t)s->n * (int64_t)(s->n - 1);
+    }
+    s->ev_used = 0;
+    return YK_OK;
+}
+
+template <int KIND, int PREC>
+void launch_pairs_k(yk_system *s, const ForceConsts &c, bool meas)
+{
+    dim3 grid(s->nblocks, s->nchunks);
+    if (meas)
+        hipLaunchKernelGGL((k_pairs<KIND, PREC, true>), grid, dim3(BLOCK), 0, s->stream,
+                           s->pos, c, s->tiles_per_chunk, s->part);
+    else
+        hipLaunchKernelGGL((k_pairs<KIND, PREC, false>), grid, dim3(BLOCK), 0, s->stream,
+                           s->pos, c, s->tiles_per_chunk, s->part);
+}
+
+int launch_pairs(yk_system *s, bool meas)
+{
+    if (s->n < 2 || s->p.sim_type == YK_BFIELD_TEST) {
+        YK_HIP(hipMemsetAsync(s->part, 0, sizeof(double) * 11 * (size_t)s->n * s->nchunks,
+                              s->stream));
+        return YK_OK;
+    }
+    ForceConsts c = force_consts(s);
+    hipEvent_t stop = nullptr;
+    int rc = timing_begin(s, &stop);
+    if (rc) return rc;
+    const bool mixed = (s->p.precision == YK_MIXED);
+    switch (s->p.sim_type) {
+    case YK_PURE_ION:
+        mixed ? launch_pairs_k<PURE, 1>(s, c, meas) : launch_pairs_k<PURE, 0>(s, c, meas);
+        break;
+    case YK_ION_MIXTURE:
+        mixed ? launch_pairs_k<MIX, 1>(s, c, meas) : launch_pairs_k<MIX, 0>(s, c, meas);
+        break;
+    case YK_NUCLEON:
+        mixed ? launch_pairs_k<NN, 1>(s, c, meas) : launch_pairs_k<NN, 0>(s, c, meas);
+        break;
+    default: return fail(YK_ERR_UNSUPPORTED, "sim_type %d", s->p.sim_type);
+    }
+    YK_HIP(hipGetLastError());
+    if (stop) YK_HIP(hipEventRecord(stop, s->stream));
+    if (s->ev_used >= 1024) return timing_collect(s);
+    return YK_OK;
+}
+
+template <int KIND, bool MEAS, int MODE>
+void launch_fin_k(yk_system *s, const StepConsts &sc)
+{
+    hipLaunchKernelGGL((k_finalize<KIND, MEAS, MODE>), dim3(s->nblocks), dim3(BLOCK), 0,
+                       s->stream, s->part, s->nchunks, s->pos, s->den, s->aii, s->acc,
+                       s->vel, s->vold, sc, s->blockpart);
+}
+
+template <int KIND>
+void launch_fin_kind(yk_system *s, const StepConsts &sc, bool meas, int mode)
+{
+    if (meas) {
+        if (mode == 0) launch_fin_k<KIND, true, 0>(s, sc);
+        else if (mode == 1) launch_fin_k<KIND, true, 1>(s, sc);
+        else launch_fin_k<KIND, true, 2>(s, sc);
+    } else {
+        if (mode == 0) launch_fin_k<KIND, false, 0>(s, sc);
+        else if (mode == 1) launch_fin_k<KIND, false, 1>(s, sc);
+        else launch_fin_k<KIND, false, 2>(s, sc);
+    }
+}
+
+// finalize + optional measurement reduction into d_meas[slot]
+int launch_finalize(yk_system *s, double dt, bool meas, int mode, int slot)
+{
+    StepConsts sc = step_consts(s, dt);
+    switch (s->p.sim_type) {
+    case YK_PURE_ION: launch_fin_kind<PURE>(s, sc, meas, mode); break;
+    case YK_ION_MIXTURE: launch_fin_kind<MIX>(s, sc, meas, mode); break;
+    default: launch_fin_kind<NN>(s, sc, meas, mode); break;   // nucleon and B-field test
+    }
+    YK_HIP(hipGetLastError());
+    if (meas) {
+        MeasConsts mc = meas_consts(s, mode);
+        hipLaunchKernelGGL(k_meas_final, dim3(1), dim3(BLOCK), 0, s->stream, s->blockpart,
+                           s->nblocks, mc, s->d_meas + slot);
+        YK_HIP(hipGetLastError());
+    }
+    return YK_OK;
+}
+
+int launch_drift(yk_system *s, double dt)
+{
+    StepConsts sc = step_consts(s, dt);
+    if (s->p.flags & YK_FLAG_KDK)
+        hipLaunchKernelGGL(k_drift<2>, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos,
+                           s->vel, s->vold, s->acc, sc);
+    else
+        hipLaunchKernelGGL(k_drift<1>, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos,
+                           s->vel, s->vold, s->acc, sc);
+    YK_HIP(hipGetLastError());
+    return YK_OK;
+}
+
+void to_meas(const MeasOut &m, yk_meas *out)
+{
+    out->ev = m.ev;
+    out->ek = m.ek;
+    out->px = m.px;
+    out->vir = m.vir;
+    for (int k = 0; k < 6; ++k) out->pp[k] = m.pp[k];
+}
+
+void zero_meas(yk_meas *out)
+{
+    if (out) std::memset(out, 0, sizeof *out);
+}
+
+// One step on the device: drift, pairs, finalize (+ measurement into slot).
+int device_step(yk_system *s, double dt, bool meas, int slot)
+{
+    int rc = launch_drift(s, dt);
+    if (rc) return rc;
+    if ((rc = launch_pairs(s, meas))) return rc;
+    const int mode = (s->p.flags & YK_FLAG_KDK) ? 2 : 1;
+    return launch_finalize(s, dt, meas, mode, slot);
+}
+
+void release(yk_system *s)
+{
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto e : s->ev) (void)hipEventDestroy(e);
+    if (s->step0) (void)hipEventDestroy(s->step0);
+    if (s->step1) (void)hipEventDestroy(s->step1);
+    (void)hipFree(s->pos);
+    (void)hipFree(s->vel);
+    (void)hipFree(s->vold);
+    (void)hipFree(s->acc);
+    (void)hipFree(s->aii);
+    (void)hipFree(s->den);
+    (void)hipFree(s->part);
+    (void)hipFree(s->blockpart);
+    (void)hipFree(s->d_meas);
+    if (s->h_stage) (void)hipHostFree(s->h_stage);
+    if (s->h_meas) (void)hipHostFree(s->h_meas);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+int32_t yk_abi_version(void) { return YK_ABI_VERSION; }
+
+const char *yk_last_error(void) { return g_err.c_str(); }
+
+int yk_create(const yk_params *params, int32_t n, const double *zii, const double *aii,
+              yk_system **out)
+{
+    if (!out) return fail(YK_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int rc = check_params(params);
+    if (rc) return rc;
+    if (n < 1) return fail(YK_ERR_ARG, "n=%d must be >= 1", n);
+    if (n > (1 << 26)) return fail(YK_ERR_ARG, "n=%d exceeds the supported 2^26", n);
+    if (params->sim_type != YK_PURE_ION && (!zii || !aii))
+        return fail(YK_ERR_ARG, "zii and aii are required for sim_type %d", params->sim_type);
+    yk_system *s = new (std::nothrow) yk_system();
+    if (!s) return fail(YK_ERR_NOMEM, "host allocation failed");
+    s->p = *params;
+    s->n = n;
+    choose_decomposition(s);
+    auto bail = [&](int code) {
+        release(s);
+        return code;
+    };
+#define YK_HIPB(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return bail(fail(e_ == hipErrorOutOfMemory ? YK_ERR_NOMEM : YK_ERR_HIP,    \
+                             "%s failed: %s", #call, hipGetErrorString(e_)));          \
+    } while (0)
+    YK_HIPB(hipGetDevice(&s->device));
+    YK_HIPB(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    const size_t nn = (size_t)n;
+    YK_HIPB(hipMalloc(&s->pos, sizeof(double4) * nn));
+    YK_HIPB(hipMalloc(&s->vel, sizeof(double) * 3 * nn));
+    YK_HIPB(hipMalloc(&s->vold, sizeof(double) * 3 * nn));
+    YK_HIPB(hipMalloc(&s->acc, sizeof(double) * 3 * nn));
+    YK_HIPB(hipMalloc(&s->aii, sizeof(double) * nn));
+    YK_HIPB(hipMalloc(&s->den, sizeof(double) * nn));
+    YK_HIPB(hipMalloc(&s->part, sizeof(double) * 11 * nn * (size_t)s->nchunks));
+    YK_HIPB(hipMalloc(&s->blockpart, sizeof(double) * NQ * (size_t)s->nblocks));
+    YK_HIPB(hipHostMalloc(&s->h_stage, sizeof(double) * 4 * nn, hipHostMallocDefault));
+    if ((rc = ensure_meas_cap(s, 1))) return bail(rc);
+    // per-particle Z (into pos.w), A and the acceleration denominator
+    double *hz = s->h_stage;            // n
+    double *ha = s->h_stage + nn;       // n
+    double *hd = s->h_stage + 2 * nn;   // n
+    for (size_t i = 0; i < nn; ++i) {
+        double zi_ = (params->sim_type == YK_PURE_ION) ? (zii ? zii[i] : params->zi) : zii[i];
+        double ai_ = (params->sim_type == YK_PURE_ION) ? (aii ? aii[i] : params->ai) : aii[i];
+        if (!std::isfinite(zi_) || !std::isfinite(ai_))
+            return bail(fail(YK_ERR_NONFINITE, "zii/aii(%zu) not finite", i));
+        hz[i] = zi_;
+        ha[i] = ai_;
+        switch (params->sim_type) {
+        case YK_PURE_ION: hd[i] = params->ai * params->xmass; break;
+        case YK_ION_MIXTURE:
+            if (!(ai_ > 0.0)) return bail(fail(YK_ERR_ARG, "aii(%zu)=%g must be > 0", i, ai_));
+            hd[i] = ai_ * params->xmass;
+            break;
+        default: hd[i] = params->xmass; break;
+        }
+    }
+    // positions start at the origin with the charges in .w
+    s->zhost.assign(hz, hz + nn);
+    std::vector<double4> p4(nn);
+    for (size_t i = 0; i < nn; ++i) p4[i] = make_double4(0.0, 0.0, 0.0, hz[i]);
+    YK_HIPB(hipMemcpyAsync(s->pos, p4.data(), sizeof(double4) * nn, hipMemcpyHostToDevice,
+                           s->stream));
+    YK_HIPB(hipMemcpyAsync(s->aii, ha, sizeof(double) * nn, hipMemcpyHostToDevice, s->stream));
+    YK_HIPB(hipMemcpyAsync(s->den, hd, sizeof(double) * nn, hipMemcpyHostToDevice, s->stream));
+    YK_HIPB(hipMemsetAsync(s->vel, 0, sizeof(double) * 3 * nn, s->stream));
+    YK_HIPB(hipMemsetAsync(s->vold, 0, sizeof(double) * 3 * nn, s->stream));
+    YK_HIPB(hipMemsetAsync(s->acc, 0, sizeof(double) * 3 * nn, s->stream));
+    YK_HIPB(hipStreamSynchronize(s->stream));
+#undef YK_HIPB
+    *out = s;
+    return YK_OK;
+}
+
+void yk_destroy(yk_system *sys) { release(sys); }
+
+int yk_set_params(yk_system *s, const yk_params *params)
+{
+    if (!s) return fail(YK_ERR_ARG, "handle is NULL");
+    int rc = check_params(params);
+    if (rc) return rc;
+    if (params->sim_type != s->p.sim_type || params->precision != s->p.precision)
+        return fail(YK_ERR_ARG, "sim_type and precision are fixed at yk_create");
+    if (params->sim_type == YK_PURE_ION &&
+        (params->ai != s->p.ai || params->xmass != s->p.xmass))
+        return fail(YK_ERR_UNSUPPORTED, "pure-ion ai/xmass are fixed at yk_create");
+    if (params->xmass != s->p.xmass)
+        return fail(YK_ERR_UNSUPPORTED, "xmass is fixed at yk_create");
+    s->p = *params;
+    s->have_accel = false;
+    return YK_OK;
+}
+
+int yk_put_state(yk_system *s, const double *x, const double *v)
+{
+    if (!s || !x) return fail(YK_ERR_ARG, "handle or x is NULL");
+    const int n = s->n;
+    const size_t nn = (size_t)n;
+    if (!finite3(x, n)) return fail(YK_ERR_NONFINITE, "positions contain NaN/Inf");
+    if (v && !finite3(v, n)) return fail(YK_ERR_NONFINITE, "velocities contain NaN/Inf");
+    YK_HIP(hipStreamSynchronize(s->stream));
+    std::vector<double4> p4(nn);
+    for (size_t i = 0; i < nn; ++i)
+        p4[i] = make_double4(x[3 * i + 0], x[3 * i + 1], x[3 * i + 2], s->zhost[i]);
+    YK_HIP(hipMemcpyAsync(s->pos, p4.data(), sizeof(double4) * nn, hipMemcpyHostToDevice,
+                          s->stream));
+    double *hv = s->h_stage;
+    if (v) {
+        for (size_t i = 0; i < nn; ++i)
+            for (int k = 0; k < 3; ++k) hv[k * nn + i] = v[3 * i + k];
+        YK_HIP(hipMemcpyAsync(s->vel, hv, sizeof(double) * 3 * nn, hipMemcpyHostToDevice,
+                              s->stream));
+    } else {
+        YK_HIP(hipMemsetAsync(s->vel, 0, sizeof(double) * 3 * nn, s->stream));
+    }
+    YK_HIP(hipStreamSynchronize(s->stream));
+    s->have_accel = false;
+    return YK_OK;
+}
+
+int yk_get_state(yk_system *s, double *x, double *v, double *a)
+{
+    if (!s) return fail(YK_ERR_ARG, "handle is NULL");
+    const size_t nn = (size_t)s->n;
+    YK_HIP(hipStreamSynchronize(s->stream));
+    if (x) {
+        std::vector<double4> p4(nn);
+        YK_HIP(hipMemcpy(p4.data(), s->pos, sizeof(double4) * nn, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < nn; ++i) {
+            x[3 * i + 0] = p4[i].x;
+            x[3 * i + 1] = p4[i].y;
+            x[3 * i + 2] = p4[i].z;
+        }
+    }
+    double *h = s->h_stage;
+    if (v) {
+        YK_HIP(hipMemcpy(h, s->vel, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < nn; ++i)
+            for (int k = 0; k < 3; ++k) v[3 * i + k] = h[k * nn + i];
+    }
+    if (a) {
+        YK_HIP(hipMemcpy(h, s->acc, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < nn; ++i)
+            for (int k = 0; k < 3; ++k) a[3 * i + k] = h[k * nn + i];
+    }
+    return YK_OK;
+}
+
+int yk_accel(yk_system *s, int32_t do_measurements, yk_meas *out)
+{
+    if (!s) return fail(YK_ERR_ARG, "handle is NULL");
+    const bool meas = do_measurements != 0;
+    int rc = launch_pairs(s, meas);
+    if (rc) return rc;
+    if ((rc = launch_finalize(s, 0.0, meas, 0, 0))) return rc;
+    if ((rc = timing_collect(s))) return rc;
+    s->have_accel = true;
+    if (meas) {
+        YK_HIP(hipMemcpyAsync(s->h_meas, s->d_meas, sizeof(MeasOut), hipMemcpyDeviceToHost,
+                              s->stream));
+        YK_HIP(hipStreamSynchronize(s->stream));
+        if (out) to_meas(s->h_meas[0], out);
+    } else {
+        YK_HIP(hipStreamSynchronize(s->stream));
+        zero_meas(out);
+    }
+    return YK_OK;
+}
+
+int yk_newton(yk_system *s, double dt, int32_t do_measurements, yk_meas *out)
+{
+    if (!s) return fail(YK_ERR_ARG, "handle is NULL");
+    if (!std::isfinite(dt)) return fail(YK_ERR_ARG, "dt=%g is not finite", dt);
+    if (!s->have_accel)
+        return fail(YK_ERR_STATE, "newton needs accelerations of the current positions: "
+                                  "call yk_accel first (src/md.F:69)");
+    const bool meas = do_measurements != 0;
+    int rc = device_step(s, dt, meas, 0);
+    if (rc) return rc;
+    if ((rc = timing_collect(s))) return rc;
+    if (meas) {
+        YK_HIP(hipMemcpyAsync(s->h_meas, s->d_meas, sizeof(MeasOut), hipMemcpyDeviceToHost,
+                              s->stream));
+        YK_HIP(hipStreamSynchronize(s->stream));
+        if (out) to_meas(s->h_meas[0], out);
+    } else {
+        YK_HIP(hipStreamSynchronize(s->stream));
+        zero_meas(out);
+    }
+    return YK_OK;
+}
+
+int yk_run(yk_system *s, double dt, int32_t nsteps, int32_t nind, yk_meas *records)
+{
+    if (!s) return fail(YK_ERR_ARG, "handle is NULL");
+    if (nsteps < 0) return fail(YK_ERR_ARG, "nsteps=%d < 0", nsteps);
+    if (!std::isfinite(dt)) return fail(YK_ERR_ARG, "dt=%g is not finite", dt);
+    if (!s->have_accel)
+        return fail(YK_ERR_STATE, "run needs accelerations of the current positions: "
+                                  "call yk_accel first (src/md.F:69)");
+    const int nrec = (nind > 0) ? nsteps / nind : 0;
+    int rc = ensure_meas_cap(s, nrec > 0 ? nrec : 1);
+    if (rc) return rc;
+    const bool timing = (s->p.flags & YK_FLAG_TIMING) != 0;
+    if (timing) {
+        if (!s->step0) YK_HIP(hipEventCreate(&s->step0));
+        if (!s->step1) YK_HIP(hipEventCreate(&s->step1));
+        YK_HIP(hipEventRecord(s->step0, s->stream));
+    }
+    for (int st = 1; st <= nsteps; ++st) {
+        const bool meas = (nind > 0) && (st % nind == 0);
+        if ((rc = device_step(s, dt, meas, meas ? st / nind - 1 : 0))) return rc;
+    }
+    if (timing) {
+        YK_HIP(hipEventRecord(s->step1, s->stream));
+        YK_HIP(hipEventSynchronize(s->step1));
+        float ms = 0.f;
+        YK_HIP(hipEventElapsedTime(&ms, s->step0, s->step1));
+        s->timing.step_ms += ms;
+        s->timing.steps += nsteps;
+    }
+    if ((rc = timing_collect(s))) return rc;
+    if (nrec > 0 && records) {
+        YK_HIP(hipMemcpyAsync(s->h_meas, s->d_meas, sizeof(MeasOut) * (size_t)nrec,
+                              hipMemcpyDeviceToHost, s->stream));
+    }
+    YK_HIP(hipStreamSynchronize(s->stream));
+    if (nrec > 0 && records)
+        for (int r = 0; r < nrec; ++r) to_meas(s->h_meas[r], &records[r]);
+    return YK_OK;
+}
+
+int yk_ttot(yk_system *s, double *eka)
+{
+    if (!s || !eka) return fail(YK_ERR_ARG, "handle or eka is NULL");
+    hipLaunchKernelGGL(k_ttot, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->vel, s->aii,
+                       s->n, s->blockpart);
+    YK_HIP(hipGetLastError());
+    MeasConsts mc = meas_consts(s, 3);
+    hipLaunchKernelGGL(k_meas_final, dim3(1), dim3(BLOCK), 0, s->stream, s->blockpart,
+                       s->nblocks, mc, s->d_meas);
+    YK_HIP(hipGetLastError());
+    YK_HIP(hipMemcpyAsync(s->h_meas, s->d_meas, sizeof(MeasOut), hipMemcpyDeviceToHost,
+                          s->stream));
+    YK_HIP(hipStreamSynchronize(s->stream));
+    *eka = s->h_meas[0].ek;
+    return YK_OK;
+}
+
+int yk_get_timing(yk_system *s, yk_timing *out)
+{
+    if (!s || !out) return fail(YK_ERR_ARG, "handle or out is NULL");
+    int rc = timing_collect(s);
+    if (rc) return rc;
+    *out = s->timing;
+    return YK_OK;
+}
+
+int yk_reset_timing(yk_system *s)
+{
+    if (!s) return fail(YK_ERR_ARG, "handle is NULL");
+    int rc = timing_collect(s);
+    if (rc) return rc;
+    s->timing = yk_timing{};
+    return YK_OK;
+}
+
+}  // extern "C"
