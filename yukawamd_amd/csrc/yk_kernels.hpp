@@ -35,6 +35,7 @@ struct ForceConsts {
     double xp2i, xpi, aa, aa2, bpc, bmc, vc;
     float muf, mul2ef, xp2il2ef, xpif, aaf, aa2f, bpcf, bmcf, vcf;
     int n, coul_on, nuc_on, pad;
+    ExpCoeffs ec;
 };
 
 struct StepConsts {
@@ -59,9 +60,11 @@ __device__ __forceinline__ void accumulate(Acc<MEAS> &a, double s, double dx,
                                            double dy, double dz, double r2,
                                            double phi)
 {
+    a.fx = __builtin_fma(s, dx, a.fx);   // identical in both templates
+    a.fy = __builtin_fma(s, dy, a.fy);
+    a.fz = __builtin_fma(s, dz, a.fz);
     if constexpr (MEAS) {
         double sx = s * dx, sy = s * dy, sz = s * dz;
-        a.fx += sx; a.fy += sy; a.fz += sz;
         a.e += phi;
         a.w = __builtin_fma(s, r2, a.w);
         a.pxx = __builtin_fma(dx, sx, a.pxx);
@@ -72,9 +75,6 @@ __device__ __forceinline__ void accumulate(Acc<MEAS> &a, double s, double dx,
         a.pzz = __builtin_fma(dz, sz, a.pzz);
     } else {
         (void)r2; (void)phi;
-        a.fx = __builtin_fma(s, dx, a.fx);
-        a.fy = __builtin_fma(s, dy, a.fy);
-        a.fz = __builtin_fma(s, dz, a.fz);
     }
 }
 
@@ -96,10 +96,10 @@ __device__ __forceinline__ void pair(const ForceConsts &c, const double4 &pi,
         if (r2 <= c.rc2 && !self) {
             double s, phi = 0.0;
             if constexpr (PREC == 0) {
-                double rinv = rsqrt64(r2);
+                double rinv = rsqrt64(r2, c.ec);
                 double r = r2 * rinv;
                 double mr = c.mu * r;
-                double ex = exp_neg64(-mr);
+                double ex = exp_neg64(-mr, c.ec);
                 double exr = ex * rinv;
                 double fc = (1.0 + mr) * exr * (rinv * rinv);
                 s = (KIND == MIX) ? pj.w * fc : fc;
@@ -131,16 +131,16 @@ __device__ __forceinline__ void pair(const ForceConsts &c, const double4 &pi,
             const bool same = (pi.w == pj.w);
             if constexpr (PREC == 0) {
                 if (hn) {
-                    double g = exp_neg64(-r2 * c.xp2i);
+                    double g = exp_neg64(-r2 * c.xp2i, c.ec);
                     double bc = same ? c.bpc : c.bmc;
                     s = c.xpi * __builtin_fma(c.aa2, g, bc) * g;
                     if constexpr (MEAS) phi = __builtin_fma(c.aa, g, bc) * g;
                 }
                 if (hc) {
-                    double rinv = rsqrt64(r2);
+                    double rinv = rsqrt64(r2, c.ec);
                     double r = r2 * rinv;
                     double mr = c.mu * r;
-                    double ex = exp_neg64(-mr);
+                    double ex = exp_neg64(-mr, c.ec);
                     double exr = c.vc * ex * rinv;
                     s += (1.0 + mr) * exr * (rinv * rinv);
                     if constexpr (MEAS) phi += exr - c.evcut;

@@ -145,6 +145,7 @@ ForceConsts force_consts(const yk_system *s)
     c.n = s->n;
     c.coul_on = p.coulomb_on ? 1 : 0;
     c.nuc_on = p.nuclear_on ? 1 : 0;
+    c.ec = exp_coeffs();
     return c;
 }
 
