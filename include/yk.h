@@ -84,7 +84,9 @@ typedef struct yk_params {
     uint32_t flags;      /* enum yk_flags                                     */
     int32_t coulomb_on;  /* coulomb .eq. 'screened-coulomb' (int_nn.f:167)    */
     int32_t nuclear_on;  /* nuclear .eq. 'HPP'              (int_nn.f:168)    */
-    int32_t reserved0;
+    int32_t resort_every;/* steps between Morton re-sorts of the device slots */
+                         /* (0 = library default); results are identical up   */
+                         /* to summation order whatever the value             */
     double xl[3];        /* box edge lengths (fm)                             */
     double xmuc;         /* inverse screening length (1/fm)                   */
     double rccut;        /* Coulomb cutoff radius (fm)                        */
@@ -142,6 +144,12 @@ int yk_set_params(yk_system *sys, const yk_params *params);
  * [0, xl] like src/vel_verlet_2.f:39-40.  Returns YK_ERR_NONFINITE if any
  * coordinate is NaN/Inf. */
 int yk_put_state(yk_system *sys, const double *x, const double *v);
+
+/* Upload accelerations (3,n) belonging to the current positions -- e.g. the
+ * a(3,0:n-1) a Fortran host kept across a host-side step -- and mark them
+ * valid for the next yk_newton.  Replaces the host's own write of a
+ * (src/md_globals.f:155). */
+int yk_put_accel(yk_system *sys, const double *a);
 
 /* Download positions, velocities, accelerations in (3,n) layout; any
  * pointer may be NULL.  Host readers of particle state (ttot, cancel_vcom,

@@ -20,7 +20,7 @@ class LibraryMissing(RuntimeError):
 class YkParams(C.Structure):
     _fields_ = [
         ("sim_type", C.c_int32), ("precision", C.c_int32), ("flags", C.c_uint32),
-        ("coulomb_on", C.c_int32), ("nuclear_on", C.c_int32), ("reserved0", C.c_int32),
+        ("coulomb_on", C.c_int32), ("nuclear_on", C.c_int32), ("resort_every", C.c_int32),
         ("xl", C.c_double * 3),
         ("xmuc", C.c_double), ("rccut", C.c_double), ("frp", C.c_double), ("vc", C.c_double),
         ("xmass", C.c_double), ("rho", C.c_double), ("kT", C.c_double),
@@ -67,6 +67,7 @@ SIGNATURES = {
     "yk_destroy": (None, [P]),
     "yk_set_params": (C.c_int, [P, C.POINTER(YkParams)]),
     "yk_put_state": (C.c_int, [P, D, D]),
+    "yk_put_accel": (C.c_int, [P, D]),
     "yk_get_state": (C.c_int, [P, D, D, D]),
     "yk_accel": (C.c_int, [P, C.c_int32, C.POINTER(YkMeas)]),
     "yk_newton": (C.c_int, [P, C.c_double, C.c_int32, C.POINTER(YkMeas)]),
@@ -112,6 +113,7 @@ def params_struct(p) -> YkParams:
     s.flags = p.flags
     s.coulomb_on = 1 if p.coulomb_on else 0
     s.nuclear_on = 1 if p.nuclear_on else 0
+    s.resort_every = int(p.resort_every)
     for k in range(3):
         s.xl[k] = p.xl[k]
     for name in ("xmuc", "rccut", "frp", "vc", "xmass", "rho", "kT", "zi", "ai", "aa", "bb",

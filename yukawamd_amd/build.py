@@ -47,16 +47,32 @@ def build_library(verbose: bool = True) -> Path:
     return LIB
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN = PKG / "fortran"
+
+
+def build_fortran() -> Path:
+    """Compile the iso_c_binding module (yk_bind.mod / yk_bind.o) in-tree."""
+    out = FORTRAN / "_build"
+    out.mkdir(exist_ok=True)
+    _run([FLANG, "-O2", "-fPIC", "-module-dir", out, "-c", FORTRAN / "yk_bind.f90",
+          "-o", out / "yk_bind.o"])
+    return out / "yk_bind.o"
+
+
 def build_oracle() -> None:
+    """The checker: C restatement always; the reference build and the drop-in
+    executables where the reference sources are present."""
     _run(["make", "-s", "-C", ROOT / "oracle"])
     if Path("/root/reference/src").is_dir():
-        _run(["make", "-s", "-C", ROOT / "oracle", "ref"])
+        _run(["make", "-s", "-C", ROOT / "oracle", "ref", "dropin"])
 
 
 def main(argv=None) -> int:
     argv = sys.argv[1:] if argv is None else argv
     build_library()
     if "--all" in argv:
+        build_fortran()
         build_oracle()
     return 0
 

@@ -1,40 +1,56 @@
 // yk_kernels.hpp -- HIP kernels of the Yukawa pair-force / velocity-Verlet
 // hot path for gfx950 (CDNA4, wave64).
 //
-// Layout in HBM (structure of arrays, one system per handle):
+// Layout in HBM (structure of arrays, one system per handle, particles kept
+// in Morton order of a 2^g x 2^g x 2^g cell grid so that 64 consecutive
+// particles -- one wave, one "tile" -- occupy a compact region):
 //   pos[n]    double4 {x, y, z, Z}   -- position and charge number, 32 B
 //   vel[3n]   component-major vx[n] | vy[n] | vz[n]
 //   vold[3n]  predictor copy for the vel_verlet_2 form
 //   acc[3n]   accelerations, component-major
 //   aii[n], den[n]   mass number, and the acceleration denominator
 //                    (A_i*xmass for ions, xmass for nucleons)
+//   orig[n]   caller's index of the particle in each slot
+//   box[T]    per tile of 64 particles: AABB centre and half-extent
 //   part[S][NC][n]   per-chunk partial sums of the pair kernel
 //
-// Pair kernel: each workgroup owns BLOCK consecutive targets i (one per
-// lane) and one chunk of the source range j; j tiles of TILE particles are
-// staged in LDS and read as wave-uniform broadcasts.  Every lane sums its own
-// target in registers (full N x N matrix, no atomics), so results are
-// deterministic; the S chunk partials are summed in fixed order by the
-// finalize kernel.  Minimum image is d - L*rint(d/L) per axis; the cutoff
-// mask is r^2 <= rc^2 and j != i.
+// Pair kernel: a workgroup is 4 waves = 4 target tiles; chunk y of the grid
+// covers a range of source tiles, staged 4 tiles (256 particles) at a time in
+// LDS and read as wave-uniform broadcasts.  For each (target tile, source
+// tile) a wave first tests the two AABBs: if their minimum-image gap exceeds
+// the cutoff the whole tile is skipped.  If every pair of the two tiles has
+// the same periodic image along each axis (the AABB spans leave room), the
+// wave shifts its own coordinates by that image once and each pair costs one
+// subtraction per axis; otherwise each pair takes d - L*rint(d/L).  Every lane
+// sums its own target in registers (full N x N matrix, no atomics), and the
+// chunk partials are summed in fixed order by the finalize kernel, so results
+// are deterministic.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "yk_math.hpp"
 
 namespace yk {
 
-constexpr int BLOCK = 256;   // targets per workgroup (4 wave64)
-constexpr int TILE = 256;    // sources per LDS tile
+constexpr int BLOCK = 256;   // threads per workgroup (4 wave64)
+constexpr int TILE = 64;     // particles per tile (one wave)
+constexpr int STAGE = 256;   // sources staged in LDS at a time (4 tiles)
 constexpr int NQ = 14;       // reduced scalars: E, W, PP[6], KE[6]
 
 enum Kind { PURE = 0, MIX = 1, NN = 2 };
 
+struct TileBox {
+    double c[3];   // AABB centre
+    double h[3];   // AABB half extent
+};
+
 struct ForceConsts {
-    double L[3], invL[3];
+    double L[3], invL[3], halfL_m[3];   // halfL_m: L/2 minus a rounding margin
     double rc2, rn2, mu, evcut;
+    double cull2;                        // (largest cutoff)^2 used for tile culling
+    double margin;                       // absolute rounding margin for AABB tests
     double xp2i, xpi, aa, aa2, bpc, bmc, vc;
     float muf, mul2ef, xp2il2ef, xpif, aaf, aa2f, bpcf, bmcf, vcf;
-    int n, coul_on, nuc_on, pad;
+    int n, ntiles, coul_on, nuc_on;
     ExpCoeffs ec;
 };
 
@@ -83,14 +99,14 @@ __device__ __forceinline__ double min_image(double d, double L, double invL)
     return __builtin_fma(-L, __builtin_rint(d * invL), d);
 }
 
-// One (i, j) pair.  PREC 0 = FP64, 1 = mixed.
+// Pair interaction given the minimum-image displacement d = x_i - x_j.
+// PREC 0 = FP64, 1 = mixed (FP64 displacement, r^2, cutoff test and
+// accumulation; FP32 1/r, exp and Gaussians).
 template <int KIND, int PREC, bool MEAS>
-__device__ __forceinline__ void pair(const ForceConsts &c, const double4 &pi,
-                                     const double4 &pj, bool self, Acc<MEAS> &acc)
+__device__ __forceinline__ void interact(const ForceConsts &c, double zi, double zj,
+                                         double dx, double dy, double dz, bool self,
+                                         Acc<MEAS> &acc)
 {
-    double dx = min_image(pi.x - pj.x, c.L[0], c.invL[0]);
-    double dy = min_image(pi.y - pj.y, c.L[1], c.invL[1]);
-    double dz = min_image(pi.z - pj.z, c.L[2], c.invL[2]);
     double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
     if constexpr (KIND != NN) {
         if (r2 <= c.rc2 && !self) {
@@ -102,10 +118,10 @@ __device__ __forceinline__ void pair(const ForceConsts &c, const double4 &pi,
                 double ex = exp_neg64(-mr, c.ec);
                 double exr = ex * rinv;
                 double fc = (1.0 + mr) * exr * (rinv * rinv);
-                s = (KIND == MIX) ? pj.w * fc : fc;
+                s = (KIND == MIX) ? zj * fc : fc;
                 if constexpr (MEAS) {
                     phi = exr - c.evcut;
-                    if (KIND == MIX) phi *= pj.w;
+                    if (KIND == MIX) phi *= zj;
                 }
             } else {
                 float r2f = (float)r2;
@@ -114,21 +130,20 @@ __device__ __forceinline__ void pair(const ForceConsts &c, const double4 &pi,
                 float ex = exp2_32(-c.mul2ef * r);
                 float exr = ex * rinv;
                 float fc = __builtin_fmaf(c.muf, r, 1.0f) * exr * (rinv * rinv);
-                float zj = (float)pj.w;
-                s = (double)((KIND == MIX) ? zj * fc : fc);
+                s = (double)((KIND == MIX) ? (float)zj * fc : fc);
                 if constexpr (MEAS) {
                     phi = (double)exr - c.evcut;
-                    if (KIND == MIX) phi *= pj.w;
+                    if (KIND == MIX) phi *= zj;
                 }
             }
             accumulate<MEAS>(acc, s, dx, dy, dz, r2, phi);
         }
     } else {
         bool hn = c.nuc_on && r2 <= c.rn2;
-        bool hc = c.coul_on && r2 <= c.rc2 && pi.w == 1.0 && pj.w == 1.0;
+        bool hc = c.coul_on && r2 <= c.rc2 && zi == 1.0 && zj == 1.0;
         if ((hn || hc) && !self) {
             double s = 0.0, phi = 0.0;
-            const bool same = (pi.w == pj.w);
+            const bool same = (zi == zj);
             if constexpr (PREC == 0) {
                 if (hn) {
                     double g = exp_neg64(-r2 * c.xp2i, c.ec);
@@ -169,33 +184,96 @@ __device__ __forceinline__ void pair(const ForceConsts &c, const double4 &pi,
     }
 }
 
+// A value equal in every lane of the wave, moved to an SGPR pair.
+__device__ __forceinline__ double uniform(double v)
+{
+    int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
 // ---------------------------------------------------------------------------
-// All-pairs partial sums over one j chunk.  grid = (ceil(n/BLOCK), S).
+// Tiled all-pairs partial sums.  grid = (ceil(n/BLOCK), S chunks).
 template <int KIND, int PREC, bool MEAS>
 __global__ __launch_bounds__(BLOCK) void k_pairs(const double4 *__restrict__ pos,
-                                                 ForceConsts c, int tiles_per_chunk,
+                                                 const TileBox *__restrict__ box,
+                                                 ForceConsts c, int stages_per_chunk,
                                                  double *__restrict__ part)
 {
-    __shared__ double4 sj[TILE];
+    __shared__ double4 sj[STAGE];
     const int n = c.n;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int lane = threadIdx.x & (TILE - 1);
+    const int itile = __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / TILE) +
+                                                     (int)(threadIdx.x / TILE));
+    const int i = itile * TILE + lane;
     const int chunk = blockIdx.y;
-    const int ntiles = (n + TILE - 1) / TILE;
+    const int nstages = (n + STAGE - 1) / STAGE;
     const double nanv = __builtin_nan("");
     const double4 pi = (i < n) ? pos[i] : make_double4(nanv, nanv, nanv, 0.0);
+    const bool wave_live = itile < c.ntiles;
+    double ic[3] = {0, 0, 0}, ih[3] = {0, 0, 0};
+    if (wave_live) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            ic[k] = uniform(box[itile].c[k]);
+            ih[k] = uniform(box[itile].h[k]);
+        }
+    }
     Acc<MEAS> acc;
-    const int t0 = chunk * tiles_per_chunk;
-    const int t1 = min(t0 + tiles_per_chunk, ntiles);
-    for (int t = t0; t < t1; ++t) {
-        const int jb = t * TILE;
+    const int s0 = chunk * stages_per_chunk;
+    const int s1 = min(s0 + stages_per_chunk, nstages);
+    for (int st = s0; st < s1; ++st) {
+        const int jb = st * STAGE;
         const int jl = jb + threadIdx.x;
         // padded sources are NaN: every comparison with them is false
         sj[threadIdx.x] = (jl < n) ? pos[jl] : make_double4(nanv, nanv, nanv, 0.0);
         __syncthreads();
-        const int self_jj = i - jb;   // in [0, TILE) only for the diagonal tile
+        if (wave_live) {
+#pragma unroll 1
+            for (int q = 0; q < STAGE / TILE; ++q) {
+                const int jt = st * (STAGE / TILE) + q;
+                if (jt >= c.ntiles) break;
+                // AABB culling and image selection, wave-uniform
+                double gap2 = 0.0, shift[3];
+                bool fast = true;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    double cd = ic[k] - box[jt].c[k];
+                    double h = ih[k] + box[jt].h[k] + c.margin;
+                    double s = c.L[k] * __builtin_rint(cd * c.invL[k]);
+                    double cm = __builtin_fabs(cd - s);
+                    double g = cm - h;
+                    g = g > 0.0 ? g : 0.0;
+                    gap2 = __builtin_fma(g, g, gap2);
+                    fast = fast && (cm + h < c.halfL_m[k]);
+                    shift[k] = s;
+                }
+                gap2 = uniform(gap2);
+                if (gap2 > c.cull2) continue;
+                const int self_jj = (jt == itile) ? lane : -1;
+                const double4 *src = sj + q * TILE;
+                if (uniform(fast ? 1.0 : 0.0) != 0.0) {
+                    const double xs = pi.x - uniform(shift[0]);
+                    const double ys = pi.y - uniform(shift[1]);
+                    const double zs = pi.z - uniform(shift[2]);
 #pragma unroll 4
-        for (int jj = 0; jj < TILE; ++jj)
-            pair<KIND, PREC, MEAS>(c, pi, sj[jj], jj == self_jj, acc);
+                    for (int jj = 0; jj < TILE; ++jj) {
+                        const double4 pj = src[jj];
+                        interact<KIND, PREC, MEAS>(c, pi.w, pj.w, xs - pj.x, ys - pj.y,
+                                                   zs - pj.z, jj == self_jj, acc);
+                    }
+                } else {
+#pragma unroll 2
+                    for (int jj = 0; jj < TILE; ++jj) {
+                        const double4 pj = src[jj];
+                        interact<KIND, PREC, MEAS>(
+                            c, pi.w, pj.w, min_image(pi.x - pj.x, c.L[0], c.invL[0]),
+                            min_image(pi.y - pj.y, c.L[1], c.invL[1]),
+                            min_image(pi.z - pj.z, c.L[2], c.invL[2]), jj == self_jj, acc);
+                    }
+                }
+            }
+        }
         __syncthreads();
     }
     if (i < n) {
@@ -208,6 +286,103 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const double4 *__restrict__ pos
             o[8 * nn] = acc.pyy; o[9 * nn] = acc.pyz; o[10 * nn] = acc.pzz;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Wave-level min/max and the per-tile AABB of 64 consecutive particles.
+__device__ __forceinline__ void wave_minmax(double &lo, double &hi)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double l2 = __shfl_xor(lo, off, 64);
+        double h2 = __shfl_xor(hi, off, 64);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+}
+
+__device__ __forceinline__ void tile_box_store(const double x[3], bool valid, int tile,
+                                               TileBox *__restrict__ box)
+{
+    double c[3], h[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double lo = valid ? x[k] : __builtin_inf();
+        double hi = valid ? x[k] : -__builtin_inf();
+        wave_minmax(lo, hi);
+        c[k] = 0.5 * (lo + hi);
+        h[k] = 0.5 * (hi - lo);
+    }
+    if ((threadIdx.x & (TILE - 1)) == 0) {
+        TileBox b;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { b.c[k] = c[k]; b.h[k] = h[k]; }
+        box[tile] = b;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_tile_boxes(const double4 *__restrict__ pos, int n,
+                                                      int ntiles, TileBox *__restrict__ box)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int tile = i / TILE;
+    double x[3] = {0, 0, 0};
+    const bool valid = i < n;
+    if (valid) { double4 p = pos[i]; x[0] = p.x; x[1] = p.y; x[2] = p.z; }
+    if (tile < ntiles) tile_box_store(x, valid, tile, box);
+}
+
+// ---------------------------------------------------------------------------
+// Morton keys of the current slots (cell grid 2^g per axis) and the gather
+// that applies a sort permutation to every per-particle array.
+__device__ __forceinline__ unsigned spread3(unsigned v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_morton(const double4 *__restrict__ pos, int n,
+                                                  double gx, double gy, double gz, int gmax,
+                                                  unsigned *__restrict__ key,
+                                                  int *__restrict__ val)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double4 p = pos[i];
+    int cx = (int)(p.x * gx), cy = (int)(p.y * gy), cz = (int)(p.z * gz);
+    cx = cx < 0 ? 0 : (cx > gmax ? gmax : cx);
+    cy = cy < 0 ? 0 : (cy > gmax ? gmax : cy);
+    cz = cz < 0 ? 0 : (cz > gmax ? gmax : cz);
+    key[i] = spread3((unsigned)cx) | (spread3((unsigned)cy) << 1) | (spread3((unsigned)cz) << 2);
+    val[i] = i;
+}
+
+struct PermArrays {
+    const double4 *pos; const double *vel, *vold, *acc, *aii, *den; const int *orig;
+    double4 *pos2; double *vel2, *vold2, *acc2, *aii2, *den2; int *orig2;
+};
+
+__global__ __launch_bounds__(BLOCK) void k_permute(const int *__restrict__ perm, int n,
+                                                   PermArrays a)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const size_t nn = (size_t)n;
+    const int s = perm[i];
+    a.pos2[i] = a.pos[s];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        a.vel2[k * nn + i] = a.vel[k * nn + s];
+        a.vold2[k * nn + i] = a.vold[k * nn + s];
+        a.acc2[k * nn + i] = a.acc[k * nn + s];
+    }
+    a.aii2[i] = a.aii[s];
+    a.den2[i] = a.den[s];
+    a.orig2[i] = a.orig[s];
 }
 
 // ---------------------------------------------------------------------------
@@ -229,12 +404,12 @@ __device__ __forceinline__ void block_reduce_store(double (&q)[NQ], double *__re
 }
 
 // MODE 0: accel only.  MODE 1: vel_verlet_2 corrector (src/vel_verlet_2.f:61-79).
-// MODE 2: kick-drift-kick closing half kick (src/vel_verlet_1.f).
+// MODE 2: kick-drift-kick closing half kick (src/vel_verlet_1.f:52-56).
 template <int KIND, bool MEAS, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_finalize(
     const double *__restrict__ part, int nchunks, const double4 *__restrict__ pos,
     const double *__restrict__ den, const double *__restrict__ aii,
-    double *__restrict__ acc, double *__restrict__ vel,
+    const int *__restrict__ orig, double *__restrict__ acc, double *__restrict__ vel,
     const double *__restrict__ vold, StepConsts sc, double *__restrict__ blockpart)
 {
     const int n = sc.n;
@@ -254,7 +429,9 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(
             for (int k = 0; k < NC; ++k) f[k] += p[k * nn];
         }
         double zi = pos[i].w;
-        double fz_i = (KIND == MIX && !(sc.ref_f1 && i == n - 1)) ? zi : 1.0;
+        // the reference's mixture kernel never applies zii to the caller's last
+        // particle (src/int_ion_mix.f:158, :201, :250); YK_FLAG_REF_F1 keeps that
+        double fz_i = (KIND == MIX && !(sc.ref_f1 && orig[i] == n - 1)) ? zi : 1.0;
         double d = den[i];
         double a[3];   // fz_i is exactly 1 unless this is a mixture
 #pragma unroll
@@ -303,37 +480,44 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(
     if constexpr (MEAS) block_reduce_store(q, blockpart + (size_t)blockIdx.x * NQ);
 }
 
-// Drift (+ velocity predictor or opening half kick).  MODE 1: vel_verlet_2
-// (src/vel_verlet_2.f:33-53); MODE 2: drift + half kick (src/vel_verlet_1.f:28-48).
+// Drift (+ velocity predictor or opening half kick), then the tile AABBs of
+// the new positions.  MODE 1: vel_verlet_2 (src/vel_verlet_2.f:33-53);
+// MODE 2: drift + half kick (src/vel_verlet_1.f:28-48).
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_drift(double4 *__restrict__ pos,
                                                  double *__restrict__ vel,
                                                  double *__restrict__ vold,
                                                  const double *__restrict__ acc,
-                                                 StepConsts sc)
+                                                 StepConsts sc, int ntiles,
+                                                 TileBox *__restrict__ box)
 {
     const int n = sc.n;
     const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
     const size_t nn = (size_t)n;
-    double4 p = pos[i];
-    double x[3] = {p.x, p.y, p.z};
+    const bool valid = i < n;
+    double x[3] = {0, 0, 0};
+    if (valid) {
+        double4 p = pos[i];
+        x[0] = p.x; x[1] = p.y; x[2] = p.z;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        double v = vel[k * nn + i], a = acc[k * nn + i], xk;
-        if constexpr (MODE == 1) {
-            xk = x[k] + v * sc.dt + sc.afac * a;   // contraction is off in this TU
-            vold[k * nn + i] = v;
-            vel[k * nn + i] = v + sc.dt * a;
-        } else {   // src/vel_verlet_1.f:30-48
-            xk = x[k] + v * sc.dt + a * sc.afac;
-            vel[k * nn + i] = v + sc.hdt * a;
+        for (int k = 0; k < 3; ++k) {
+            double v = vel[k * nn + i], a = acc[k * nn + i], xk;
+            if constexpr (MODE == 1) {
+                xk = x[k] + v * sc.dt + sc.afac * a;   // contraction is off in this TU
+                vold[k * nn + i] = v;
+                vel[k * nn + i] = v + sc.dt * a;
+            } else {   // src/vel_verlet_1.f:30-48
+                xk = x[k] + v * sc.dt + a * sc.afac;
+                vel[k * nn + i] = v + sc.hdt * a;
+            }
+            if (xk < 0.0) xk = xk + sc.L[k];
+            if (xk > sc.L[k]) xk = xk - sc.L[k];
+            x[k] = xk;
         }
-        if (xk < 0.0) xk = xk + sc.L[k];
-        if (xk > sc.L[k]) xk = xk - sc.L[k];
-        x[k] = xk;
+        pos[i] = make_double4(x[0], x[1], x[2], p.w);
     }
-    pos[i] = make_double4(x[0], x[1], x[2], p.w);
+    const int tile = i / TILE;
+    if (tile < ntiles) tile_box_store(x, valid, tile, box);
 }
 
 // Kinetic energy sum A*(vx^2+vy^2+vz^2) per block (ttot, src/gen_utils.f:40-55).

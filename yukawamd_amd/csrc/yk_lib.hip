@@ -15,6 +15,8 @@
 #include <string>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "../../include/yk.h"
 #include "yk_kernels.hpp"
 
@@ -49,23 +51,36 @@ struct yk_system {
     yk_params p{};
     int n = 0;
     int device = 0;
-    std::vector<double> zhost;     // charge numbers, kept for position uploads
+    // caller-order host copies of the per-particle constants
+    std::vector<double> zhost, ahost, dhost;
     hipStream_t stream = nullptr;
-    // device state
-    double4 *pos = nullptr;
+    // device state, in Morton-sorted slot order (second set: permutation target)
+    double4 *pos = nullptr, *pos2 = nullptr;
     double *vel = nullptr, *vold = nullptr, *acc = nullptr;
-    double *aii = nullptr, *den = nullptr;
+    double *vel2 = nullptr, *vold2 = nullptr, *acc2 = nullptr;
+    double *aii = nullptr, *den = nullptr, *aii2 = nullptr, *den2 = nullptr;
+    int *orig = nullptr, *orig2 = nullptr;   // caller index of each slot
+    TileBox *box = nullptr;                  // [ntiles]
     double *part = nullptr;        // [nchunks][11][n]
     double *blockpart = nullptr;   // [nblocks][NQ]
     MeasOut *d_meas = nullptr;     // device scalar records
     int meas_cap = 0;
+    // sort scratch
+    unsigned *keys = nullptr, *keys2 = nullptr;
+    int *vals = nullptr, *vals2 = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int grid_bits = 1;
     // host staging (pinned)
     double *h_stage = nullptr;     // 4n doubles
+    int *h_orig = nullptr;         // n
     MeasOut *h_meas = nullptr;
     int h_meas_cap = 0;
     // decomposition
-    int nblocks = 0, ntiles = 0, nchunks = 0, tiles_per_chunk = 0;
+    int nblocks = 0, ntiles = 0, nstages = 0, nchunks = 0, stages_per_chunk = 0;
     bool have_accel = false;
+    int64_t steps_done = 0;        // newton steps since the last put_state
+    int resort_every = 0;
     // timing
     std::vector<hipEvent_t> ev;    // pairs of (start, stop) for force launches
     size_t ev_used = 0;
@@ -143,9 +158,21 @@ ForceConsts force_consts(const yk_system *s)
     c.bmcf = (float)c.bmc;
     c.vcf = (float)p.vc;
     c.n = s->n;
+    c.ntiles = s->ntiles;
     c.coul_on = p.coulomb_on ? 1 : 0;
     c.nuc_on = p.nuclear_on ? 1 : 0;
     c.ec = exp_coeffs();
+    double lmax = std::max(p.xl[0], std::max(p.xl[1], p.xl[2]));
+    c.margin = 1e-12 * lmax;
+    for (int k = 0; k < 3; ++k) c.halfL_m[k] = 0.5 * p.xl[k] - 4.0 * c.margin;
+    double cut = p.rccut;
+    if (p.sim_type == YK_NUCLEON) {
+        cut = 0.0;
+        if (p.coulomb_on) cut = p.rccut;
+        if (p.nuclear_on) cut = std::max(cut, p.rncut);
+    }
+    // tiles farther apart than the cutoff (plus rounding slack) are skipped
+    c.cull2 = (cut >= 1e100) ? 1e300 : cut * cut * (1.0 + 1e-12) + c.margin;
     return c;
 }
 
@@ -192,12 +219,17 @@ void choose_decomposition(yk_system *s)
     const int n = s->n;
     s->nblocks = (n + BLOCK - 1) / BLOCK;
     s->ntiles = (n + TILE - 1) / TILE;
+    s->nstages = (n + STAGE - 1) / STAGE;
     // enough workgroups to fill 256 CUs several times over
     int want = (2048 + s->nblocks - 1) / s->nblocks;
     if (want < 1) want = 1;
-    if (want > s->ntiles) want = s->ntiles;
-    s->tiles_per_chunk = (s->ntiles + want - 1) / want;
-    s->nchunks = (s->ntiles + s->tiles_per_chunk - 1) / s->tiles_per_chunk;
+    if (want > s->nstages) want = s->nstages;
+    s->stages_per_chunk = (s->nstages + want - 1) / want;
+    s->nchunks = (s->nstages + s->stages_per_chunk - 1) / s->stages_per_chunk;
+    // Morton grid: about 1-2 particles per cell
+    int g = 1;
+    while (g < 10 && (double)(1 << (3 * g)) < 0.5 * n) ++g;
+    s->grid_bits = g;
 }
 
 int ensure_meas_cap(yk_system *s, int cap)
@@ -255,10 +287,10 @@ void launch_pairs_k(yk_system *s, const ForceConsts &c, bool meas)
     dim3 grid(s->nblocks, s->nchunks);
     if (meas)
         hipLaunchKernelGGL((k_pairs<KIND, PREC, true>), grid, dim3(BLOCK), 0, s->stream,
-                           s->pos, c, s->tiles_per_chunk, s->part);
+                           s->pos, s->box, c, s->stages_per_chunk, s->part);
     else
         hipLaunchKernelGGL((k_pairs<KIND, PREC, false>), grid, dim3(BLOCK), 0, s->stream,
-                           s->pos, c, s->tiles_per_chunk, s->part);
+                           s->pos, s->box, c, s->stages_per_chunk, s->part);
 }
 
 int launch_pairs(yk_system *s, bool meas)
@@ -295,8 +327,8 @@ template <int KIND, bool MEAS, int MODE>
 void launch_fin_k(yk_system *s, const StepConsts &sc)
 {
     hipLaunchKernelGGL((k_finalize<KIND, MEAS, MODE>), dim3(s->nblocks), dim3(BLOCK), 0,
-                       s->stream, s->part, s->nchunks, s->pos, s->den, s->aii, s->acc,
-                       s->vel, s->vold, sc, s->blockpart);
+                       s->stream, s->part, s->nchunks, s->pos, s->den, s->aii, s->orig,
+                       s->acc, s->vel, s->vold, sc, s->blockpart);
 }
 
 template <int KIND>
@@ -337,10 +369,10 @@ int launch_drift(yk_system *s, double dt)
     StepConsts sc = step_consts(s, dt);
     if (s->p.flags & YK_FLAG_KDK)
         hipLaunchKernelGGL(k_drift<2>, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos,
-                           s->vel, s->vold, s->acc, sc);
+                           s->vel, s->vold, s->acc, sc, s->ntiles, s->box);
     else
         hipLaunchKernelGGL(k_drift<1>, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos,
-                           s->vel, s->vold, s->acc, sc);
+                           s->vel, s->vold, s->acc, sc, s->ntiles, s->box);
     YK_HIP(hipGetLastError());
     return YK_OK;
 }
@@ -359,10 +391,46 @@ void zero_meas(yk_meas *out)
     if (out) std::memset(out, 0, sizeof *out);
 }
 
-// One step on the device: drift, pairs, finalize (+ measurement into slot).
+// Re-establish Morton order of the slots: keys, stable radix sort, gather
+// of every per-particle array into the second buffer set, swap, tile AABBs.
+int resort(yk_system *s)
+{
+    const int n = s->n;
+    const int g = s->grid_bits;
+    const double G = (double)(1 << g);
+    hipLaunchKernelGGL(k_morton, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos, n,
+                       G / s->p.xl[0], G / s->p.xl[1], G / s->p.xl[2], (1 << g) - 1, s->keys,
+                       s->vals);
+    YK_HIP(hipGetLastError());
+    size_t bytes = s->sort_tmp_bytes;
+    YK_HIP(hipcub::DeviceRadixSort::SortPairs(s->sort_tmp, bytes, s->keys, s->keys2, s->vals,
+                                              s->vals2, n, 0, 3 * g, s->stream));
+    PermArrays a{s->pos, s->vel, s->vold, s->acc, s->aii, s->den, s->orig,
+                 s->pos2, s->vel2, s->vold2, s->acc2, s->aii2, s->den2, s->orig2};
+    hipLaunchKernelGGL(k_permute, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->vals2, n, a);
+    YK_HIP(hipGetLastError());
+    std::swap(s->pos, s->pos2);
+    std::swap(s->vel, s->vel2);
+    std::swap(s->vold, s->vold2);
+    std::swap(s->acc, s->acc2);
+    std::swap(s->aii, s->aii2);
+    std::swap(s->den, s->den2);
+    std::swap(s->orig, s->orig2);
+    hipLaunchKernelGGL(k_tile_boxes, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos, n,
+                       s->ntiles, s->box);
+    YK_HIP(hipGetLastError());
+    return YK_OK;
+}
+
+// One step on the device: (periodic re-sort), drift, pairs, finalize
+// (+ measurement into slot).
 int device_step(yk_system *s, double dt, bool meas, int slot)
 {
-    int rc = launch_drift(s, dt);
+    int rc;
+    if (s->resort_every > 0 && s->steps_done > 0 && s->steps_done % s->resort_every == 0)
+        if ((rc = resort(s))) return rc;
+    s->steps_done += 1;
+    rc = launch_drift(s, dt);
     if (rc) return rc;
     if ((rc = launch_pairs(s, meas))) return rc;
     const int mode = (s->p.flags & YK_FLAG_KDK) ? 2 : 1;
@@ -376,16 +444,15 @@ void release(yk_system *s)
     for (auto e : s->ev) (void)hipEventDestroy(e);
     if (s->step0) (void)hipEventDestroy(s->step0);
     if (s->step1) (void)hipEventDestroy(s->step1);
-    (void)hipFree(s->pos);
-    (void)hipFree(s->vel);
-    (void)hipFree(s->vold);
-    (void)hipFree(s->acc);
-    (void)hipFree(s->aii);
-    (void)hipFree(s->den);
-    (void)hipFree(s->part);
-    (void)hipFree(s->blockpart);
-    (void)hipFree(s->d_meas);
+    for (void *ptr : {(void *)s->pos, (void *)s->pos2, (void *)s->vel, (void *)s->vel2,
+                      (void *)s->vold, (void *)s->vold2, (void *)s->acc, (void *)s->acc2,
+                      (void *)s->aii, (void *)s->aii2, (void *)s->den, (void *)s->den2,
+                      (void *)s->orig, (void *)s->orig2, (void *)s->box, (void *)s->part,
+                      (void *)s->blockpart, (void *)s->d_meas, (void *)s->keys,
+                      (void *)s->keys2, (void *)s->vals, (void *)s->vals2, s->sort_tmp})
+        if (ptr) (void)hipFree(ptr);
     if (s->h_stage) (void)hipHostFree(s->h_stage);
+    if (s->h_orig) (void)hipHostFree(s->h_orig);
     if (s->h_meas) (void)hipHostFree(s->h_meas);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -415,6 +482,7 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
     if (!s) return fail(YK_ERR_NOMEM, "host allocation failed");
     s->p = *params;
     s->n = n;
+    s->resort_every = params->resort_every > 0 ? params->resort_every : 16;
     choose_decomposition(s);
     auto bail = [&](int code) {
         release(s);
@@ -427,51 +495,54 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
             return bail(fail(e_ == hipErrorOutOfMemory ? YK_ERR_NOMEM : YK_ERR_HIP,    \
                              "%s failed: %s", #call, hipGetErrorString(e_)));          \
     } while (0)
-    YK_HIPB(hipGetDevice(&s->device));
-    YK_HIPB(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     const size_t nn = (size_t)n;
-    YK_HIPB(hipMalloc(&s->pos, sizeof(double4) * nn));
-    YK_HIPB(hipMalloc(&s->vel, sizeof(double) * 3 * nn));
-    YK_HIPB(hipMalloc(&s->vold, sizeof(double) * 3 * nn));
-    YK_HIPB(hipMalloc(&s->acc, sizeof(double) * 3 * nn));
-    YK_HIPB(hipMalloc(&s->aii, sizeof(double) * nn));
-    YK_HIPB(hipMalloc(&s->den, sizeof(double) * nn));
-    YK_HIPB(hipMalloc(&s->part, sizeof(double) * 11 * nn * (size_t)s->nchunks));
-    YK_HIPB(hipMalloc(&s->blockpart, sizeof(double) * NQ * (size_t)s->nblocks));
-    YK_HIPB(hipHostMalloc(&s->h_stage, sizeof(double) * 4 * nn, hipHostMallocDefault));
-    if ((rc = ensure_meas_cap(s, 1))) return bail(rc);
-    // per-particle Z (into pos.w), A and the acceleration denominator
-    double *hz = s->h_stage;            // n
-    double *ha = s->h_stage + nn;       // n
-    double *hd = s->h_stage + 2 * nn;   // n
+    // per-particle Z, A and the acceleration denominator, caller order
+    s->zhost.resize(nn);
+    s->ahost.resize(nn);
+    s->dhost.resize(nn);
     for (size_t i = 0; i < nn; ++i) {
         double zi_ = (params->sim_type == YK_PURE_ION) ? (zii ? zii[i] : params->zi) : zii[i];
         double ai_ = (params->sim_type == YK_PURE_ION) ? (aii ? aii[i] : params->ai) : aii[i];
         if (!std::isfinite(zi_) || !std::isfinite(ai_))
             return bail(fail(YK_ERR_NONFINITE, "zii/aii(%zu) not finite", i));
-        hz[i] = zi_;
-        ha[i] = ai_;
+        s->zhost[i] = zi_;
+        s->ahost[i] = ai_;
         switch (params->sim_type) {
-        case YK_PURE_ION: hd[i] = params->ai * params->xmass; break;
+        case YK_PURE_ION: s->dhost[i] = params->ai * params->xmass; break;
         case YK_ION_MIXTURE:
             if (!(ai_ > 0.0)) return bail(fail(YK_ERR_ARG, "aii(%zu)=%g must be > 0", i, ai_));
-            hd[i] = ai_ * params->xmass;
+            s->dhost[i] = ai_ * params->xmass;
             break;
-        default: hd[i] = params->xmass; break;
+        default: s->dhost[i] = params->xmass; break;
         }
     }
-    // positions start at the origin with the charges in .w
-    s->zhost.assign(hz, hz + nn);
-    std::vector<double4> p4(nn);
-    for (size_t i = 0; i < nn; ++i) p4[i] = make_double4(0.0, 0.0, 0.0, hz[i]);
-    YK_HIPB(hipMemcpyAsync(s->pos, p4.data(), sizeof(double4) * nn, hipMemcpyHostToDevice,
-                           s->stream));
-    YK_HIPB(hipMemcpyAsync(s->aii, ha, sizeof(double) * nn, hipMemcpyHostToDevice, s->stream));
-    YK_HIPB(hipMemcpyAsync(s->den, hd, sizeof(double) * nn, hipMemcpyHostToDevice, s->stream));
-    YK_HIPB(hipMemsetAsync(s->vel, 0, sizeof(double) * 3 * nn, s->stream));
-    YK_HIPB(hipMemsetAsync(s->vold, 0, sizeof(double) * 3 * nn, s->stream));
-    YK_HIPB(hipMemsetAsync(s->acc, 0, sizeof(double) * 3 * nn, s->stream));
-    YK_HIPB(hipStreamSynchronize(s->stream));
+    YK_HIPB(hipGetDevice(&s->device));
+    YK_HIPB(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    YK_HIPB(hipMalloc(&s->pos, sizeof(double4) * nn));
+    YK_HIPB(hipMalloc(&s->pos2, sizeof(double4) * nn));
+    for (double **pp : {&s->vel, &s->vel2, &s->vold, &s->vold2, &s->acc, &s->acc2})
+        YK_HIPB(hipMalloc(pp, sizeof(double) * 3 * nn));
+    for (double **pp : {&s->aii, &s->aii2, &s->den, &s->den2})
+        YK_HIPB(hipMalloc(pp, sizeof(double) * nn));
+    YK_HIPB(hipMalloc(&s->orig, sizeof(int) * nn));
+    YK_HIPB(hipMalloc(&s->orig2, sizeof(int) * nn));
+    YK_HIPB(hipMalloc(&s->box, sizeof(TileBox) * (size_t)s->ntiles));
+    YK_HIPB(hipMalloc(&s->part, sizeof(double) * 11 * nn * (size_t)s->nchunks));
+    YK_HIPB(hipMalloc(&s->blockpart, sizeof(double) * NQ * (size_t)s->nblocks));
+    YK_HIPB(hipMalloc(&s->keys, sizeof(unsigned) * nn));
+    YK_HIPB(hipMalloc(&s->keys2, sizeof(unsigned) * nn));
+    YK_HIPB(hipMalloc(&s->vals, sizeof(int) * nn));
+    YK_HIPB(hipMalloc(&s->vals2, sizeof(int) * nn));
+    YK_HIPB(hipcub::DeviceRadixSort::SortPairs(nullptr, s->sort_tmp_bytes, s->keys, s->keys2,
+                                               s->vals, s->vals2, n, 0, 3 * s->grid_bits,
+                                               s->stream));
+    YK_HIPB(hipMalloc(&s->sort_tmp, s->sort_tmp_bytes > 0 ? s->sort_tmp_bytes : 16));
+    YK_HIPB(hipHostMalloc(&s->h_stage, sizeof(double) * 4 * nn, hipHostMallocDefault));
+    YK_HIPB(hipHostMalloc(&s->h_orig, sizeof(int) * nn, hipHostMallocDefault));
+    if ((rc = ensure_meas_cap(s, 1))) return bail(rc);
+    // all particles at the origin until the caller uploads positions
+    std::vector<double> zero(3 * nn, 0.0);
+    if ((rc = yk_put_state(s, zero.data(), nullptr))) return bail(rc);
 #undef YK_HIPB
     *out = s;
     return YK_OK;
@@ -492,6 +563,7 @@ int yk_set_params(yk_system *s, const yk_params *params)
     if (params->xmass != s->p.xmass)
         return fail(YK_ERR_UNSUPPORTED, "xmass is fixed at yk_create");
     s->p = *params;
+    if (params->resort_every > 0) s->resort_every = params->resort_every;
     s->have_accel = false;
     return YK_OK;
 }
@@ -504,10 +576,20 @@ int yk_put_state(yk_system *s, const double *x, const double *v)
     if (!finite3(x, n)) return fail(YK_ERR_NONFINITE, "positions contain NaN/Inf");
     if (v && !finite3(v, n)) return fail(YK_ERR_NONFINITE, "velocities contain NaN/Inf");
     YK_HIP(hipStreamSynchronize(s->stream));
+    // caller order first (orig = identity), then Morton-sort on the device
     std::vector<double4> p4(nn);
-    for (size_t i = 0; i < nn; ++i)
+    std::vector<int> iota(nn);
+    for (size_t i = 0; i < nn; ++i) {
         p4[i] = make_double4(x[3 * i + 0], x[3 * i + 1], x[3 * i + 2], s->zhost[i]);
+        iota[i] = (int)i;
+    }
     YK_HIP(hipMemcpyAsync(s->pos, p4.data(), sizeof(double4) * nn, hipMemcpyHostToDevice,
+                          s->stream));
+    YK_HIP(hipMemcpyAsync(s->orig, iota.data(), sizeof(int) * nn, hipMemcpyHostToDevice,
+                          s->stream));
+    YK_HIP(hipMemcpyAsync(s->aii, s->ahost.data(), sizeof(double) * nn, hipMemcpyHostToDevice,
+                          s->stream));
+    YK_HIP(hipMemcpyAsync(s->den, s->dhost.data(), sizeof(double) * nn, hipMemcpyHostToDevice,
                           s->stream));
     double *hv = s->h_stage;
     if (v) {
@@ -518,8 +600,30 @@ int yk_put_state(yk_system *s, const double *x, const double *v)
     } else {
         YK_HIP(hipMemsetAsync(s->vel, 0, sizeof(double) * 3 * nn, s->stream));
     }
+    YK_HIP(hipMemsetAsync(s->vold, 0, sizeof(double) * 3 * nn, s->stream));
+    YK_HIP(hipMemsetAsync(s->acc, 0, sizeof(double) * 3 * nn, s->stream));
+    int rc = resort(s);
+    if (rc) return rc;
     YK_HIP(hipStreamSynchronize(s->stream));
     s->have_accel = false;
+    s->steps_done = 0;
+    return YK_OK;
+}
+
+int yk_put_accel(yk_system *s, const double *a)
+{
+    if (!s || !a) return fail(YK_ERR_ARG, "handle or a is NULL");
+    const size_t nn = (size_t)s->n;
+    if (!finite3(a, s->n)) return fail(YK_ERR_NONFINITE, "accelerations contain NaN/Inf");
+    YK_HIP(hipStreamSynchronize(s->stream));
+    YK_HIP(hipMemcpy(s->h_orig, s->orig, sizeof(int) * nn, hipMemcpyDeviceToHost));
+    double *h = s->h_stage;
+    for (size_t i = 0; i < nn; ++i)
+        for (int k = 0; k < 3; ++k) h[k * nn + i] = a[3 * (size_t)s->h_orig[i] + k];
+    YK_HIP(hipMemcpyAsync(s->acc, h, sizeof(double) * 3 * nn, hipMemcpyHostToDevice,
+                          s->stream));
+    YK_HIP(hipStreamSynchronize(s->stream));
+    s->have_accel = true;
     return YK_OK;
 }
 
@@ -528,25 +632,28 @@ int yk_get_state(yk_system *s, double *x, double *v, double *a)
     if (!s) return fail(YK_ERR_ARG, "handle is NULL");
     const size_t nn = (size_t)s->n;
     YK_HIP(hipStreamSynchronize(s->stream));
+    YK_HIP(hipMemcpy(s->h_orig, s->orig, sizeof(int) * nn, hipMemcpyDeviceToHost));
+    const int *o = s->h_orig;
     if (x) {
         std::vector<double4> p4(nn);
         YK_HIP(hipMemcpy(p4.data(), s->pos, sizeof(double4) * nn, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < nn; ++i) {
-            x[3 * i + 0] = p4[i].x;
-            x[3 * i + 1] = p4[i].y;
-            x[3 * i + 2] = p4[i].z;
+            double *xi = x + 3 * (size_t)o[i];
+            xi[0] = p4[i].x;
+            xi[1] = p4[i].y;
+            xi[2] = p4[i].z;
         }
     }
     double *h = s->h_stage;
     if (v) {
         YK_HIP(hipMemcpy(h, s->vel, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < nn; ++i)
-            for (int k = 0; k < 3; ++k) v[3 * i + k] = h[k * nn + i];
+            for (int k = 0; k < 3; ++k) v[3 * (size_t)o[i] + k] = h[k * nn + i];
     }
     if (a) {
         YK_HIP(hipMemcpy(h, s->acc, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < nn; ++i)
-            for (int k = 0; k < 3; ++k) a[3 * i + k] = h[k * nn + i];
+            for (int k = 0; k < 3; ++k) a[3 * (size_t)o[i] + k] = h[k * nn + i];
     }
     return YK_OK;
 }

@@ -1,0 +1,181 @@
+!-------------------------------------------------------------------------------
+! yk_bind.f90 -- iso_c_binding interface to libyukawa_hip.so (include/yk.h).
+!
+! A Fortran host keeps control of the run and calls the MI355X pair-force /
+! velocity-Verlet kernels through these interfaces.  The derived types are
+! interoperable copies of yk_params / yk_meas / yk_timing; logicals cross the
+! boundary as integer(c_int32_t); arrays are the reference's (3,n) layout.
+!
+! Entry points and the reference routines they replace:
+!   yk_accel   accel(do_measurements)   src/int_control.f:52-81
+!   yk_newton  newton(do_measurements)  src/vel_verlet_2.f:18-120
+!   yk_run     nsteps of newton with the md.F measurement cadence (src/md.F:129)
+!-------------------------------------------------------------------------------
+module yk_bind
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int32_t), parameter, public :: YK_PURE_ION = 0, YK_ION_MIXTURE = 1, &
+                                           YK_NUCLEON = 2, YK_BFIELD_TEST = 3
+  integer(c_int32_t), parameter, public :: YK_FP64 = 0, YK_MIXED = 1
+  integer(c_int32_t), parameter, public :: YK_FLAG_REF_F1 = 1, YK_FLAG_KDK = 2, &
+                                           YK_FLAG_TIMING = 4
+  integer(c_int), parameter, public :: YK_OK = 0
+
+  type, bind(C), public :: yk_params_t
+    integer(c_int32_t) :: sim_type = YK_PURE_ION
+    integer(c_int32_t) :: precision = YK_FP64
+    integer(c_int32_t) :: flags = 0
+    integer(c_int32_t) :: coulomb_on = 1
+    integer(c_int32_t) :: nuclear_on = 1
+    integer(c_int32_t) :: resort_every = 0
+    real(c_double) :: xl(3) = 0.0_c_double
+    real(c_double) :: xmuc = 0.0_c_double
+    real(c_double) :: rccut = 1.0e50_c_double
+    real(c_double) :: frp = 1.0_c_double
+    real(c_double) :: vc = 197.327_c_double / 137.036_c_double
+    real(c_double) :: xmass = 931.0_c_double
+    real(c_double) :: rho = 0.0_c_double
+    real(c_double) :: kT = 0.0_c_double
+    real(c_double) :: zi = 0.0_c_double
+    real(c_double) :: ai = 0.0_c_double
+    real(c_double) :: aa = 110.0_c_double
+    real(c_double) :: bb = -26.0_c_double
+    real(c_double) :: cc = 24.0_c_double
+    real(c_double) :: xpacket = 1.25_c_double
+    real(c_double) :: rncut = 0.0_c_double
+    real(c_double) :: bfield = 0.0_c_double
+  end type yk_params_t
+
+  type, bind(C), public :: yk_meas_t
+    real(c_double) :: ev = 0, ek = 0, px = 0, vir = 0
+    real(c_double) :: pp(6) = 0
+  end type yk_meas_t
+
+  type, bind(C), public :: yk_timing_t
+    real(c_double) :: force_ms = 0
+    integer(c_int64_t) :: force_launches = 0
+    real(c_double) :: step_ms = 0
+    integer(c_int64_t) :: steps = 0
+    integer(c_int64_t) :: pairs_swept = 0
+  end type yk_timing_t
+
+  public :: yk_abi_version, yk_create, yk_destroy, yk_set_params, yk_put_state, &
+            yk_put_accel, yk_get_state, yk_accel, yk_newton, yk_run, yk_ttot, &
+            yk_get_timing, yk_reset_timing, yk_error_message
+
+  interface
+    integer(c_int32_t) function yk_abi_version() bind(C, name='yk_abi_version')
+      import :: c_int32_t
+    end function
+
+    type(c_ptr) function yk_last_error_c() bind(C, name='yk_last_error')
+      import :: c_ptr
+    end function
+
+    integer(c_int) function yk_create(params, n, zii, aii, sys) bind(C, name='yk_create')
+      import :: c_int, c_int32_t, c_double, c_ptr, yk_params_t
+      type(yk_params_t), intent(in) :: params
+      integer(c_int32_t), value :: n
+      real(c_double), intent(in) :: zii(*), aii(*)
+      type(c_ptr), intent(out) :: sys
+    end function
+
+    subroutine yk_destroy(sys) bind(C, name='yk_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: sys
+    end subroutine
+
+    integer(c_int) function yk_set_params(sys, params) bind(C, name='yk_set_params')
+      import :: c_int, c_ptr, yk_params_t
+      type(c_ptr), value :: sys
+      type(yk_params_t), intent(in) :: params
+    end function
+
+    integer(c_int) function yk_put_state(sys, x, v) bind(C, name='yk_put_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: sys
+      real(c_double), intent(in) :: x(3, *), v(3, *)
+    end function
+
+    integer(c_int) function yk_put_accel(sys, a) bind(C, name='yk_put_accel')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: sys
+      real(c_double), intent(in) :: a(3, *)
+    end function
+
+    integer(c_int) function yk_get_state(sys, x, v, a) bind(C, name='yk_get_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: sys
+      real(c_double), intent(out) :: x(3, *), v(3, *), a(3, *)
+    end function
+
+    integer(c_int) function yk_accel(sys, do_measurements, meas) bind(C, name='yk_accel')
+      import :: c_int, c_int32_t, c_ptr, yk_meas_t
+      type(c_ptr), value :: sys
+      integer(c_int32_t), value :: do_measurements
+      type(yk_meas_t), intent(out) :: meas
+    end function
+
+    integer(c_int) function yk_newton(sys, dt, do_measurements, meas) bind(C, name='yk_newton')
+      import :: c_int, c_int32_t, c_ptr, c_double, yk_meas_t
+      type(c_ptr), value :: sys
+      real(c_double), value :: dt
+      integer(c_int32_t), value :: do_measurements
+      type(yk_meas_t), intent(out) :: meas
+    end function
+
+    integer(c_int) function yk_run(sys, dt, nsteps, nind, records) bind(C, name='yk_run')
+      import :: c_int, c_int32_t, c_ptr, c_double, yk_meas_t
+      type(c_ptr), value :: sys
+      real(c_double), value :: dt
+      integer(c_int32_t), value :: nsteps, nind
+      type(yk_meas_t), intent(inout) :: records(*)
+    end function
+
+    integer(c_int) function yk_ttot(sys, eka) bind(C, name='yk_ttot')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: sys
+      real(c_double), intent(out) :: eka
+    end function
+
+    integer(c_int) function yk_get_timing(sys, t) bind(C, name='yk_get_timing')
+      import :: c_int, c_ptr, yk_timing_t
+      type(c_ptr), value :: sys
+      type(yk_timing_t), intent(out) :: t
+    end function
+
+    integer(c_int) function yk_reset_timing(sys) bind(C, name='yk_reset_timing')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sys
+    end function
+
+    integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
+      import :: c_size_t, c_ptr
+      type(c_ptr), value :: s
+    end function
+  end interface
+
+contains
+
+  ! yk_last_error() as a Fortran string.
+  function yk_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: k, m
+    p = yk_last_error_c()
+    if (.not. c_associated(p)) then
+      msg = ''
+      return
+    end if
+    m = int(c_strlen(p))
+    call c_f_pointer(p, chars, [m])
+    allocate(character(len=m) :: msg)
+    do k = 1, m
+      msg(k:k) = chars(k)
+    end do
+  end function yk_error_message
+
+end module yk_bind

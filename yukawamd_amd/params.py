@@ -55,6 +55,7 @@ class SimParams:
     ref_compat_f1: bool = False         # keep the mixture kernel's last-particle defect
     integrator: str = "vv2"             # 'vv2' (vel_verlet_2.f) | 'kdk' (vel_verlet_1.f)
     timing: bool = False
+    resort_every: int = 0               # device re-sort cadence (0 = library default)
 
     def __post_init__(self):
         if self.sim_type not in SIM_TYPES:

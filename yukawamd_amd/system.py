@@ -91,6 +91,10 @@ class YukawaSystem:
         va = _as_aos(v, self.n) if v is not None else None
         _lib.check(self.lib.yk_put_state(self._h, _dptr(xa), _dptr(va)))
 
+    def put_accel(self, a):
+        aa = _as_aos(a, self.n)
+        _lib.check(self.lib.yk_put_accel(self._h, _dptr(aa)))
+
     def get_state(self):
         x = np.empty((self.n, 3))
         v = np.empty((self.n, 3))
