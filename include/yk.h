@@ -62,7 +62,10 @@ enum yk_flags {
      * predictor/corrector form (src/vel_verlet_2.f:33-79). */
     YK_FLAG_KDK = 1u << 1,
     /* Record HIP-event timings of every force launch (yk_get_timing). */
-    YK_FLAG_TIMING = 1u << 2
+    YK_FLAG_TIMING = 1u << 2,
+    /* Evaluate all tile pairs (AABB-culled) instead of the cluster-pair
+     * lists; fixed at yk_create. */
+    YK_FLAG_ALLPAIRS = 1u << 3
 };
 
 enum yk_status {
@@ -118,7 +121,8 @@ typedef struct yk_timing {
     int64_t force_launches; /* number of force-kernel launches timed       */
     double step_ms;         /* sum of whole-step durations (HIP events)    */
     int64_t steps;          /* number of steps timed                       */
-    int64_t pairs_swept;    /* directed pairs evaluated by timed launches  */
+    int64_t pairs_swept;    /* directed pairs evaluated since the last     */
+                            /* yk_reset_timing (masked lanes included)     */
 } yk_timing;
 
 /* Library -------------------------------------------------------------- */

@@ -83,7 +83,8 @@ def initial_line(log):
 
 
 @pytest.mark.parametrize("name,runmd", [("pure_ion_3456", pure_runmd()),
-                                        ("ion_mixture_2000", mix_runmd())])
+                                        ("ion_mixture_2000", mix_runmd())],
+                         ids=["pure_ion_3456", "ion_mixture_2000"])
 def test_dropin_matches_reference_host(tmp_path, name, runmd):
     log_c, (st_c, m_c) = run(MD_OMP, runmd, tmp_path / "cpu", threads=8)
     log_g, (st_g, m_g) = run(MD_GPU, runmd, tmp_path / "gpu", threads=1)

@@ -53,7 +53,8 @@ def abs_pair_force_sum(p, x, Z, A):
     return out
 
 
-def gpu_accel(p, x, Z, A, do_meas=True):
+def gpu_accel(p, x, Z, A, do_meas=True, allpairs=False):
+    p = p.replace(allpairs=allpairs)
     with YukawaSystem(p, Z=Z, A=A) as s:
         s.put_state(x)
         m = s.accel(do_meas)
@@ -119,14 +120,15 @@ def test_golden_fp64_three_steps(name):
     lambda: W.nucleon(n=4000, seed=52),
     lambda: W.ion_mixture(n=1024, seed=53, kappa=2.0),
 ])
-def test_fp64_against_oracle(make):
+@pytest.mark.parametrize("allpairs", [False, True], ids=["pairlist", "allpairs"])
+def test_fp64_against_oracle(make, allpairs):
     w = make()
     p = w.params
     for dm in (True, False):
         a_o, m_o = oracle.accel(p, w.x, w.Z, w.A, dm, ref_f1=0)
-        a_g, m_g = gpu_accel(p, w.x, w.Z, w.A, dm)
+        a_g, m_g = gpu_accel(p, w.x, w.Z, w.A, dm, allpairs)
         rel, cond = force_errors(p, a_g, a_o, w.x, w.Z, w.A)
-        print(f"{w.name} meas={dm}: max rel {rel:.2e} cond {cond:.2e}")
+        print(f"{w.name} allpairs={allpairs} meas={dm}: max rel {rel:.2e} cond {cond:.2e}")
         assert cond <= 1e-12 and rel <= 1e-10
         if dm:
             check_meas(m_g, m_o)
@@ -218,3 +220,39 @@ def test_error_paths():
         assert e.value.code == 4
         s.accel(False)
         s.newton(False)
+
+
+def test_positions_outside_the_box():
+    """Positions handed over outside [0, L) give the forces of their wrapped
+    images (minimum image is translation-invariant by whole boxes)."""
+    w = W.ion_mixture(n=1500, seed=101)
+    L = w.params.xl[0]
+    shift = np.random.default_rng(3).integers(-1, 2, size=w.x.shape) * L
+    a0, m0 = gpu_accel(w.params, w.x, w.Z, w.A, True)
+    a1, m1 = gpu_accel(w.params, w.x + shift, w.Z, w.A, True)
+    np.testing.assert_allclose(a1, a0, rtol=1e-10, atol=1e-12 * np.abs(a0).max())
+    assert m1["ev"] == pytest.approx(m0["ev"], rel=1e-12)
+
+
+def test_pairlist_matches_allpairs_over_a_trajectory():
+    """60 steps from a bcc start: the pair lists (rebuilt on the device when a
+    particle has moved skin/2) and the all-pairs kernel give the same
+    trajectory to summation-order rounding."""
+    w = W.ion_mixture(n=3456, seed=102, start="bcc")
+    out = {}
+    for ap in (False, True):
+        with YukawaSystem(w.params.replace(allpairs=ap), Z=w.Z, A=w.A, dt=w.dt * 4) as s:
+            s.put_state(w.x, w.v)
+            s.accel(True)
+            recs = s.run(60, nind=20)
+            out[ap] = (s.get_state(), recs)
+    (xa, va, aa), ra = out[False]
+    (xb, vb, ab), rb = out[True]
+    L = w.params.xl[0]
+    dx = np.abs(xa - xb)
+    dx = np.minimum(dx, L - dx)
+    assert dx.max() <= 1e-9 * L
+    np.testing.assert_allclose(aa, ab, rtol=1e-6, atol=1e-9 * np.abs(ab).max())
+    for r1, r2 in zip(ra, rb):
+        assert r1["ev"] == pytest.approx(r2["ev"], rel=1e-11)
+        assert r1["ek"] == pytest.approx(r2["ek"], rel=1e-10)

@@ -198,8 +198,10 @@ template <int KIND, int PREC, bool MEAS>
 __global__ __launch_bounds__(BLOCK) void k_pairs(const double4 *__restrict__ pos,
                                                  const TileBox *__restrict__ box,
                                                  ForceConsts c, int stages_per_chunk,
-                                                 double *__restrict__ part)
+                                                 double *__restrict__ part,
+                                                 unsigned long long *__restrict__ swept)
 {
+    int nswept = 0;   // source tiles this wave evaluated
     __shared__ double4 sj[STAGE];
     const int n = c.n;
     const int lane = threadIdx.x & (TILE - 1);
@@ -250,6 +252,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const double4 *__restrict__ pos
                 }
                 gap2 = uniform(gap2);
                 if (gap2 > c.cull2) continue;
+                ++nswept;
                 const int self_jj = (jt == itile) ? lane : -1;
                 const double4 *src = sj + q * TILE;
                 if (uniform(fast ? 1.0 : 0.0) != 0.0) {
@@ -276,6 +279,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const double4 *__restrict__ pos
         }
         __syncthreads();
     }
+    if (lane == 0 && nswept > 0) atomicAdd(swept, (unsigned long long)nswept * TILE * TILE);
     if (i < n) {
         const size_t nn = (size_t)n;
         double *o = part + (size_t)chunk * (MEAS ? 11 : 3) * nn + i;
@@ -287,6 +291,190 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const double4 *__restrict__ pos
         }
     }
 }
+
+// ---------------------------------------------------------------------------
+// Cluster-pair lists.  For every target tile t the list holds the source
+// slots whose minimum-image distance to the tile's AABB is within the list
+// radius rl = cutoff + skin.  An entry is the slot index (bits 0-25) and, per
+// axis, the periodic image of the source relative to the tile centre
+// (bits 26+2k: 0 = -L, 1 = 0, 2 = +L).  "Fast" entries -- every target of the
+// tile sees the source through that one image as long as no particle has
+// moved more than skin/2 -- are stored from the front of the tile's region;
+// the rest ("general", minimum image per pair) from the back.  The order is
+// deterministic (by slot).  A list stays valid while the largest displacement
+// since its build is below skin/2; the drift kernel tracks that maximum and
+// the build kernel runs only when it is exceeded (or after a re-sort).
+struct ListConsts {
+    double L[3], invL[3], halfL_m[3];
+    double rl2;          // (cutoff + skin)^2
+    double skin;
+    double thresh2;      // (skin/2)^2: rebuild when a displacement exceeds it
+    int n, ntiles, cap;  // cap = entries per tile region (= n)
+    int pad;
+};
+
+constexpr unsigned SLOT_MASK = (1u << 26) - 1u;
+
+__device__ __forceinline__ unsigned long long as_bits(double v)
+{
+    return (unsigned long long)__double_as_longlong(v);
+}
+
+// One workgroup per target tile; sources scanned in slot order, 256 at a time.
+__global__ __launch_bounds__(BLOCK) void k_list_build(const double4 *__restrict__ pos,
+                                                      const TileBox *__restrict__ box,
+                                                      ListConsts lc,
+                                                      const unsigned long long *__restrict__ maxdisp,
+                                                      unsigned *__restrict__ list,
+                                                      int2 *__restrict__ counts,
+                                                      double4 *__restrict__ xref)
+{
+    if (*maxdisp <= as_bits(lc.thresh2)) return;   // list still valid (uniform)
+    const int t = blockIdx.x;
+    const int lane = threadIdx.x & (TILE - 1), wv = threadIdx.x / TILE;
+    __shared__ int wf[BLOCK / TILE], wg[BLOCK / TILE];
+    double c[3], h[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { c[k] = box[t].c[k]; h[k] = box[t].h[k]; }
+    unsigned *mine = list + (size_t)t * lc.cap;
+    int nf = 0, ng = 0;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int jb = 0; jb < lc.n; jb += BLOCK) {
+        const int j = jb + threadIdx.x;
+        bool cand = false, fast = true;
+        unsigned code = 0;
+        if (j < lc.n) {
+            const double4 pj = pos[j];
+            const double xj[3] = {pj.x, pj.y, pj.z};
+            double d2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double cd = c[k] - xj[k];
+                double m = __builtin_rint(cd * lc.invL[k]);
+                double e = __builtin_fabs(cd - lc.L[k] * m);
+                double g = e - h[k];
+                g = g > 0.0 ? g : 0.0;
+                d2 = __builtin_fma(g, g, d2);
+                // positions outside [0, L) can give |m| > 1: those take the
+                // per-pair minimum image and keep a zero image code
+                const bool m_ok = (m >= -1.0) && (m <= 1.0);
+                fast = fast && m_ok && (e + h[k] + lc.skin < lc.halfL_m[k]);
+                code |= (unsigned)(m_ok ? (int)m + 1 : 1) << (26 + 2 * k);
+            }
+            cand = d2 <= lc.rl2;
+        }
+        const unsigned long long bf = __ballot(cand && fast);
+        const unsigned long long bg = __ballot(cand && !fast);
+        if (lane == 0) {
+            wf[wv] = __popcll(bf);
+            wg[wv] = __popcll(bg);
+        }
+        __syncthreads();
+        int of = nf, og = ng, tf = 0, tg = 0;
+#pragma unroll
+        for (int w = 0; w < BLOCK / TILE; ++w) {
+            if (w < wv) { of += wf[w]; og += wg[w]; }
+            tf += wf[w];
+            tg += wg[w];
+        }
+        if (cand && fast) mine[of + __popcll(bf & lt)] = (unsigned)j | code;
+        if (cand && !fast) mine[lc.cap - 1 - (og + __popcll(bg & lt))] = (unsigned)j | code;
+        nf += tf;
+        ng += tg;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) counts[t] = make_int2(nf, ng);
+    if (threadIdx.x < TILE) {
+        const int i = t * TILE + threadIdx.x;
+        if (i < lc.n) xref[i] = pos[i];
+    }
+}
+
+// Pair sums over the lists.  Wave = one (tile, item) with item m of M
+// covering an equal share of the tile's list.  Sources of each batch of 64
+// entries are gathered by the lanes into the wave's own LDS slice, already
+// shifted to their image, then read as broadcasts.  Self pairs are the
+// pairs with d = 0 (the source is the target itself); coincident distinct
+// particles are skipped the same way.
+template <int KIND, int PREC, bool MEAS>
+__global__ __launch_bounds__(BLOCK) void k_pairs_list(const double4 *__restrict__ pos,
+                                                      const unsigned *__restrict__ list,
+                                                      const int2 *__restrict__ counts,
+                                                      ForceConsts c, int cap, int M,
+                                                      double *__restrict__ part,
+                                                      unsigned long long *__restrict__ maxdisp,
+                                                      unsigned long long thresh_bits,
+                                                      unsigned long long *__restrict__ swept)
+{
+    __shared__ double4 sj[BLOCK / TILE][TILE];
+    const int n = c.n;
+    // the build kernel (previous launch) has consumed the displacement flag
+    if (blockIdx.x == 0 && threadIdx.x == 0 && *maxdisp > thresh_bits) *maxdisp = 0ull;
+    const int lane = threadIdx.x & (TILE - 1);
+    const int wv = threadIdx.x / TILE;
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / TILE) + wv);
+    const int t = item / M, m = item % M;
+    if (t >= c.ntiles) return;   // whole wave; this kernel has no block barrier
+    const int i = t * TILE + lane;
+    const double nanv = __builtin_nan("");
+    const double4 pi = (i < n) ? pos[i] : make_double4(nanv, nanv, nanv, 0.0);
+    const int2 cnt = counts[t];
+    const int nf = __builtin_amdgcn_readfirstlane(cnt.x);
+    const int total = __builtin_amdgcn_readfirstlane(cnt.x + cnt.y);
+    const int per = (total + M - 1) / M;
+    const int lo = min(m * per, total), hi = min(lo + per, total);
+    const unsigned *mine = list + (size_t)t * cap;
+    Acc<MEAS> acc;
+    double4 *sw = sj[wv];
+    if (lane == 0 && hi > lo) atomicAdd(swept, (unsigned long long)(hi - lo) * TILE);
+    for (int v0 = lo; v0 < hi; v0 += TILE) {
+        const int v = v0 + lane;
+        if (v < hi) {
+            const unsigned e = (v < nf) ? mine[v] : mine[cap - 1 - (v - nf)];
+            double4 pj = pos[e & SLOT_MASK];
+            if (v < nf) {   // apply the entry's periodic image
+                pj.x += c.L[0] * (double)((int)((e >> 26) & 3u) - 1);
+                pj.y += c.L[1] * (double)((int)((e >> 28) & 3u) - 1);
+                pj.z += c.L[2] * (double)((int)((e >> 30) & 3u) - 1);
+            }
+            sw[lane] = pj;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int cntb = min(TILE, hi - v0);
+        const int nfast = max(0, min(cntb, nf - v0));
+#pragma unroll 4
+        for (int jj = 0; jj < nfast; ++jj) {
+            const double4 pj = sw[jj];
+            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const bool self = (dx == 0.0) & (dy == 0.0) & (dz == 0.0);
+            interact<KIND, PREC, MEAS>(c, pi.w, pj.w, dx, dy, dz, self, acc);
+        }
+#pragma unroll 2
+        for (int jj = nfast; jj < cntb; ++jj) {
+            const double4 pj = sw[jj];
+            const double dx = min_image(pi.x - pj.x, c.L[0], c.invL[0]);
+            const double dy = min_image(pi.y - pj.y, c.L[1], c.invL[1]);
+            const double dz = min_image(pi.z - pj.z, c.L[2], c.invL[2]);
+            const bool self = (dx == 0.0) & (dy == 0.0) & (dz == 0.0);
+            interact<KIND, PREC, MEAS>(c, pi.w, pj.w, dx, dy, dz, self, acc);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    constexpr int NC = MEAS ? 11 : 3;
+    double *o = part + ((size_t)t * M + m) * NC * TILE + lane;
+    o[0] = acc.fx; o[TILE] = acc.fy; o[2 * TILE] = acc.fz;
+    if constexpr (MEAS) {
+        o[3 * TILE] = acc.e; o[4 * TILE] = acc.w;
+        o[5 * TILE] = acc.pxx; o[6 * TILE] = acc.pxy; o[7 * TILE] = acc.pxz;
+        o[8 * TILE] = acc.pyy; o[9 * TILE] = acc.pyz; o[10 * TILE] = acc.pzz;
+    }
+}
+
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
 // ---------------------------------------------------------------------------
 // Wave-level min/max and the per-tile AABB of 64 consecutive particles.
@@ -405,9 +593,11 @@ __device__ __forceinline__ void block_reduce_store(double (&q)[NQ], double *__re
 
 // MODE 0: accel only.  MODE 1: vel_verlet_2 corrector (src/vel_verlet_2.f:61-79).
 // MODE 2: kick-drift-kick closing half kick (src/vel_verlet_1.f:52-56).
-template <int KIND, bool MEAS, int MODE>
+// LAYOUT 0: partials part[c][NC][n] from k_pairs (nparts chunks);
+// LAYOUT 1: part[t][m][NC][TILE] from k_pairs_list (nparts items per tile).
+template <int KIND, bool MEAS, int MODE, int LAYOUT>
 __global__ __launch_bounds__(BLOCK) void k_finalize(
-    const double *__restrict__ part, int nchunks, const double4 *__restrict__ pos,
+    const double *__restrict__ part, int nparts, const double4 *__restrict__ pos,
     const double *__restrict__ den, const double *__restrict__ aii,
     const int *__restrict__ orig, double *__restrict__ acc, double *__restrict__ vel,
     const double *__restrict__ vold, StepConsts sc, double *__restrict__ blockpart)
@@ -423,10 +613,19 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(
         double f[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) f[k] = 0.0;
-        for (int c = 0; c < nchunks; ++c) {
-            const double *p = part + (size_t)c * NC * nn + i;
+        if constexpr (LAYOUT == 0) {
+            for (int c = 0; c < nparts; ++c) {
+                const double *p = part + (size_t)c * NC * nn + i;
 #pragma unroll
-            for (int k = 0; k < NC; ++k) f[k] += p[k * nn];
+                for (int k = 0; k < NC; ++k) f[k] += p[k * nn];
+            }
+        } else {
+            const int t = i / TILE, lane = i % TILE;
+            for (int m = 0; m < nparts; ++m) {
+                const double *p = part + ((size_t)t * nparts + m) * NC * TILE + lane;
+#pragma unroll
+                for (int k = 0; k < NC; ++k) f[k] += p[k * TILE];
+            }
         }
         double zi = pos[i].w;
         // the reference's mixture kernel never applies zii to the caller's last
@@ -489,7 +688,9 @@ __global__ __launch_bounds__(BLOCK) void k_drift(double4 *__restrict__ pos,
                                                  double *__restrict__ vold,
                                                  const double *__restrict__ acc,
                                                  StepConsts sc, int ntiles,
-                                                 TileBox *__restrict__ box)
+                                                 TileBox *__restrict__ box,
+                                                 const double4 *__restrict__ xref,
+                                                 unsigned long long *__restrict__ maxdisp)
 {
     const int n = sc.n;
     const int i = blockIdx.x * BLOCK + threadIdx.x;
@@ -518,6 +719,25 @@ __global__ __launch_bounds__(BLOCK) void k_drift(double4 *__restrict__ pos,
     }
     const int tile = i / TILE;
     if (tile < ntiles) tile_box_store(x, valid, tile, box);
+    if (xref) {   // largest minimum-image displacement since the list build
+        double d2 = 0.0;
+        if (valid) {
+            const double4 r = xref[i];
+            const double xr[3] = {r.x, r.y, r.z};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double d = x[k] - xr[k];
+                d = __builtin_fma(-sc.L[k], __builtin_rint(d / sc.L[k]), d);
+                d2 = __builtin_fma(d, d, d2);
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            double o = __shfl_xor(d2, off, 64);
+            d2 = o > d2 ? o : d2;
+        }
+        if ((threadIdx.x & (TILE - 1)) == 0 && d2 > 0.0) atomicMax(maxdisp, as_bits(d2));
+    }
 }
 
 // Kinetic energy sum A*(vx^2+vy^2+vz^2) per block (ttot, src/gen_utils.f:40-55).

@@ -61,7 +61,18 @@ struct yk_system {
     double *aii = nullptr, *den = nullptr, *aii2 = nullptr, *den2 = nullptr;
     int *orig = nullptr, *orig2 = nullptr;   // caller index of each slot
     TileBox *box = nullptr;                  // [ntiles]
-    double *part = nullptr;        // [nchunks][11][n]
+    // cluster-pair lists
+    bool use_list = true;
+    int list_M = 16;                         // work items per target tile
+    double skin = 0.0;
+    unsigned *list = nullptr;                // [ntiles][n]
+    int2 *counts = nullptr;                  // [ntiles] (fast, general)
+    double4 *xref = nullptr;                 // positions at the last build
+    unsigned long long *maxdisp = nullptr;   // max displacement^2 since build (bits)
+    unsigned long long *swept = nullptr;     // directed pairs evaluated (counter)
+    unsigned long long swept_base = 0;
+    double *part = nullptr;        // [nchunks][11][n] or [ntiles][M][11][64]
+    size_t part_doubles = 0;
     double *blockpart = nullptr;   // [nblocks][NQ]
     MeasOut *d_meas = nullptr;     // device scalar records
     int meas_cap = 0;
@@ -176,6 +187,55 @@ ForceConsts force_consts(const yk_system *s)
     return c;
 }
 
+double list_cutoff(const yk_params &p)
+{
+    double cut = p.rccut;
+    if (p.sim_type == YK_NUCLEON) {
+        cut = 0.0;
+        if (p.coulomb_on) cut = p.rccut;
+        if (p.nuclear_on) cut = std::max(cut, p.rncut);
+    }
+    return cut;
+}
+
+ListConsts list_consts(const yk_system *s)
+{
+    const yk_params &p = s->p;
+    ListConsts l{};
+    double lmax = std::max(p.xl[0], std::max(p.xl[1], p.xl[2]));
+    double margin = 1e-12 * lmax;
+    for (int k = 0; k < 3; ++k) {
+        l.L[k] = p.xl[k];
+        l.invL[k] = 1.0 / p.xl[k];
+        l.halfL_m[k] = 0.5 * p.xl[k] - 4.0 * margin;
+    }
+    double rl = list_cutoff(p) + s->skin + margin;
+    l.rl2 = rl * rl;
+    l.skin = s->skin;
+    l.thresh2 = 0.25 * s->skin * s->skin;
+    l.n = s->n;
+    l.ntiles = s->ntiles;
+    l.cap = s->n;
+    return l;
+}
+
+unsigned long long bits_of(double v)
+{
+    unsigned long long b;
+    std::memcpy(&b, &v, sizeof b);
+    return b;
+}
+
+// Forces a list rebuild before the next pair evaluation.
+int invalidate_list(yk_system *s)
+{
+    if (!s->use_list) return YK_OK;
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s->stream, s->maxdisp,
+                       bits_of(__builtin_inf()));
+    YK_HIP(hipGetLastError());
+    return YK_OK;
+}
+
 StepConsts step_consts(const yk_system *s, double dt)
 {
     const yk_params &p = s->p;
@@ -275,35 +335,70 @@ int timing_collect(yk_system *s)
         YK_HIP(hipEventElapsedTime(&ms, s->ev[k], s->ev[k + 1]));
         s->timing.force_ms += ms;
         s->timing.force_launches += 1;
-        s->timing.pairs_swept += (int64_t)s->n * (int64_t)(s->n - 1);
     }
     s->ev_used = 0;
+    return YK_OK;
+}
+
+// Directed pairs evaluated by the pair kernels since the last reset
+// (lanes of a swept tile or list batch count, masked or not).
+int read_swept(yk_system *s, unsigned long long *out)
+{
+    YK_HIP(hipStreamSynchronize(s->stream));
+    YK_HIP(hipMemcpy(out, s->swept, sizeof *out, hipMemcpyDeviceToHost));
     return YK_OK;
 }
 
 template <int KIND, int PREC>
 void launch_pairs_k(yk_system *s, const ForceConsts &c, bool meas)
 {
+    if (s->use_list) {
+        const ListConsts lc = list_consts(s);
+        const unsigned long long tb = bits_of(lc.thresh2);
+        const int items = s->ntiles * s->list_M;
+        dim3 grid((items + BLOCK / TILE - 1) / (BLOCK / TILE));
+        if (meas)
+            hipLaunchKernelGGL((k_pairs_list<KIND, PREC, true>), grid, dim3(BLOCK), 0, s->stream,
+                               s->pos, s->list, s->counts, c, s->n, s->list_M, s->part,
+                               s->maxdisp, tb, s->swept);
+        else
+            hipLaunchKernelGGL((k_pairs_list<KIND, PREC, false>), grid, dim3(BLOCK), 0,
+                               s->stream, s->pos, s->list, s->counts, c, s->n, s->list_M,
+                               s->part, s->maxdisp, tb, s->swept);
+        return;
+    }
     dim3 grid(s->nblocks, s->nchunks);
     if (meas)
         hipLaunchKernelGGL((k_pairs<KIND, PREC, true>), grid, dim3(BLOCK), 0, s->stream,
-                           s->pos, s->box, c, s->stages_per_chunk, s->part);
+                           s->pos, s->box, c, s->stages_per_chunk, s->part, s->swept);
     else
         hipLaunchKernelGGL((k_pairs<KIND, PREC, false>), grid, dim3(BLOCK), 0, s->stream,
-                           s->pos, s->box, c, s->stages_per_chunk, s->part);
+                           s->pos, s->box, c, s->stages_per_chunk, s->part, s->swept);
+}
+
+// Rebuild the pair lists if any particle moved more than skin/2 since the
+// last build (decided on the device; the kernel exits at once otherwise).
+int launch_list_build(yk_system *s)
+{
+    if (!s->use_list) return YK_OK;
+    const ListConsts lc = list_consts(s);
+    hipLaunchKernelGGL(k_list_build, dim3(s->ntiles), dim3(BLOCK), 0, s->stream, s->pos,
+                       s->box, lc, s->maxdisp, s->list, s->counts, s->xref);
+    YK_HIP(hipGetLastError());
+    return YK_OK;
 }
 
 int launch_pairs(yk_system *s, bool meas)
 {
     if (s->n < 2 || s->p.sim_type == YK_BFIELD_TEST) {
-        YK_HIP(hipMemsetAsync(s->part, 0, sizeof(double) * 11 * (size_t)s->n * s->nchunks,
-                              s->stream));
+        YK_HIP(hipMemsetAsync(s->part, 0, sizeof(double) * s->part_doubles, s->stream));
         return YK_OK;
     }
     ForceConsts c = force_consts(s);
-    hipEvent_t stop = nullptr;
-    int rc = timing_begin(s, &stop);
+    int rc = launch_list_build(s);
     if (rc) return rc;
+    hipEvent_t stop = nullptr;
+    if ((rc = timing_begin(s, &stop))) return rc;
     const bool mixed = (s->p.precision == YK_MIXED);
     switch (s->p.sim_type) {
     case YK_PURE_ION:
@@ -326,9 +421,14 @@ int launch_pairs(yk_system *s, bool meas)
 template <int KIND, bool MEAS, int MODE>
 void launch_fin_k(yk_system *s, const StepConsts &sc)
 {
-    hipLaunchKernelGGL((k_finalize<KIND, MEAS, MODE>), dim3(s->nblocks), dim3(BLOCK), 0,
-                       s->stream, s->part, s->nchunks, s->pos, s->den, s->aii, s->orig,
-                       s->acc, s->vel, s->vold, sc, s->blockpart);
+    if (s->use_list)
+        hipLaunchKernelGGL((k_finalize<KIND, MEAS, MODE, 1>), dim3(s->nblocks), dim3(BLOCK), 0,
+                           s->stream, s->part, s->list_M, s->pos, s->den, s->aii, s->orig,
+                           s->acc, s->vel, s->vold, sc, s->blockpart);
+    else
+        hipLaunchKernelGGL((k_finalize<KIND, MEAS, MODE, 0>), dim3(s->nblocks), dim3(BLOCK), 0,
+                           s->stream, s->part, s->nchunks, s->pos, s->den, s->aii, s->orig,
+                           s->acc, s->vel, s->vold, sc, s->blockpart);
 }
 
 template <int KIND>
@@ -367,12 +467,13 @@ int launch_finalize(yk_system *s, double dt, bool meas, int mode, int slot)
 int launch_drift(yk_system *s, double dt)
 {
     StepConsts sc = step_consts(s, dt);
+    const double4 *xr = s->use_list ? s->xref : nullptr;
     if (s->p.flags & YK_FLAG_KDK)
         hipLaunchKernelGGL(k_drift<2>, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos,
-                           s->vel, s->vold, s->acc, sc, s->ntiles, s->box);
+                           s->vel, s->vold, s->acc, sc, s->ntiles, s->box, xr, s->maxdisp);
     else
         hipLaunchKernelGGL(k_drift<1>, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos,
-                           s->vel, s->vold, s->acc, sc, s->ntiles, s->box);
+                           s->vel, s->vold, s->acc, sc, s->ntiles, s->box, xr, s->maxdisp);
     YK_HIP(hipGetLastError());
     return YK_OK;
 }
@@ -419,7 +520,7 @@ int resort(yk_system *s)
     hipLaunchKernelGGL(k_tile_boxes, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos, n,
                        s->ntiles, s->box);
     YK_HIP(hipGetLastError());
-    return YK_OK;
+    return invalidate_list(s);   // slots moved: the lists refer to old slots
 }
 
 // One step on the device: (periodic re-sort), drift, pairs, finalize
@@ -449,7 +550,9 @@ void release(yk_system *s)
                       (void *)s->aii, (void *)s->aii2, (void *)s->den, (void *)s->den2,
                       (void *)s->orig, (void *)s->orig2, (void *)s->box, (void *)s->part,
                       (void *)s->blockpart, (void *)s->d_meas, (void *)s->keys,
-                      (void *)s->keys2, (void *)s->vals, (void *)s->vals2, s->sort_tmp})
+                      (void *)s->keys2, (void *)s->vals, (void *)s->vals2, s->sort_tmp,
+                      (void *)s->list, (void *)s->counts, (void *)s->xref,
+                      (void *)s->maxdisp, (void *)s->swept})
         if (ptr) (void)hipFree(ptr);
     if (s->h_stage) (void)hipHostFree(s->h_stage);
     if (s->h_orig) (void)hipHostFree(s->h_orig);
@@ -527,7 +630,21 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
     YK_HIPB(hipMalloc(&s->orig, sizeof(int) * nn));
     YK_HIPB(hipMalloc(&s->orig2, sizeof(int) * nn));
     YK_HIPB(hipMalloc(&s->box, sizeof(TileBox) * (size_t)s->ntiles));
-    YK_HIPB(hipMalloc(&s->part, sizeof(double) * 11 * nn * (size_t)s->nchunks));
+    s->use_list = !(params->flags & YK_FLAG_ALLPAIRS);
+    s->skin = 0.05 * list_cutoff(*params);
+    if (!(s->skin < 0.05 * params->xl[0])) s->skin = 0.0;   // huge cutoff: every pair listed
+    s->part_doubles = 11 * nn * (size_t)s->nchunks;
+    if (s->use_list) {
+        s->part_doubles = std::max(s->part_doubles,
+                                   (size_t)s->ntiles * s->list_M * 11 * TILE);
+        YK_HIPB(hipMalloc(&s->list, sizeof(unsigned) * (size_t)s->ntiles * nn));
+        YK_HIPB(hipMalloc(&s->counts, sizeof(int2) * (size_t)s->ntiles));
+        YK_HIPB(hipMalloc(&s->xref, sizeof(double4) * nn));
+    }
+    YK_HIPB(hipMalloc(&s->maxdisp, sizeof(unsigned long long)));
+    YK_HIPB(hipMalloc(&s->swept, sizeof(unsigned long long)));
+    YK_HIPB(hipMemsetAsync(s->swept, 0, sizeof(unsigned long long), s->stream));
+    YK_HIPB(hipMalloc(&s->part, sizeof(double) * s->part_doubles));
     YK_HIPB(hipMalloc(&s->blockpart, sizeof(double) * NQ * (size_t)s->nblocks));
     YK_HIPB(hipMalloc(&s->keys, sizeof(unsigned) * nn));
     YK_HIPB(hipMalloc(&s->keys2, sizeof(unsigned) * nn));
@@ -562,10 +679,14 @@ int yk_set_params(yk_system *s, const yk_params *params)
         return fail(YK_ERR_UNSUPPORTED, "pure-ion ai/xmass are fixed at yk_create");
     if (params->xmass != s->p.xmass)
         return fail(YK_ERR_UNSUPPORTED, "xmass is fixed at yk_create");
+    if ((params->flags & YK_FLAG_ALLPAIRS) != (s->p.flags & YK_FLAG_ALLPAIRS))
+        return fail(YK_ERR_UNSUPPORTED, "YK_FLAG_ALLPAIRS is fixed at yk_create");
+    if (list_cutoff(*params) + 0.0 != list_cutoff(s->p) && s->use_list)
+        s->skin = 0.05 * list_cutoff(*params);
     s->p = *params;
     if (params->resort_every > 0) s->resort_every = params->resort_every;
     s->have_accel = false;
-    return YK_OK;
+    return invalidate_list(s);
 }
 
 int yk_put_state(yk_system *s, const double *x, const double *v)
@@ -764,7 +885,10 @@ int yk_get_timing(yk_system *s, yk_timing *out)
     if (!s || !out) return fail(YK_ERR_ARG, "handle or out is NULL");
     int rc = timing_collect(s);
     if (rc) return rc;
+    unsigned long long sw = 0;
+    if ((rc = read_swept(s, &sw))) return rc;
     *out = s->timing;
+    out->pairs_swept = (int64_t)(sw - s->swept_base);
     return YK_OK;
 }
 
@@ -773,6 +897,7 @@ int yk_reset_timing(yk_system *s)
     if (!s) return fail(YK_ERR_ARG, "handle is NULL");
     int rc = timing_collect(s);
     if (rc) return rc;
+    if ((rc = read_swept(s, &s->swept_base))) return rc;
     s->timing = yk_timing{};
     return YK_OK;
 }

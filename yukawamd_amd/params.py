@@ -26,6 +26,7 @@ PRECISIONS = {"fp64": 0, "mixed": 1}
 FLAG_REF_F1 = 1 << 0
 FLAG_KDK = 1 << 1
 FLAG_TIMING = 1 << 2
+FLAG_ALLPAIRS = 1 << 3
 
 
 @dataclass
@@ -56,6 +57,7 @@ class SimParams:
     integrator: str = "vv2"             # 'vv2' (vel_verlet_2.f) | 'kdk' (vel_verlet_1.f)
     timing: bool = False
     resort_every: int = 0               # device re-sort cadence (0 = library default)
+    allpairs: bool = False              # AABB-culled tile pairs instead of pair lists
 
     def __post_init__(self):
         if self.sim_type not in SIM_TYPES:
@@ -84,6 +86,8 @@ class SimParams:
             f |= FLAG_KDK
         if self.timing:
             f |= FLAG_TIMING
+        if self.allpairs:
+            f |= FLAG_ALLPAIRS
         return f
 
     def replace(self, **kw) -> "SimParams":
