@@ -533,14 +533,21 @@ __device__ __forceinline__ unsigned spread3(unsigned v)
     return v;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_morton(const double4 *__restrict__ pos, int n,
+// Also wraps every position into [0, L) (the slots are about to move and
+// every pair list is rebuilt afterwards).
+__global__ __launch_bounds__(BLOCK) void k_morton(double4 *__restrict__ pos, int n,
                                                   double gx, double gy, double gz, int gmax,
+                                                  double Lx, double Ly, double Lz,
                                                   unsigned *__restrict__ key,
                                                   int *__restrict__ val)
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     double4 p = pos[i];
+    p.x -= Lx * __builtin_floor(p.x / Lx);
+    p.y -= Ly * __builtin_floor(p.y / Ly);
+    p.z -= Lz * __builtin_floor(p.z / Lz);
+    pos[i] = p;
     int cx = (int)(p.x * gx), cy = (int)(p.y * gy), cz = (int)(p.z * gz);
     cx = cx < 0 ? 0 : (cx > gmax ? gmax : cx);
     cy = cy < 0 ? 0 : (cy > gmax ? gmax : cy);
@@ -680,8 +687,9 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(
 }
 
 // Drift (+ velocity predictor or opening half kick), then the tile AABBs of
-// the new positions.  MODE 1: vel_verlet_2 (src/vel_verlet_2.f:33-53);
-// MODE 2: drift + half kick (src/vel_verlet_1.f:28-48).
+// the new positions and the largest displacement since the pair lists were
+// built.  MODE 1: vel_verlet_2 (src/vel_verlet_2.f:33-53); MODE 2: drift +
+// half kick (src/vel_verlet_1.f:28-48).
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_drift(double4 *__restrict__ pos,
                                                  double *__restrict__ vel,
@@ -711,15 +719,18 @@ __global__ __launch_bounds__(BLOCK) void k_drift(double4 *__restrict__ pos,
                 xk = x[k] + v * sc.dt + a * sc.afac;
                 vel[k * nn + i] = v + sc.hdt * a;
             }
-            if (xk < 0.0) xk = xk + sc.L[k];
-            if (xk > sc.L[k]) xk = xk - sc.L[k];
+            // No periodic wrap here: positions stay continuous between
+            // re-sorts so that the periodic image stored with every pair-list
+            // entry stays valid.  k_morton wraps them into the box at each
+            // re-sort and the host boundary wraps them on download with the
+            // reference's rule (src/vel_verlet_2.f:39-40).
             x[k] = xk;
         }
         pos[i] = make_double4(x[0], x[1], x[2], p.w);
     }
     const int tile = i / TILE;
     if (tile < ntiles) tile_box_store(x, valid, tile, box);
-    if (xref) {   // largest minimum-image displacement since the list build
+    if (xref) {   // largest displacement since the list build
         double d2 = 0.0;
         if (valid) {
             const double4 r = xref[i];
@@ -727,7 +738,6 @@ __global__ __launch_bounds__(BLOCK) void k_drift(double4 *__restrict__ pos,
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 double d = x[k] - xr[k];
-                d = __builtin_fma(-sc.L[k], __builtin_rint(d / sc.L[k]), d);
                 d2 = __builtin_fma(d, d, d2);
             }
         }

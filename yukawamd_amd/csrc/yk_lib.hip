@@ -101,6 +101,17 @@ struct yk_system {
 
 namespace {
 
+// Device positions are continuous between re-sorts; the caller sees them in
+// the box: the reference's single wrap (src/vel_verlet_2.f:39-40) when that
+// suffices, a floor-based wrap for anything farther out.
+double wrap_into_box(double x, double L)
+{
+    if (x < 0.0) x = x + L;
+    if (x > L) x = x - L;
+    if (x < 0.0 || x > L) x = x - L * std::floor(x / L);
+    return x;
+}
+
 bool finite3(const double *x, int n)
 {
     for (size_t k = 0; k < (size_t)3 * n; ++k)
@@ -500,8 +511,8 @@ int resort(yk_system *s)
     const int g = s->grid_bits;
     const double G = (double)(1 << g);
     hipLaunchKernelGGL(k_morton, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos, n,
-                       G / s->p.xl[0], G / s->p.xl[1], G / s->p.xl[2], (1 << g) - 1, s->keys,
-                       s->vals);
+                       G / s->p.xl[0], G / s->p.xl[1], G / s->p.xl[2], (1 << g) - 1,
+                       s->p.xl[0], s->p.xl[1], s->p.xl[2], s->keys, s->vals);
     YK_HIP(hipGetLastError());
     size_t bytes = s->sort_tmp_bytes;
     YK_HIP(hipcub::DeviceRadixSort::SortPairs(s->sort_tmp, bytes, s->keys, s->keys2, s->vals,
@@ -760,9 +771,8 @@ int yk_get_state(yk_system *s, double *x, double *v, double *a)
         YK_HIP(hipMemcpy(p4.data(), s->pos, sizeof(double4) * nn, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < nn; ++i) {
             double *xi = x + 3 * (size_t)o[i];
-            xi[0] = p4[i].x;
-            xi[1] = p4[i].y;
-            xi[2] = p4[i].z;
+            const double xs[3] = {p4[i].x, p4[i].y, p4[i].z};
+            for (int k = 0; k < 3; ++k) xi[k] = wrap_into_box(xs[k], s->p.xl[k]);
         }
     }
     double *h = s->h_stage;
