@@ -1,5 +1,5 @@
-﻿!mod$ v1 sum:ac8f6727cffd3342
-!need$ 160fde2b6f23a965 n yk_bind
+﻿!mod$ v1 sum:dbef63210b5b7c92
+!need$ fead504c5b658531 n yk_bind
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins
 module md_yk_shim_state
@@ -87,6 +87,7 @@ use yk_bind,only:yk_mixed
 use yk_bind,only:yk_flag_ref_f1
 use yk_bind,only:yk_flag_kdk
 use yk_bind,only:yk_flag_timing
+use yk_bind,only:yk_flag_allpairs
 use yk_bind,only:yk_ok
 use yk_bind,only:yk_params_t
 use yk_bind,only:yk_meas_t

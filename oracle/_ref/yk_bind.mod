@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:160fde2b6f23a965
+﻿!mod$ v1 sum:fead504c5b658531
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins
 module yk_bind
@@ -163,6 +163,7 @@ integer(4),parameter::yk_mixed=1_4
 integer(4),parameter::yk_flag_ref_f1=1_4
 integer(4),parameter::yk_flag_kdk=2_4
 integer(4),parameter::yk_flag_timing=4_4
+integer(4),parameter::yk_flag_allpairs=8_4
 integer(4),parameter::yk_ok=0_4
 type,bind(c)::yk_params_t
 integer(4)::sim_type=0_4

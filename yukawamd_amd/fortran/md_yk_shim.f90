@@ -15,7 +15,8 @@
 ! exact drop-in semantics.  A host that keeps the state on the device calls
 ! yk_run instead.  The mixture kernel's last-particle behaviour
 ! (src/int_ion_mix.f:158, :201, :250) is reproduced unless YK_FIX_F1=1 is set
-! in the environment; YK_PRECISION=mixed selects the mixed-precision kernel.
+! in the environment; YK_PRECISION=mixed selects the mixed-precision kernel and
+! YK_ALLPAIRS=1 the all-pairs tile kernel instead of the pair lists.
 !
 ! Compiles against the host's md_types / md_globals / md_comm modules and
 ! perf.h; errors stop the run through md_exit (src/md_utils.f:158-183).
@@ -62,6 +63,8 @@ contains
     p%flags = YK_FLAG_REF_F1
     call get_environment_variable('YK_FIX_F1', env, ln, st)
     if (st == 0 .and. env(1:1) == '1') p%flags = 0
+    call get_environment_variable('YK_ALLPAIRS', env, ln, st)
+    if (st == 0 .and. env(1:1) == '1') p%flags = ior(p%flags, YK_FLAG_ALLPAIRS)
     p%coulomb_on = merge(1, 0, coulomb == 'screened-coulomb')
     p%nuclear_on = merge(1, 0, nuclear == 'HPP')
     p%resort_every = 0
