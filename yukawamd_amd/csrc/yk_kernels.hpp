@@ -102,12 +102,15 @@ __device__ __forceinline__ double min_image(double d, double L, double invL)
 // Pair interaction given the minimum-image displacement d = x_i - x_j.
 // PREC 0 = FP64, 1 = mixed (FP64 displacement, r^2, cutoff test and
 // accumulation; FP32 1/r, exp and Gaussians).
-template <int KIND, int PREC, bool MEAS>
+// SELF_BY_R2: the self pair is recognised by r^2 == 0 instead of the caller's
+// flag (pair-list kernel: a source is its own target exactly when d = 0).
+template <int KIND, int PREC, bool MEAS, bool SELF_BY_R2 = false>
 __device__ __forceinline__ void interact(const ForceConsts &c, double zi, double zj,
                                          double dx, double dy, double dz, bool self,
                                          Acc<MEAS> &acc)
 {
     double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+    if constexpr (SELF_BY_R2) self = !(r2 > 0.0);
     if constexpr (KIND != NN) {
         if (r2 <= c.rc2 && !self) {
             double s, phi = 0.0;
@@ -394,8 +397,8 @@ __global__ __launch_bounds__(BLOCK) void k_list_build(const double4 *__restrict_
 // covering an equal share of the tile's list.  Sources of each batch of 64
 // entries are gathered by the lanes into the wave's own LDS slice, already
 // shifted to their image, then read as broadcasts.  Self pairs are the
-// pairs with d = 0 (the source is the target itself); coincident distinct
-// particles are skipped the same way.
+// pairs with r^2 = 0 (the source is the target itself); coincident distinct
+// particles, where the reference divides by zero, are skipped the same way.
 template <int KIND, int PREC, bool MEAS>
 __global__ __launch_bounds__(BLOCK) void k_pairs_list(const double4 *__restrict__ pos,
                                                       const unsigned *__restrict__ list,
@@ -427,42 +430,54 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_list(const double4 *__restrict_
     Acc<MEAS> acc;
     double4 *sw = sj[wv];
     if (lane == 0 && hi > lo) atomicAdd(swept, (unsigned long long)(hi - lo) * TILE);
+    // Software pipeline over batches of 64 entries: while batch k is computed
+    // from LDS, the positions of batch k+1 and the entries of batch k+2 are
+    // in flight in registers.
+    auto entry_at = [&](int v) -> unsigned {
+        return (v < hi) ? ((v < nf) ? mine[v] : mine[cap - 1 - (v - nf)]) : 0u;
+    };
+    unsigned e_cur = entry_at(lo + lane);
+    double4 pj_cur = pos[e_cur & SLOT_MASK];
+    unsigned e_nxt = entry_at(lo + TILE + lane);
     for (int v0 = lo; v0 < hi; v0 += TILE) {
         const int v = v0 + lane;
         if (v < hi) {
-            const unsigned e = (v < nf) ? mine[v] : mine[cap - 1 - (v - nf)];
-            double4 pj = pos[e & SLOT_MASK];
+            double4 pj = pj_cur;
             if (v < nf) {   // apply the entry's periodic image
-                pj.x += c.L[0] * (double)((int)((e >> 26) & 3u) - 1);
-                pj.y += c.L[1] * (double)((int)((e >> 28) & 3u) - 1);
-                pj.z += c.L[2] * (double)((int)((e >> 30) & 3u) - 1);
+                pj.x += c.L[0] * (double)((int)((e_cur >> 26) & 3u) - 1);
+                pj.y += c.L[1] * (double)((int)((e_cur >> 28) & 3u) - 1);
+                pj.z += c.L[2] * (double)((int)((e_cur >> 30) & 3u) - 1);
             }
             sw[lane] = pj;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // issue the next batch's loads before computing this one
+        e_cur = e_nxt;
+        pj_cur = pos[e_cur & SLOT_MASK];
+        e_nxt = entry_at(v0 + 2 * TILE + lane);
         const int cntb = min(TILE, hi - v0);
         const int nfast = max(0, min(cntb, nf - v0));
+        // the next source is read from LDS before the current pair is
+        // computed, so the LDS latency overlaps the arithmetic
+        double4 pn = sw[0];
 #pragma unroll 4
         for (int jj = 0; jj < nfast; ++jj) {
-            const double4 pj = sw[jj];
-            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const bool self = (dx == 0.0) & (dy == 0.0) & (dz == 0.0);
-            interact<KIND, PREC, MEAS>(c, pi.w, pj.w, dx, dy, dz, self, acc);
+            const double4 pj = pn;
+            pn = sw[min(jj + 1, TILE - 1)];
+            interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w, pi.x - pj.x, pi.y - pj.y,
+                                             pi.z - pj.z, false, acc);
         }
 #pragma unroll 2
         for (int jj = nfast; jj < cntb; ++jj) {
-            const double4 pj = sw[jj];
-            const double dx = min_image(pi.x - pj.x, c.L[0], c.invL[0]);
-            const double dy = min_image(pi.y - pj.y, c.L[1], c.invL[1]);
-            const double dz = min_image(pi.z - pj.z, c.L[2], c.invL[2]);
-            const bool self = (dx == 0.0) & (dy == 0.0) & (dz == 0.0);
-            interact<KIND, PREC, MEAS>(c, pi.w, pj.w, dx, dy, dz, self, acc);
+            const double4 pj = pn;
+            pn = sw[min(jj + 1, TILE - 1)];
+            interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w,
+                                             min_image(pi.x - pj.x, c.L[0], c.invL[0]),
+                                             min_image(pi.y - pj.y, c.L[1], c.invL[1]),
+                                             min_image(pi.z - pj.z, c.L[2], c.invL[2]),
+                                             false, acc);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     constexpr int NC = MEAS ? 11 : 3;
     double *o = part + ((size_t)t * M + m) * NC * TILE + lane;

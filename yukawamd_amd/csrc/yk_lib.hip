@@ -63,7 +63,7 @@ struct yk_system {
     TileBox *box = nullptr;                  // [ntiles]
     // cluster-pair lists
     bool use_list = true;
-    int list_M = 16;                         // work items per target tile
+    int list_M = 64;                         // work items per target tile
     double skin = 0.0;
     unsigned *list = nullptr;                // [ntiles][n]
     int2 *counts = nullptr;                  // [ntiles] (fast, general)
