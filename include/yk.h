@@ -185,6 +185,61 @@ int yk_ttot(yk_system *sys, double *eka);
 int yk_get_timing(yk_system *sys, yk_timing *out);
 int yk_reset_timing(yk_system *sys);
 
+/* Batched ensembles and continuous batching ------------------------------
+ *
+ * No counterpart in the reference, which runs one system per process.  A
+ * yk_batch holds many independent ion systems (pure or mixture, FP64 or
+ * mixed precision, each with its own N, box, screening, cutoff, timestep and
+ * step budget) in one device arena and advances them together: one
+ * persistent pair launch per step pulls (replica, target tile, source range)
+ * work items from a device queue, and the step is replayed from a captured
+ * hipGraph.  Replicas may be added and removed between runs; admission is by
+ * arena footprint (yk_batch_footprint) and fails with YK_ERR_CAPACITY when
+ * the arena or the replica table is full.  A replica's results do not depend
+ * on the other replicas in the arena.  Integrator: vel_verlet_2
+ * (src/vel_verlet_2.f:18-120) without a B field. */
+typedef struct yk_batch yk_batch;
+
+const char *yk_batch_last_error(void);
+
+/* capacity: particle slots in the arena (rounded up to 256); max_records:
+ * measurement records kept per replica. */
+int yk_batch_create(int64_t capacity, int32_t max_replicas, int32_t max_records,
+                    yk_batch **out);
+void yk_batch_destroy(yk_batch *b);
+
+/* Arena slots a replica of n particles occupies. */
+int yk_batch_footprint(int32_t n, int64_t *slots);
+
+/* Admit a replica: positions/velocities in (3,n) layout (v may be NULL),
+ * zii/aii as in yk_create (NULL allowed for pure-ion), its timestep and its
+ * step budget.  Accelerations of the initial positions are computed at the
+ * next yk_batch_run (the initial record, yk_batch_get's init). */
+int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *zii,
+                 const double *aii, const double *x, const double *v, double dt,
+                 int32_t steps, int32_t *id);
+int yk_batch_remove(yk_batch *b, int32_t id);
+
+/* Source tiles per work item for replicas with n <= n_upper[k] (first match),
+ * and the waves per workgroup of the persistent launch. */
+int yk_batch_set_tuning(yk_batch *b, int32_t nbuckets, const int32_t *n_upper,
+                        const int32_t *chunk_tiles, int32_t waves_per_block);
+
+/* Advance every replica by up to nsteps (a replica stops when its budget is
+ * used up), measuring every nind-th step (src/md.F:129 cadence). */
+int yk_batch_run(yk_batch *b, int32_t nsteps, int32_t nind);
+
+int yk_batch_status(yk_batch *b, int32_t id, int32_t *steps_left, int32_t *nrec);
+
+/* State of one replica in its caller order, the initial measurement and up
+ * to rec_cap records; any pointer may be NULL. */
+int yk_batch_get(yk_batch *b, int32_t id, double *x, double *v, double *a, yk_meas *init,
+                 yk_meas *records, int32_t rec_cap, int32_t *nrec);
+
+/* Device time (HIP events on the batch's stream) of the last yk_batch_run,
+ * its step count and the number of work items per step. */
+int yk_batch_last_run(yk_batch *b, double *ms, int64_t *steps, int64_t *items);
+
 #ifdef __cplusplus
 }
 #endif

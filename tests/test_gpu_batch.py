@@ -1,0 +1,112 @@
+"""Batched ensembles (yk_batch_*) on the GPU, through the C ABI.
+
+* FP64 replicas of different kinds and sizes in one arena reproduce the
+  oracle's accelerations (condition-aware 1e-12) and energies (1e-13);
+* a replica gives bitwise the same trajectory and records alone and inside a
+  mix of other replicas;
+* the batch matches the single-system path to summation-order rounding;
+* step budgets, removal/re-admission and the capacity limit behave.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from yukawamd_amd import workloads as W
+from yukawamd_amd._lib import YkError
+from yukawamd_amd.batch import Ensemble
+from yukawamd_amd.system import YukawaSystem
+
+pytestmark = pytest.mark.gpu
+
+
+def small_set():
+    return [W.replica(11, 300, False, 175.0, 1.0, "fp64"),
+            W.replica(12, 500, True, 160.0, 0.7, "fp64"),
+            W.replica(13, 777, True, 210.0, 1.8, "fp64")]
+
+
+def test_batch_fp64_accel_matches_oracle():
+    reps = small_set()
+    with Ensemble(capacity=4096, max_replicas=8) as e:
+        ids = [e.add_workload(w, steps=0) for w in reps]
+        e.run(0)
+        for rid, w in zip(ids, reps):
+            g = e.get(rid)
+            a_o, m_o = oracle.accel(w.params, w.x, w.Z, w.A, True, ref_f1=0)
+            scale = np.abs(a_o).max()
+            err = np.abs(g["a"] - a_o).max() / scale
+            print(w.name, w.n, "max |da|/max|a|", err)
+            assert err <= 1e-12
+            assert g["init"]["ev"] == pytest.approx(m_o["ev"], rel=1e-13)
+            assert g["init"]["px"] == pytest.approx(m_o["px"], rel=1e-13)
+
+
+def test_replica_alone_equals_in_mix():
+    target = W.replica(21, 1500, True, 180.0, 0.9, "mixed")
+    others = [W.replica(22 + k, n, k % 2 == 0, 170.0, 1.3, "mixed")
+              for k, n in enumerate([3000, 700, 2100, 1100])]
+    results = []
+    for company in ([], others):
+        with Ensemble(capacity=16384, max_replicas=8, max_records=8) as e:
+            ids = [e.add_workload(w, steps=20) for w in company[:2]]
+            tid = e.add_workload(target, steps=20)
+            ids += [e.add_workload(w, steps=20) for w in company[2:]]
+            e.run(20, nind=10)
+            results.append(e.get(tid))
+    a, b = results
+    assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["v"], b["v"])
+    assert a["records"] == b["records"] and len(a["records"]) == 2
+
+
+def test_batch_matches_single_system():
+    w = W.replica(31, 2000, True, 190.0, 1.2, "fp64")
+    with Ensemble(capacity=4096, max_replicas=2, max_records=4) as e:
+        rid = e.add_workload(w, steps=10)
+        e.run(10, nind=5)
+        g = e.get(rid)
+    with YukawaSystem(w.params, Z=w.Z, A=w.A, dt=w.dt) as s:
+        s.put_state(w.x, w.v)
+        s.accel(True)
+        recs = s.run(10, nind=5)
+        x, v, a = s.get_state()
+    L = w.params.xl[0]
+    dx = np.abs(x - g["x"])
+    dx = np.minimum(dx, L - dx)
+    assert dx.max() <= 1e-10 * L
+    np.testing.assert_allclose(g["v"], v, rtol=1e-8, atol=1e-12 * np.abs(v).max())
+    for r1, r2 in zip(g["records"], recs):
+        assert r1["ev"] == pytest.approx(r2["ev"], rel=1e-11)
+        assert r1["ek"] == pytest.approx(r2["ek"], rel=1e-10)
+
+
+def test_step_budgets_and_readmission():
+    ws = [W.replica(41 + k, 400 + 100 * k, k == 1, 175.0, 1.0, "fp64") for k in range(3)]
+    budgets = [5, 12, 20]
+    with Ensemble(capacity=2048, max_replicas=4, max_records=4) as e:
+        ids = [e.add_workload(w, steps=s) for w, s in zip(ws, budgets)]
+        e.run(20)
+        for rid in ids:
+            assert e.status(rid)[0] == 0
+        first = e.get(ids[0])
+        # the same replica alone for exactly 5 steps
+        with Ensemble(capacity=1024, max_replicas=1) as e2:
+            r2 = e2.add_workload(ws[0], steps=5)
+            e2.run(5)
+            alone = e2.get(r2)
+        assert np.array_equal(first["x"], alone["x"])
+        # remove and admit again into the freed slots
+        e.remove(ids[1])
+        nid = e.add_workload(ws[1], steps=3)
+        e.run(3)
+        assert e.status(nid)[0] == 0
+
+
+def test_capacity_limit():
+    w = W.replica(51, 900, False, 175.0, 1.0, "mixed")
+    with Ensemble(capacity=1024, max_replicas=4) as e:
+        rid = e.add_workload(w, steps=1)
+        with pytest.raises(YkError) as err:
+            e.add_workload(w, steps=1)
+        assert err.value.code == 7
+        e.remove(rid)
+        e.add_workload(w, steps=1)

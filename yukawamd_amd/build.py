@@ -31,6 +31,7 @@ HIP_FLAGS = [
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
+    "-Wno-unused-function",
 ]
 
 
@@ -40,7 +41,7 @@ def _run(cmd, cwd=None):
 
 
 def build_library(verbose: bool = True) -> Path:
-    srcs = [CSRC / "yk_lib.hip"]
+    srcs = [CSRC / "yk_lib.hip", CSRC / "yk_batch.hip"]
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, *HIP_FLAGS, "-o", tmp, *srcs])
     os.replace(tmp, LIB)

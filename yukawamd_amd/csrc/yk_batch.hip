@@ -1,0 +1,675 @@
+// yk_batch.hip -- the batched ensemble engine behind yk_batch_* (include/yk.h).
+//
+// Many independent ion systems ("replicas": pure or mixture, FP64 or mixed,
+// each with its own N, box, screening length, cutoff, timestep and step
+// budget) live in one device arena and advance together: per step one drift
+// launch, one persistent pair launch pulling (replica, tile, source range)
+// items from a device queue, one finalize launch and one per-replica launch
+// (records, step countdown, queue reset).  The step sequence is captured
+// once as a hipGraph and replayed.  Replicas can be added and removed
+// between runs (continuous batching); a replica's arithmetic does not depend
+// on what else is in the arena.  Only the vel_verlet_2 integrator without a
+// B field is offered here (the single-system API has the rest).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/yk.h"
+#include "yk_batch_kernels.hpp"
+
+using namespace yk;
+
+namespace {
+
+thread_local std::string b_err;
+
+int bfail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    b_err = buf;
+    return code;
+}
+
+#define YB_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return bfail(e_ == hipErrorOutOfMemory ? YK_ERR_NOMEM : YK_ERR_HIP,         \
+                         "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                         __LINE__);                                                    \
+    } while (0)
+
+struct HostReplica {
+    bool active = false;
+    int n = 0, off = 0, nblocks = 0;   // slots [off, off + n), 256-slot blocks used
+    yk_params p{};
+    double dt = 0.0;
+    std::vector<int> orig;             // sorted slot k -> caller index
+};
+
+}  // namespace
+
+struct yk_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cap = 0;             // slots (multiple of 256)
+    int max_rep = 0, rec_cap = 0;
+    int cus = 256;
+    // device arena
+    double4 *pos = nullptr;
+    double *vel = nullptr, *vold = nullptr, *acc = nullptr, *aii = nullptr, *den = nullptr;
+    int *orig = nullptr, *rep = nullptr, *tile_rep = nullptr;
+    TileBox *box = nullptr;
+    RepConsts *reps = nullptr;
+    RepState *state = nullptr;
+    MeasOut *records = nullptr, *init_records = nullptr;
+    BatchItem *items = nullptr;
+    int2 *tile_items = nullptr;
+    double *part = nullptr;
+    size_t part_items_cap = 0;
+    double *blockpart = nullptr;
+    unsigned *queue = nullptr;
+    // host
+    std::vector<HostReplica> table;
+    std::vector<char> used;  // per 256-slot block
+    bool dirty = true;       // items / constants must be rebuilt
+    bool need_accel = true;  // accelerations of new replicas are missing
+    int nitems = 0;
+    std::vector<int> tune_nupper, tune_chunk;
+    int waves = 4;           // waves per workgroup of the persistent launch
+    // captured step graph (valid while !dirty)
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int g_steps = 0, g_nind = -1;
+    // timing of the last run
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double last_ms = 0.0;
+    int64_t last_steps = 0;
+    int64_t last_pairs = 0;
+};
+
+namespace {
+
+void drop_graph(yk_batch *b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->g_steps = 0;
+    b->g_nind = -1;
+}
+
+int chunk_tiles_for(const yk_batch *b, int n)
+{
+    const int nt = (n + TILE - 1) / TILE;
+    for (size_t k = 0; k < b->tune_nupper.size(); ++k)
+        if (n <= b->tune_nupper[k]) return std::max(1, std::min(nt, b->tune_chunk[k]));
+    return std::max(1, (nt + 3) / 4);   // default: four source ranges per target tile
+}
+
+RepConsts rep_consts(const HostReplica &h)
+{
+    const yk_params &p = h.p;
+    RepConsts c{};
+    double lmax = std::max(p.xl[0], std::max(p.xl[1], p.xl[2]));
+    c.margin = 1e-12 * lmax;
+    for (int k = 0; k < 3; ++k) {
+        c.L[k] = p.xl[k];
+        c.invL[k] = 1.0 / p.xl[k];
+        c.halfL_m[k] = 0.5 * p.xl[k] - 4.0 * c.margin;
+    }
+    c.rc2 = p.rccut * p.rccut;
+    c.cull2 = c.rc2 * (1.0 + 1e-12) + c.margin;
+    c.mu = p.xmuc;
+    double hd = std::sqrt(p.xl[0] * p.xl[0] + p.xl[1] * p.xl[1] + p.xl[2] * p.xl[2]) / 2.0;
+    c.evcut = (p.rccut <= hd) ? std::exp(-(p.xmuc * p.rccut)) / p.rccut : 0.0;
+    c.cnum = (p.sim_type == YK_PURE_ION) ? p.frp * p.zi * p.zi * p.vc : p.frp * p.vc;
+    c.dt = h.dt;
+    c.afac = 0.5 * (h.dt * h.dt);   // src/vel_verlet_2.f:33
+    c.rho = p.rho;
+    c.kT = p.kT;
+    c.xmass = p.xmass;
+    c.vol = p.xl[0] * p.xl[1] * p.xl[2];
+    c.muf = (float)p.xmuc;
+    c.mul2ef = (float)(p.xmuc * 1.4426950408889634074);
+    c.three_n = 3.0f * (float)h.n;
+    c.n_f = (float)h.n;
+    c.kind = (p.sim_type == YK_PURE_ION) ? PURE : MIX;
+    c.prec = p.precision == YK_MIXED ? 1 : 0;
+    c.off = h.off;
+    c.n = h.n;
+    c.tile0 = h.off / TILE;
+    c.ntiles = (h.n + TILE - 1) / TILE;
+    c.blk0 = h.off / BLOCK;
+    c.nblk = (h.n + BLOCK - 1) / BLOCK;
+    c.ref_f1 = (p.flags & YK_FLAG_REF_F1) ? 1 : 0;
+    return c;
+}
+
+__global__ void k_fill_rep(int *rep, const int *tile_rep, const RepConsts *reps, int cap)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= cap) return;
+    const int r = tile_rep[i / TILE];
+    rep[i] = (r >= 0 && i - reps[r].off < reps[r].n) ? r : -1;
+}
+
+// Items, tile item ranges, per-slot replica index and replica constants for
+// the current membership.
+int rebuild(yk_batch *b)
+{
+    drop_graph(b);
+    const int ntiles_cap = b->cap / TILE;
+    std::vector<RepConsts> rc(b->max_rep);
+    std::vector<int> tile_rep(ntiles_cap, -1);
+    std::vector<int2> tile_items(ntiles_cap, make_int2(0, 0));
+    std::vector<BatchItem> items;
+    std::vector<int> order;
+    for (int r = 0; r < b->max_rep; ++r) {
+        const HostReplica &h = b->table[r];
+        if (h.active) {
+            rc[r] = rep_consts(h);
+            order.push_back(r);
+        } else {
+            rc[r] = RepConsts{};
+        }
+    }
+    // largest replicas first: their items start early and the queue's tail
+    // is made of small items
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int c) { return b->table[a].n > b->table[c].n; });
+    for (int r : order) {
+        const HostReplica &h = b->table[r];
+        const int t0 = h.off / TILE, nt = (h.n + TILE - 1) / TILE;
+        const int C = chunk_tiles_for(b, h.n);
+        for (int t = t0; t < t0 + nt; ++t) {
+            tile_rep[t] = r;
+            tile_items[t].x = (int)items.size();
+            for (int s = t0; s < t0 + nt; s += C)
+                items.push_back(BatchItem{r, t, s, std::min(s + C, t0 + nt)});
+            tile_items[t].y = (int)items.size() - tile_items[t].x;
+        }
+    }
+    b->nitems = (int)items.size();
+    if ((size_t)b->nitems > b->part_items_cap) {
+        if (b->part) YB_HIP(hipFree(b->part));
+        if (b->items) YB_HIP(hipFree(b->items));
+        b->part = nullptr;
+        b->items = nullptr;
+        size_t want = std::max<size_t>((size_t)b->nitems + b->nitems / 4, 1024);
+        YB_HIP(hipMalloc(&b->part, sizeof(double) * 11 * TILE * want));
+        YB_HIP(hipMalloc(&b->items, sizeof(BatchItem) * want));
+        b->part_items_cap = want;
+    }
+    YB_HIP(hipMemcpyAsync(b->reps, rc.data(), sizeof(RepConsts) * b->max_rep,
+                          hipMemcpyHostToDevice, b->stream));
+    YB_HIP(hipMemcpyAsync(b->tile_rep, tile_rep.data(), sizeof(int) * ntiles_cap,
+                          hipMemcpyHostToDevice, b->stream));
+    YB_HIP(hipMemcpyAsync(b->tile_items, tile_items.data(), sizeof(int2) * ntiles_cap,
+                          hipMemcpyHostToDevice, b->stream));
+    if (b->nitems)
+        YB_HIP(hipMemcpyAsync(b->items, items.data(), sizeof(BatchItem) * b->nitems,
+                              hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(k_fill_rep, dim3((b->cap + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, b->stream,
+                       b->rep, b->tile_rep, b->reps, b->cap);
+    YB_HIP(hipGetLastError());
+    YB_HIP(hipStreamSynchronize(b->stream));
+    b->dirty = false;
+    return YK_OK;
+}
+
+int persistent_blocks(const yk_batch *b)
+{
+    // fill every CU with 32 waves (8 per SIMD), never more blocks than items
+    int per_cu = std::max(1, 32 / b->waves);
+    int blocks = b->cus * per_cu;
+    int need = (b->nitems + b->waves - 1) / b->waves;
+    return std::max(1, std::min(blocks, need));
+}
+
+// One step (MODE 1) or the accel pass (MODE 0), stream-ordered, graph-safe:
+// no memsets, the queue counter is reset by the per-replica kernel.
+template <bool MEAS, int MODE>
+int enqueue_step(yk_batch *b)
+{
+    const dim3 slots((b->cap + BLOCK - 1) / BLOCK);
+    const ExpCoeffs ec = exp_coeffs();
+    hipLaunchKernelGGL((k_batch_drift<MODE == 1>), slots, dim3(BLOCK), 0, b->stream, b->pos, b->vel,
+                           b->vold, b->acc, b->rep, b->reps, b->state, b->cap, b->box);
+    const size_t lds = sizeof(double4) * TILE * b->waves;
+    hipLaunchKernelGGL((k_batch_pairs<MEAS>), dim3(persistent_blocks(b)), dim3(TILE * b->waves),
+                       lds, b->stream, b->pos, b->box, b->reps, b->state, b->items, b->nitems,
+                       b->queue, ec, MODE == 0 ? 1 : 0, b->part);
+    hipLaunchKernelGGL((k_batch_finalize<MEAS, MODE>), slots, dim3(BLOCK), 0, b->stream, b->part,
+                       b->tile_items, b->pos, b->den, b->aii, b->orig, b->rep, b->reps,
+                       b->state, b->cap, b->acc, b->vel, b->vold, b->blockpart);
+    hipLaunchKernelGGL((k_batch_replica<MEAS, MODE>), dim3(b->max_rep), dim3(BLOCK), 0,
+                       b->stream, b->blockpart, b->reps, b->state, b->max_rep, b->records,
+                       b->rec_cap, b->init_records, b->queue);
+    YB_HIP(hipGetLastError());
+    return YK_OK;
+}
+
+// Accelerations (and the initial measurement) of every active replica.
+int accel_pass(yk_batch *b)
+{
+    // the queue counter is zero here: it starts zeroed and every step's
+    // per-replica kernel resets it for the next launch
+    int rc = enqueue_step<true, 0>(b);
+    if (rc) return rc;
+    YB_HIP(hipStreamSynchronize(b->stream));
+    b->need_accel = false;
+    return YK_OK;
+}
+
+// Capture `steps` steps (measuring every nind-th) as one graph.
+int capture(yk_batch *b, int steps, int nind)
+{
+    drop_graph(b);
+    YB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = YK_OK;
+    for (int s = 1; s <= steps && rc == YK_OK; ++s) {
+        const bool meas = nind > 0 && (s % nind == 0);
+        rc = meas ? enqueue_step<true, 1>(b) : enqueue_step<false, 1>(b);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return bfail(YK_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    b->graph = g;
+    YB_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    b->g_steps = steps;
+    b->g_nind = nind;
+    return YK_OK;
+}
+
+// Morton order of one replica's particles (host; done once per admission).
+std::vector<int> morton_order(const double *x, int n, const double *L)
+{
+    int g = 1;
+    while (g < 10 && (double)(1 << (3 * g)) < 0.5 * n) ++g;
+    const double G = (double)(1 << g);
+    auto spread = [](unsigned v) {
+        v &= 0x3ffu;
+        v = (v | (v << 16)) & 0x030000ffu;
+        v = (v | (v << 8)) & 0x0300f00fu;
+        v = (v | (v << 4)) & 0x030c30c3u;
+        v = (v | (v << 2)) & 0x09249249u;
+        return v;
+    };
+    std::vector<unsigned> key(n);
+    for (int i = 0; i < n; ++i) {
+        unsigned c[3];
+        for (int k = 0; k < 3; ++k) {
+            double xk = x[3 * (size_t)i + k];
+            xk -= L[k] * std::floor(xk / L[k]);
+            int ci = (int)(xk * G / L[k]);
+            ci = ci < 0 ? 0 : (ci > (1 << g) - 1 ? (1 << g) - 1 : ci);
+            c[k] = (unsigned)ci;
+        }
+        key[i] = spread(c[0]) | (spread(c[1]) << 1) | (spread(c[2]) << 2);
+    }
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return key[a] < key[c]; });
+    return order;
+}
+
+int check_replica_params(const yk_params *p, int n)
+{
+    if (!p) return bfail(YK_ERR_ARG, "params is NULL");
+    if (p->sim_type != YK_PURE_ION && p->sim_type != YK_ION_MIXTURE)
+        return bfail(YK_ERR_UNSUPPORTED, "batched replicas are ion systems (pure or mixture)");
+    if (p->precision != YK_FP64 && p->precision != YK_MIXED)
+        return bfail(YK_ERR_ARG, "precision=%d", p->precision);
+    if (p->bfield != 0.0 || (p->flags & YK_FLAG_KDK))
+        return bfail(YK_ERR_UNSUPPORTED, "batched replicas use vel_verlet_2 without a B field");
+    for (int k = 0; k < 3; ++k)
+        if (!(p->xl[k] > 0.0)) return bfail(YK_ERR_ARG, "xl(%d)=%g", k + 1, p->xl[k]);
+    if (!(p->rccut > 0.0) || !(p->xmass > 0.0) || p->xmuc < 0.0)
+        return bfail(YK_ERR_ARG, "rccut, xmass must be > 0 and xmuc >= 0");
+    if (p->sim_type == YK_PURE_ION && !(p->ai > 0.0))
+        return bfail(YK_ERR_ARG, "pure-ion needs ai > 0");
+    if (n < 1) return bfail(YK_ERR_ARG, "n=%d must be >= 1", n);
+    return YK_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char *yk_batch_last_error(void) { return b_err.c_str(); }
+
+int yk_batch_create(int64_t capacity, int32_t max_replicas, int32_t max_records,
+                    yk_batch **out)
+{
+    if (!out) return bfail(YK_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (capacity < 1 || capacity > ((int64_t)1 << 28))
+        return bfail(YK_ERR_ARG, "capacity=%lld slots out of range", (long long)capacity);
+    if (max_replicas < 1 || max_records < 0)
+        return bfail(YK_ERR_ARG, "max_replicas must be >= 1 and max_records >= 0");
+    yk_batch *b = new (std::nothrow) yk_batch();
+    if (!b) return bfail(YK_ERR_NOMEM, "host allocation failed");
+    b->cap = (int)((capacity + BLOCK - 1) / BLOCK * BLOCK);
+    b->max_rep = max_replicas;
+    b->rec_cap = std::max(1, (int)max_records);
+    b->table.resize(max_replicas);
+    b->used.assign(b->cap / BLOCK, 0);
+    auto bail = [&](int code) {
+        yk_batch_destroy(b);
+        return code;
+    };
+#define YB_HIPB(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return bail(bfail(e_ == hipErrorOutOfMemory ? YK_ERR_NOMEM : YK_ERR_HIP,    \
+                              "%s failed: %s", #call, hipGetErrorString(e_)));         \
+    } while (0)
+    YB_HIPB(hipGetDevice(&b->device));
+    hipDeviceProp_t prop;
+    YB_HIPB(hipGetDeviceProperties(&prop, b->device));
+    b->cus = prop.multiProcessorCount;
+    YB_HIPB(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    const size_t nn = (size_t)b->cap;
+    YB_HIPB(hipMalloc(&b->pos, sizeof(double4) * nn));
+    for (double **pp : {&b->vel, &b->vold, &b->acc})
+        YB_HIPB(hipMalloc(pp, sizeof(double) * 3 * nn));
+    YB_HIPB(hipMalloc(&b->aii, sizeof(double) * nn));
+    YB_HIPB(hipMalloc(&b->den, sizeof(double) * nn));
+    YB_HIPB(hipMalloc(&b->orig, sizeof(int) * nn));
+    YB_HIPB(hipMalloc(&b->rep, sizeof(int) * nn));
+    YB_HIPB(hipMalloc(&b->tile_rep, sizeof(int) * (nn / TILE)));
+    YB_HIPB(hipMalloc(&b->box, sizeof(TileBox) * (nn / TILE)));
+    YB_HIPB(hipMalloc(&b->tile_items, sizeof(int2) * (nn / TILE)));
+    YB_HIPB(hipMalloc(&b->blockpart, sizeof(double) * NQ * (nn / BLOCK)));
+    YB_HIPB(hipMalloc(&b->reps, sizeof(RepConsts) * (size_t)max_replicas));
+    YB_HIPB(hipMalloc(&b->state, sizeof(RepState) * (size_t)max_replicas));
+    YB_HIPB(hipMalloc(&b->records, sizeof(MeasOut) * (size_t)max_replicas * b->rec_cap));
+    YB_HIPB(hipMalloc(&b->init_records, sizeof(MeasOut) * (size_t)max_replicas));
+    YB_HIPB(hipMalloc(&b->queue, sizeof(unsigned)));
+    YB_HIPB(hipEventCreate(&b->e0));
+    YB_HIPB(hipEventCreate(&b->e1));
+    // empty arena: every slot unused, every replica idle
+    YB_HIPB(hipMemsetAsync(b->pos, 0, sizeof(double4) * nn, b->stream));
+    YB_HIPB(hipMemsetAsync(b->vel, 0, sizeof(double) * 3 * nn, b->stream));
+    YB_HIPB(hipMemsetAsync(b->vold, 0, sizeof(double) * 3 * nn, b->stream));
+    YB_HIPB(hipMemsetAsync(b->acc, 0, sizeof(double) * 3 * nn, b->stream));
+    YB_HIPB(hipMemsetAsync(b->state, 0, sizeof(RepState) * (size_t)max_replicas, b->stream));
+    YB_HIPB(hipMemsetAsync(b->queue, 0, sizeof(unsigned), b->stream));
+    std::vector<double> one(nn, 1.0);
+    YB_HIPB(hipMemcpyAsync(b->aii, one.data(), sizeof(double) * nn, hipMemcpyHostToDevice,
+                           b->stream));
+    YB_HIPB(hipMemcpyAsync(b->den, one.data(), sizeof(double) * nn, hipMemcpyHostToDevice,
+                           b->stream));
+    YB_HIPB(hipStreamSynchronize(b->stream));
+    int rc = rebuild(b);
+    if (rc) return bail(rc);
+#undef YB_HIPB
+    *out = b;
+    return YK_OK;
+}
+
+void yk_batch_destroy(yk_batch *b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (void *p : {(void *)b->pos, (void *)b->vel, (void *)b->vold, (void *)b->acc,
+                    (void *)b->aii, (void *)b->den, (void *)b->orig, (void *)b->rep,
+                    (void *)b->tile_rep, (void *)b->box, (void *)b->tile_items,
+                    (void *)b->blockpart, (void *)b->reps, (void *)b->state,
+                    (void *)b->records, (void *)b->init_records, (void *)b->queue,
+                    (void *)b->part, (void *)b->items})
+        if (p) (void)hipFree(p);
+    if (b->e0) (void)hipEventDestroy(b->e0);
+    if (b->e1) (void)hipEventDestroy(b->e1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int yk_batch_footprint(int32_t n, int64_t *slots_out)
+{
+    if (!slots_out || n < 1) return bfail(YK_ERR_ARG, "n must be >= 1");
+    *slots_out = ((int64_t)n + BLOCK - 1) / BLOCK * BLOCK;
+    return YK_OK;
+}
+
+int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *zii,
+                 const double *aii, const double *x, const double *v, double dt,
+                 int32_t steps, int32_t *id_out)
+{
+    if (!b || !id_out || !x) return bfail(YK_ERR_ARG, "batch, x or id_out is NULL");
+    int rc = check_replica_params(params, n);
+    if (rc) return rc;
+    if (params->sim_type == YK_ION_MIXTURE && (!zii || !aii))
+        return bfail(YK_ERR_ARG, "zii and aii are required for a mixture");
+    if (!std::isfinite(dt) || steps < 0) return bfail(YK_ERR_ARG, "dt or steps invalid");
+    int id = -1;
+    for (int r = 0; r < b->max_rep; ++r)
+        if (!b->table[r].active) {
+            id = r;
+            break;
+        }
+    if (id < 0) return bfail(YK_ERR_CAPACITY, "all %d replica slots are in use", b->max_rep);
+    const int nblk = (n + BLOCK - 1) / BLOCK;
+    int start = -1;
+    for (int s = 0, run = 0; s < (int)b->used.size(); ++s) {
+        run = b->used[s] ? 0 : run + 1;
+        if (run == nblk) {
+            start = s - nblk + 1;
+            break;
+        }
+    }
+    if (start < 0)
+        return bfail(YK_ERR_CAPACITY, "no room for %d particles in the arena", n);
+    for (size_t k = 0; k < (size_t)3 * n; ++k)
+        if (!std::isfinite(x[k]) || (v && !std::isfinite(v[k])))
+            return bfail(YK_ERR_NONFINITE, "positions or velocities contain NaN/Inf");
+    HostReplica &h = b->table[id];
+    h = HostReplica{};
+    h.p = *params;
+    h.n = n;
+    h.off = start * BLOCK;
+    h.nblocks = nblk;
+    h.dt = dt;
+    h.orig = morton_order(x, n, params->xl);
+    // sorted host staging
+    const size_t nb = (size_t)nblk * BLOCK;
+    std::vector<double4> p4(nb, make_double4(0, 0, 0, 0));
+    std::vector<double> vv(3 * nb, 0.0), aa(nb, 1.0), dd(nb, 1.0);
+    std::vector<int> oo(nb, -1);
+    for (int k = 0; k < n; ++k) {
+        const int i = h.orig[k];
+        const double zi = (params->sim_type == YK_PURE_ION) ? (zii ? zii[i] : params->zi) : zii[i];
+        const double ai = (params->sim_type == YK_PURE_ION) ? (aii ? aii[i] : params->ai) : aii[i];
+        if (!(ai > 0.0) || !std::isfinite(zi))
+            return bfail(YK_ERR_ARG, "particle %d: A=%g Z=%g", i, ai, zi);
+        double xs[3];
+        for (int c = 0; c < 3; ++c) {
+            double xc = x[3 * (size_t)i + c];
+            xs[c] = xc - params->xl[c] * std::floor(xc / params->xl[c]);
+        }
+        p4[k] = make_double4(xs[0], xs[1], xs[2], zi);
+        for (int c = 0; c < 3; ++c) vv[c * nb + k] = v ? v[3 * (size_t)i + c] : 0.0;
+        aa[k] = ai;
+        dd[k] = (params->sim_type == YK_PURE_ION) ? params->ai * params->xmass : ai * params->xmass;
+        oo[k] = i;
+    }
+    const size_t o = (size_t)h.off;
+    const size_t cap = (size_t)b->cap;
+    YB_HIP(hipMemcpyAsync(b->pos + o, p4.data(), sizeof(double4) * nb, hipMemcpyHostToDevice,
+                          b->stream));
+    for (int c = 0; c < 3; ++c)
+        YB_HIP(hipMemcpyAsync(b->vel + c * cap + o, vv.data() + c * nb, sizeof(double) * nb,
+                              hipMemcpyHostToDevice, b->stream));
+    YB_HIP(hipMemcpyAsync(b->aii + o, aa.data(), sizeof(double) * nb, hipMemcpyHostToDevice,
+                          b->stream));
+    YB_HIP(hipMemcpyAsync(b->den + o, dd.data(), sizeof(double) * nb, hipMemcpyHostToDevice,
+                          b->stream));
+    YB_HIP(hipMemcpyAsync(b->orig + o, oo.data(), sizeof(int) * nb, hipMemcpyHostToDevice,
+                          b->stream));
+    RepState st{steps, 0, 0, 0};
+    YB_HIP(hipMemcpyAsync(b->state + id, &st, sizeof st, hipMemcpyHostToDevice, b->stream));
+    YB_HIP(hipStreamSynchronize(b->stream));
+    for (int s = start; s < start + nblk; ++s) b->used[s] = 1;
+    h.active = true;
+    b->dirty = true;
+    b->need_accel = true;
+    *id_out = id;
+    return YK_OK;
+}
+
+int yk_batch_remove(yk_batch *b, int32_t id)
+{
+    if (!b || id < 0 || id >= b->max_rep || !b->table[id].active)
+        return bfail(YK_ERR_ARG, "no active replica %d", id);
+    HostReplica &h = b->table[id];
+    YB_HIP(hipStreamSynchronize(b->stream));
+    RepState st{0, 0, 0, 0};
+    YB_HIP(hipMemcpy(b->state + id, &st, sizeof st, hipMemcpyHostToDevice));
+    for (int s = h.off / BLOCK; s < h.off / BLOCK + h.nblocks; ++s) b->used[s] = 0;
+    h = HostReplica{};
+    b->dirty = true;
+    return YK_OK;
+}
+
+int yk_batch_set_tuning(yk_batch *b, int32_t nbuckets, const int32_t *n_upper,
+                        const int32_t *chunk_tiles, int32_t waves_per_block)
+{
+    if (!b || nbuckets < 0 || (nbuckets > 0 && (!n_upper || !chunk_tiles)))
+        return bfail(YK_ERR_ARG, "invalid tuning table");
+    if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4 &&
+        waves_per_block != 8)
+        return bfail(YK_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
+    b->tune_nupper.assign(n_upper, n_upper + nbuckets);
+    b->tune_chunk.assign(chunk_tiles, chunk_tiles + nbuckets);
+    for (int k = 0; k < nbuckets; ++k)
+        if (b->tune_chunk[k] < 1) return bfail(YK_ERR_ARG, "chunk_tiles must be >= 1");
+    b->waves = waves_per_block;
+    b->dirty = true;
+    return YK_OK;
+}
+
+int yk_batch_run(yk_batch *b, int32_t nsteps, int32_t nind)
+{
+    if (!b || nsteps < 0) return bfail(YK_ERR_ARG, "invalid arguments");
+    int rc;
+    if (b->dirty && (rc = rebuild(b))) return rc;
+    if (b->need_accel && (rc = accel_pass(b))) return rc;
+    b->last_steps = 0;
+    b->last_ms = 0.0;
+    if (nsteps == 0 || b->nitems == 0) return YK_OK;
+    // one graph holds a block of steps: nind steps when measuring, else up to 16
+    const int block = (nind > 0) ? nind : std::min(nsteps, 16);
+    if (!b->gexec || b->g_steps != block || b->g_nind != nind)
+        if ((rc = capture(b, block, nind))) return rc;
+    YB_HIP(hipEventRecord(b->e0, b->stream));
+    int done = 0;
+    for (; done + block <= nsteps; done += block) YB_HIP(hipGraphLaunch(b->gexec, b->stream));
+    for (; done < nsteps; ++done)   // remainder, never a measuring step when nind > 0
+        if ((rc = enqueue_step<false, 1>(b))) return rc;
+    YB_HIP(hipEventRecord(b->e1, b->stream));
+    YB_HIP(hipEventSynchronize(b->e1));
+    float ms = 0.f;
+    YB_HIP(hipEventElapsedTime(&ms, b->e0, b->e1));
+    b->last_ms = ms;
+    b->last_steps = nsteps;
+    return YK_OK;
+}
+
+int yk_batch_status(yk_batch *b, int32_t id, int32_t *steps_left, int32_t *nrec)
+{
+    if (!b || id < 0 || id >= b->max_rep || !b->table[id].active)
+        return bfail(YK_ERR_ARG, "no active replica %d", id);
+    RepState st;
+    YB_HIP(hipStreamSynchronize(b->stream));
+    YB_HIP(hipMemcpy(&st, b->state + id, sizeof st, hipMemcpyDeviceToHost));
+    if (steps_left) *steps_left = st.steps_left;
+    if (nrec) *nrec = std::min(st.rec_count, b->rec_cap);
+    return YK_OK;
+}
+
+int yk_batch_get(yk_batch *b, int32_t id, double *x, double *v, double *a, yk_meas *init,
+                 yk_meas *records, int32_t rec_cap, int32_t *nrec)
+{
+    if (!b || id < 0 || id >= b->max_rep || !b->table[id].active)
+        return bfail(YK_ERR_ARG, "no active replica %d", id);
+    const HostReplica &h = b->table[id];
+    YB_HIP(hipStreamSynchronize(b->stream));
+    const size_t o = (size_t)h.off, n = (size_t)h.n, cap = (size_t)b->cap;
+    if (x) {
+        std::vector<double4> p4(n);
+        YB_HIP(hipMemcpy(p4.data(), b->pos + o, sizeof(double4) * n, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < n; ++k) {
+            double *xi = x + 3 * (size_t)h.orig[k];
+            xi[0] = p4[k].x;
+            xi[1] = p4[k].y;
+            xi[2] = p4[k].z;
+        }
+    }
+    std::vector<double> t(n);
+    for (int which = 0; which < 2; ++which) {
+        double *dst = which == 0 ? v : a;
+        const double *src = which == 0 ? b->vel : b->acc;
+        if (!dst) continue;
+        for (int c = 0; c < 3; ++c) {
+            YB_HIP(hipMemcpy(t.data(), src + c * cap + o, sizeof(double) * n,
+                             hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < n; ++k) dst[3 * (size_t)h.orig[k] + c] = t[k];
+        }
+    }
+    auto conv = [](const MeasOut &m, yk_meas *d) {
+        d->ev = m.ev;
+        d->ek = m.ek;
+        d->px = m.px;
+        d->vir = m.vir;
+        for (int k = 0; k < 6; ++k) d->pp[k] = m.pp[k];
+    };
+    if (init) {
+        MeasOut m;
+        YB_HIP(hipMemcpy(&m, b->init_records + id, sizeof m, hipMemcpyDeviceToHost));
+        conv(m, init);
+    }
+    RepState st;
+    YB_HIP(hipMemcpy(&st, b->state + id, sizeof st, hipMemcpyDeviceToHost));
+    const int have = std::min(st.rec_count, b->rec_cap);
+    if (nrec) *nrec = have;
+    if (records && rec_cap > 0 && have > 0) {
+        const int k = std::min(have, (int)rec_cap);
+        std::vector<MeasOut> m(k);
+        YB_HIP(hipMemcpy(m.data(), b->records + (size_t)id * b->rec_cap,
+                         sizeof(MeasOut) * k, hipMemcpyDeviceToHost));
+        for (int q = 0; q < k; ++q) conv(m[q], &records[q]);
+    }
+    return YK_OK;
+}
+
+int yk_batch_last_run(yk_batch *b, double *ms, int64_t *steps, int64_t *items)
+{
+    if (!b) return bfail(YK_ERR_ARG, "batch is NULL");
+    if (ms) *ms = b->last_ms;
+    if (steps) *steps = b->last_steps;
+    if (items) *items = b->nitems;
+    return YK_OK;
+}
+
+}  // extern "C"

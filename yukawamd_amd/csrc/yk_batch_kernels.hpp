@@ -1,0 +1,344 @@
+// yk_batch_kernels.hpp -- kernels of the batched ensemble engine: many
+// independent ion replicas (pure or mixture, FP64 or mixed precision, each
+// with its own N, box, screening, cutoff and timestep) integrated together.
+//
+// Arena layout (one per engine): replica r owns slots [off_r, off_r + n_r)
+// with off_r a multiple of 256, so every 64-slot tile and every 256-slot
+// finalize block belongs to exactly one replica; unused slots carry rep = -1.
+// Per slot: pos (double4 {x,y,z,Z}), vel/vold/acc (component-major over the
+// arena capacity), aii, den, orig (caller index inside the replica).
+//
+// Pair forces: one persistent launch.  Work items (replica, target tile,
+// range of source tiles of the same replica) are pulled from a device queue
+// with an atomic counter; each wave owns one item at a time, stages source
+// tiles in its own LDS slice, culls them by AABB and takes the shift-only
+// minimum image when the two tiles allow it.  Every item writes its own
+// partial sums, and the finalize kernel adds a tile's items in item order,
+// so a replica's result does not depend on which wave ran which item nor on
+// the other replicas in the launch.
+#pragma once
+#include "yk_kernels.hpp"
+
+namespace yk {
+
+struct RepConsts {
+    double L[3], invL[3], halfL_m[3];
+    double rc2, mu, evcut, cnum, margin, cull2;
+    double dt, afac, rho, kT, xmass, vol;
+    float muf, mul2ef;
+    float three_n, n_f;
+    int kind, prec, off, n;
+    int tile0, ntiles, blk0, nblk;
+    int ref_f1, pad0;
+};
+
+struct RepState {            // device-updated per replica
+    int steps_left;          // newton steps still to do (0 = idle)
+    int rec_count;           // measurement records written
+    int pad0, pad1;
+};
+
+struct BatchItem {
+    int rep, tile, s0, s1;   // target tile and source tiles [s0, s1), global indices
+};
+
+// ---------------------------------------------------------------------------
+template <int KIND, int PREC, bool MEAS>
+__device__ __forceinline__ void batch_item(const RepConsts &rc, const ExpCoeffs &ec,
+                                           const double4 *__restrict__ pos,
+                                           const TileBox *__restrict__ box, const BatchItem &it,
+                                           double4 *__restrict__ sw, int lane,
+                                           double *__restrict__ out)
+{
+    ForceConsts c{};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        c.L[k] = rc.L[k];
+        c.invL[k] = rc.invL[k];
+        c.halfL_m[k] = rc.halfL_m[k];
+    }
+    c.rc2 = rc.rc2;
+    c.mu = rc.mu;
+    c.evcut = rc.evcut;
+    c.muf = rc.muf;
+    c.mul2ef = rc.mul2ef;
+    c.ec = ec;
+    const int t = it.tile;
+    const int i = t * TILE + lane;
+    const int ilocal = i - rc.off;
+    const double nanv = __builtin_nan("");
+    const double4 pi = (ilocal < rc.n) ? pos[i] : make_double4(nanv, nanv, nanv, 0.0);
+    double ic[3], ih[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ic[k] = uniform(box[t].c[k]);
+        ih[k] = uniform(box[t].h[k]);
+    }
+    Acc<MEAS> acc;
+    for (int st = it.s0; st < it.s1; ++st) {
+        double gap2 = 0.0, shift[3];
+        bool fast = true;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double cd = ic[k] - box[st].c[k];
+            double h = ih[k] + box[st].h[k] + rc.margin;
+            double s = c.L[k] * __builtin_rint(cd * c.invL[k]);
+            double cm = __builtin_fabs(cd - s);
+            double g = cm - h;
+            g = g > 0.0 ? g : 0.0;
+            gap2 = __builtin_fma(g, g, gap2);
+            fast = fast && (cm + h < c.halfL_m[k]);
+            shift[k] = s;
+        }
+        gap2 = uniform(gap2);
+        if (gap2 > rc.cull2) continue;
+        const int j = st * TILE + lane;
+        sw[lane] = (j - rc.off < rc.n) ? pos[j] : make_double4(nanv, nanv, nanv, 0.0);
+        __builtin_amdgcn_wave_barrier();
+        const int self_jj = (st == t) ? lane : -1;
+        if (uniform(fast ? 1.0 : 0.0) != 0.0) {
+            const double xs = pi.x - uniform(shift[0]);
+            const double ys = pi.y - uniform(shift[1]);
+            const double zs = pi.z - uniform(shift[2]);
+#pragma unroll 4
+            for (int jj = 0; jj < TILE; ++jj) {
+                const double4 pj = sw[jj];
+                interact<KIND, PREC, MEAS>(c, pi.w, pj.w, xs - pj.x, ys - pj.y, zs - pj.z,
+                                           jj == self_jj, acc);
+            }
+        } else {
+#pragma unroll 2
+            for (int jj = 0; jj < TILE; ++jj) {
+                const double4 pj = sw[jj];
+                interact<KIND, PREC, MEAS>(c, pi.w, pj.w,
+                                           min_image(pi.x - pj.x, c.L[0], c.invL[0]),
+                                           min_image(pi.y - pj.y, c.L[1], c.invL[1]),
+                                           min_image(pi.z - pj.z, c.L[2], c.invL[2]),
+                                           jj == self_jj, acc);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    double *o = out + lane;
+    o[0] = acc.fx; o[TILE] = acc.fy; o[2 * TILE] = acc.fz;
+    if constexpr (MEAS) {
+        o[3 * TILE] = acc.e; o[4 * TILE] = acc.w;
+        o[5 * TILE] = acc.pxx; o[6 * TILE] = acc.pxy; o[7 * TILE] = acc.pxz;
+        o[8 * TILE] = acc.pyy; o[9 * TILE] = acc.pyz; o[10 * TILE] = acc.pzz;
+    }
+}
+
+// Persistent: every wave loops pulling items until the queue is empty.
+// all_reps != 0 (the initial accel pass) evaluates every listed replica;
+// otherwise replicas whose step budget is used up are skipped.
+template <bool MEAS>
+__global__ void k_batch_pairs(const double4 *__restrict__ pos, const TileBox *__restrict__ box,
+                              const RepConsts *__restrict__ reps,
+                              const RepState *__restrict__ state,
+                              const BatchItem *__restrict__ items, int nitems,
+                              unsigned *__restrict__ queue, ExpCoeffs ec, int all_reps,
+                              double *__restrict__ part)
+{
+    extern __shared__ double4 sj_dyn[];
+    const int lane = threadIdx.x & (TILE - 1);
+    double4 *sw = sj_dyn + (threadIdx.x / TILE) * TILE;
+    constexpr int NC = MEAS ? 11 : 3;
+    for (;;) {
+        unsigned k = 0;
+        if (lane == 0) k = atomicAdd(queue, 1u);
+        k = (unsigned)__builtin_amdgcn_readfirstlane((int)__shfl(k, 0, 64));
+        if (k >= (unsigned)nitems) break;   // every wave reaches this exit
+        const BatchItem it = items[k];
+        double *out = part + (size_t)k * NC * TILE;
+        const RepConsts &rc = reps[it.rep];
+        if (!all_reps && state[it.rep].steps_left <= 0) {
+            // idle replica: leave zero partials
+            out[lane] = 0.0; out[TILE + lane] = 0.0; out[2 * TILE + lane] = 0.0;
+            if constexpr (MEAS)
+                for (int q = 3; q < NC; ++q) out[q * TILE + lane] = 0.0;
+            continue;
+        }
+        const int kind = __builtin_amdgcn_readfirstlane(rc.kind);
+        const int prec = __builtin_amdgcn_readfirstlane(rc.prec);
+        if (kind == PURE) {
+            if (prec == 0) batch_item<PURE, 0, MEAS>(rc, ec, pos, box, it, sw, lane, out);
+            else batch_item<PURE, 1, MEAS>(rc, ec, pos, box, it, sw, lane, out);
+        } else {
+            if (prec == 0) batch_item<MIX, 0, MEAS>(rc, ec, pos, box, it, sw, lane, out);
+            else batch_item<MIX, 1, MEAS>(rc, ec, pos, box, it, sw, lane, out);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-slot drift (vel_verlet_2 order, reference wrap) + tile AABBs.
+// MOVE = false only recomputes the tile AABBs (before the initial accel pass).
+template <bool MOVE>
+__global__ __launch_bounds__(BLOCK) void k_batch_drift(double4 *__restrict__ pos,
+                                                       double *__restrict__ vel,
+                                                       double *__restrict__ vold,
+                                                       const double *__restrict__ acc,
+                                                       const int *__restrict__ rep,
+                                                       const RepConsts *__restrict__ reps,
+                                                       const RepState *__restrict__ state,
+                                                       int cap, TileBox *__restrict__ box)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const size_t nn = (size_t)cap;
+    const int r = (i < cap) ? rep[i] : -1;
+    const bool valid = (r >= 0);
+    double x[3] = {0, 0, 0};
+    if (valid) {
+        const RepConsts &rc = reps[r];
+        double4 p = pos[i];
+        x[0] = p.x; x[1] = p.y; x[2] = p.z;
+        if (MOVE && state[r].steps_left > 0) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double v = vel[k * nn + i], a = acc[k * nn + i];
+                double xk = x[k] + v * rc.dt + rc.afac * a;
+                vold[k * nn + i] = v;
+                vel[k * nn + i] = v + rc.dt * a;
+                if (xk < 0.0) xk = xk + rc.L[k];        // src/vel_verlet_2.f:39-40
+                if (xk > rc.L[k]) xk = xk - rc.L[k];
+                x[k] = xk;
+            }
+            pos[i] = make_double4(x[0], x[1], x[2], p.w);
+        }
+    }
+    const int tile = i / TILE;
+    if (tile < cap / TILE) tile_box_store(x, valid, tile, box);
+}
+
+// Per-slot finalize: sum the tile's item partials in item order, scale,
+// corrector (MODE 1) and per-block measurement partials.
+template <bool MEAS, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_batch_finalize(
+    const double *__restrict__ part, const int2 *__restrict__ tile_items,
+    const double4 *__restrict__ pos, const double *__restrict__ den,
+    const double *__restrict__ aii, const int *__restrict__ orig, const int *__restrict__ rep,
+    const RepConsts *__restrict__ reps, const RepState *__restrict__ state, int cap,
+    double *__restrict__ acc, double *__restrict__ vel, const double *__restrict__ vold,
+    double *__restrict__ blockpart)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const size_t nn = (size_t)cap;
+    constexpr int NC = MEAS ? 11 : 3;
+    double q[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) q[k] = 0.0;
+    const int r = (i < cap) ? rep[i] : -1;
+    if (r >= 0 && (MODE == 0 || state[r].steps_left > 0)) {
+        const RepConsts &rc = reps[r];
+        const int t = i / TILE, lane = i % TILE;
+        const int2 ti = tile_items[t];
+        double f[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) f[k] = 0.0;
+        for (int m = 0; m < ti.y; ++m) {
+            const double *p = part + (size_t)(ti.x + m) * NC * TILE + lane;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) f[k] += p[k * TILE];
+        }
+        const double zi = pos[i].w;
+        const bool mix = (rc.kind == MIX);
+        const double fz_i = (mix && !(rc.ref_f1 && orig[i] == rc.n - 1)) ? zi : 1.0;
+        const double d = den[i];
+        double a[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a[k] = (rc.cnum * (fz_i * f[k])) / d;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc[k * nn + i] = a[k];
+        double v[3] = {0, 0, 0};
+        if constexpr (MODE == 1) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double vp = vold[k * nn + i] + rc.dt * a[k];
+                v[k] = 0.5 * (vel[k * nn + i] + vp);
+                vel[k * nn + i] = v[k];
+            }
+        }
+        if constexpr (MEAS) {
+            const double zs = mix ? zi : 1.0;
+            q[0] = 0.5 * zs * f[3];
+            q[1] = 0.5 * zs * f[4];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) q[2 + k] = 0.5 * zs * f[5 + k];
+            if constexpr (MODE != 0) {
+                const double ai = aii[i];
+                q[8] = ai * v[0] * v[0];
+                q[9] = ai * v[0] * v[1];
+                q[10] = ai * v[0] * v[2];
+                q[11] = ai * v[1] * v[1];
+                q[12] = ai * v[1] * v[2];
+                q[13] = ai * v[2] * v[2];
+            }
+        }
+    }
+    if constexpr (MEAS) block_reduce_store(q, blockpart + (size_t)blockIdx.x * NQ);
+}
+
+// One workgroup per replica: optional measurement record (fixed-order sum of
+// the replica's finalize blocks), step countdown, and the queue reset for
+// the next step (block 0).
+// MODE 0 (initial accel) writes its record to init_records[r] and keeps the
+// step budget; MODE 1 appends to records[r][rec_count] and counts a step.
+template <bool MEAS, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_batch_replica(const double *__restrict__ blockpart,
+                                                         const RepConsts *__restrict__ reps,
+                                                         RepState *__restrict__ state,
+                                                         int nrep, MeasOut *__restrict__ records,
+                                                         int rec_cap,
+                                                         MeasOut *__restrict__ init_records,
+                                                         unsigned *__restrict__ queue)
+{
+    const int r = blockIdx.x;
+    if (r == 0 && threadIdx.x == 0) *queue = 0u;
+    if (r >= nrep) return;
+    const RepConsts &rc = reps[r];
+    const bool live = rc.n > 0 && ((MODE == 0) || state[r].steps_left > 0);
+    if constexpr (MEAS) {
+        double q[NQ];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) q[k] = 0.0;
+        if (live)
+            for (int b = threadIdx.x; b < rc.nblk; b += BLOCK) {
+#pragma unroll
+                for (int k = 0; k < NQ; ++k) q[k] += blockpart[(size_t)(rc.blk0 + b) * NQ + k];
+            }
+        __shared__ double red[NQ][BLOCK];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) red[k][threadIdx.x] = q[k];
+        __syncthreads();
+        for (int s = BLOCK / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+#pragma unroll
+                for (int k = 0; k < NQ; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0 && live) {
+            MeasOut m;
+            const double ev = rc.cnum * red[0][0], vir = rc.cnum * red[1][0];
+            m.ev = ev / (double)rc.n_f;
+            m.vir = vir;
+            m.px = rc.rho * (rc.kT + vir / (double)rc.three_n);
+            for (int k = 0; k < 6; ++k) m.pp[k] = rc.cnum * red[2 + k][0];
+            m.ek = 0.0;
+            if (MODE != 0) {
+                m.ek = (0.5 * rc.xmass) * (red[8][0] + red[11][0] + red[13][0]) / (double)rc.n_f;
+                for (int k = 0; k < 6; ++k) m.pp[k] = (m.pp[k] + rc.xmass * red[8 + k][0]) / rc.vol;
+            }
+            if (MODE == 0) {
+                init_records[r] = m;
+            } else {
+                const int slot = state[r].rec_count;
+                if (slot < rec_cap) records[(size_t)r * rec_cap + slot] = m;
+                state[r].rec_count = slot + 1;
+            }
+        }
+    }
+    if (MODE != 0 && threadIdx.x == 0 && state[r].steps_left > 0) state[r].steps_left -= 1;
+}
+
+}  // namespace yk

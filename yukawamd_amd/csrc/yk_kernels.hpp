@@ -324,7 +324,7 @@ __device__ __forceinline__ unsigned long long as_bits(double v)
 }
 
 // One workgroup per target tile; sources scanned in slot order, 256 at a time.
-__global__ __launch_bounds__(BLOCK) void k_list_build(const double4 *__restrict__ pos,
+static __global__ __launch_bounds__(BLOCK) void k_list_build(const double4 *__restrict__ pos,
                                                       const TileBox *__restrict__ box,
                                                       ListConsts lc,
                                                       const unsigned long long *__restrict__ maxdisp,
@@ -489,7 +489,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_list(const double4 *__restrict_
     }
 }
 
-__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+static __global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
 // ---------------------------------------------------------------------------
 // Wave-level min/max and the per-tile AABB of 64 consecutive particles.
@@ -524,7 +524,7 @@ __device__ __forceinline__ void tile_box_store(const double x[3], bool valid, in
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_tile_boxes(const double4 *__restrict__ pos, int n,
+static __global__ __launch_bounds__(BLOCK) void k_tile_boxes(const double4 *__restrict__ pos, int n,
                                                       int ntiles, TileBox *__restrict__ box)
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
@@ -550,7 +550,7 @@ __device__ __forceinline__ unsigned spread3(unsigned v)
 
 // Also wraps every position into [0, L) (the slots are about to move and
 // every pair list is rebuilt afterwards).
-__global__ __launch_bounds__(BLOCK) void k_morton(double4 *__restrict__ pos, int n,
+static __global__ __launch_bounds__(BLOCK) void k_morton(double4 *__restrict__ pos, int n,
                                                   double gx, double gy, double gz, int gmax,
                                                   double Lx, double Ly, double Lz,
                                                   unsigned *__restrict__ key,
@@ -576,7 +576,7 @@ struct PermArrays {
     double4 *pos2; double *vel2, *vold2, *acc2, *aii2, *den2; int *orig2;
 };
 
-__global__ __launch_bounds__(BLOCK) void k_permute(const int *__restrict__ perm, int n,
+static __global__ __launch_bounds__(BLOCK) void k_permute(const int *__restrict__ perm, int n,
                                                    PermArrays a)
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
@@ -766,7 +766,7 @@ __global__ __launch_bounds__(BLOCK) void k_drift(double4 *__restrict__ pos,
 }
 
 // Kinetic energy sum A*(vx^2+vy^2+vz^2) per block (ttot, src/gen_utils.f:40-55).
-__global__ __launch_bounds__(BLOCK) void k_ttot(const double *__restrict__ vel,
+static __global__ __launch_bounds__(BLOCK) void k_ttot(const double *__restrict__ vel,
                                                 const double *__restrict__ aii, int n,
                                                 double *__restrict__ blockpart)
 {
@@ -794,7 +794,7 @@ struct MeasConsts {
     int mode;        // 0 accel, 1/2 newton, 3 ttot
 };
 
-__global__ __launch_bounds__(BLOCK) void k_meas_final(const double *__restrict__ blockpart,
+static __global__ __launch_bounds__(BLOCK) void k_meas_final(const double *__restrict__ blockpart,
                                                       int nblocks, MeasConsts mc,
                                                       MeasOut *__restrict__ out)
 {

@@ -159,3 +159,46 @@ def replica(seed: int, n: int, mixture: bool, gamma: float, kappa: float,
         p, x, v, Z, A, dt = ion_system(n, [(8.0, 16.0)], [1.0], 7.18e-5, kappa, gamma, 8.0,
                                        seed, precision)
     return Workload(f"replica_{seed}", p, x, v, Z, A, dt, 0)
+
+
+def ensemble(replicas: int = 256, seed: int = 1003, precision: str = "mixed",
+             n_min: int = 1024, n_max: int = 16384) -> list:
+    """The batched ensemble: per replica (seed sequence [seed, r]) N log-uniform
+    in [n_min, n_max], pure Z=8 or the three-species mixture with probability
+    1/2, Gamma uniform in [150, 220], kappa uniform in [0.5, 2]; same density,
+    cutoff min(L/2, 12 lambda) and dt = 0.05/omega_p rule."""
+    out = []
+    for r in range(replicas):
+        rng = np.random.Generator(np.random.PCG64(np.random.SeedSequence([seed, r])))
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        mixture = bool(rng.random() < 0.5)
+        gamma = float(rng.uniform(150.0, 220.0))
+        kappa = float(rng.uniform(0.5, 2.0))
+        sub_seed = int(rng.integers(0, 2 ** 31 - 1))
+        out.append(replica(sub_seed, n, mixture, gamma, kappa, precision))
+    return out
+
+
+def serving_requests(count: int = 2000, seed: int = 1004, n_min: int = 2048,
+                     n_max: int = 262144, steps_min: int = 100, steps_max: int = 2000,
+                     rate_per_s: float = 10.0, mixed_fraction: float = 0.8):
+    """Serving stream: Poisson arrivals at rate_per_s; per request the
+    three-species mixture with N log-uniform in [n_min, n_max], steps uniform
+    in [steps_min, steps_max], Gamma U[150, 220], kappa U[0.5, 2]; a fraction
+    mixed_fraction asks for mixed precision, the rest FP64.  Returns request
+    descriptions (the particle state is generated when a request is admitted)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = 0.0
+    reqs = []
+    for k in range(count):
+        t += float(rng.exponential(1.0 / rate_per_s))
+        reqs.append({
+            "id": k, "arrival_s": t,
+            "n": int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max))))),
+            "steps": int(rng.integers(steps_min, steps_max + 1)),
+            "gamma": float(rng.uniform(150.0, 220.0)),
+            "kappa": float(rng.uniform(0.5, 2.0)),
+            "precision": "mixed" if rng.random() < mixed_fraction else "fp64",
+            "seed": int(rng.integers(0, 2 ** 31 - 1)),
+        })
+    return reqs
