@@ -43,8 +43,17 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nind", type=int, default=10, help="measure every nind steps")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-calls", type=int, default=3)
+    ap.add_argument("--cpu-calls", type=int, default=25)
     ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-replicas", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--batch-crust", type=int, default=4,
+                    help="replicas of the 27,648-ion mixture in the batched mixed form")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=0.35, help="Poisson arrivals per s")
+    ap.add_argument("--serve-round", type=int, default=50, help="steps per serving round")
     return ap.parse_args()
 
 
@@ -63,6 +72,60 @@ def run_form(w, precision, steps, warmup, nind):
         return {"seconds": t, "timing": tim, "records": recs}
     finally:
         s.close()
+
+
+def run_batch(ws, steps, warmup, nind):
+    """Step the replicas ws together in one ensemble (budgets warmup + steps);
+    the timed region is the `steps` steps with every state resident."""
+    from yukawamd_amd.batch import Ensemble
+    cap = sum(((w.n + 255) // 256) * 256 for w in ws)
+    if nind:   # warm up over whole measuring blocks so the timed run replays the same graph
+        warmup = max(1, -(-warmup // nind)) * nind
+    nrec = ((warmup + steps) // nind + 1) if nind else 1
+    with Ensemble(capacity=cap, max_replicas=len(ws), max_records=nrec) as e:
+        ids = [e.add_workload(w, steps=warmup + steps) for w in ws]
+        e.run(warmup, nind)     # accelerations of the initial state + warm-up
+        t0 = time.perf_counter()
+        e.run(steps, nind)
+        t = time.perf_counter() - t0
+        lr = e.last_run()
+        left = [e.status(r)[0] for r in ids]
+        last = e.get(ids[0])["records"]
+    pairs = sum(w.n * (w.n - 1) // 2 for w in ws)
+    return {
+        "replicas": len(ws),
+        "particles": int(sum(w.n for w in ws)),
+        "pairs_per_step": pairs,
+        "steps": steps,
+        "seconds": t,
+        "device_ms": lr["ms"],
+        "ms_per_step": 1e3 * t / steps,
+        "pair_interactions_per_s": pairs * steps / t,
+        "work_items_per_step": lr["items"],
+        "all_budgets_done": all(v == 0 for v in left),
+        "replica0_last_record": last[-1] if last else None,
+    }
+
+
+def run_serving(count, rate, round_steps):
+    """The first `count` requests of the seeded serving stream (seed 1004),
+    arriving in real time at `rate` per second, served to completion."""
+    from yukawamd_amd.serving import Server, stream, summarize
+    reqs = stream(count, rate)
+    with Server(hbm_budget_bytes=64 * 2 ** 30, round_steps=round_steps) as srv:
+        t0 = time.perf_counter()
+        res = srv.serve(reqs)
+        wall = time.perf_counter() - t0
+        out = summarize(res, wall, srv.device_ms)
+        out.update({"rounds": srv.rounds, "peak_concurrent_runs": srv.peak_live,
+                    "peak_footprint_gib": srv.peak_bytes / 2 ** 30,
+                    "device_utilization": srv.device_ms * 1e-3 / wall})
+    out["workload"] = (f"first {count} requests of the serving stream (seed 1004): crust "
+                       "mixture, N log-uniform [2048, 262144], steps U[100, 2000], Gamma "
+                       f"U[150,220], kappa U[0.5,2], 80% mixed / 20% FP64; Poisson arrivals "
+                       f"at {rate}/s in real time; rounds of {round_steps} steps; HBM budget "
+                       "64 GiB")
+    return out
 
 
 def cpu_baseline(calls):
@@ -137,6 +200,25 @@ def main():
     if "mixed" in forms and "fp64" in forms:
         line["mixed_over_fp64"] = (forms["mixed"]["pair_interactions_per_s"]
                                    / forms["fp64"]["pair_interactions_per_s"])
+    if not args.no_batch:
+        from yukawamd_amd.batch import TUNING_FILE
+        crust = [W.ion_mixture(seed=2001 + k, precision="mixed") for k in range(args.batch_crust)]
+        if crust:
+            b = run_batch(crust, args.steps, args.warmup, args.nind)
+            b["workload"] = f"{len(crust)} x ion_mixture_27648 (seeds 2001..), mixed"
+            forms["mixed_batched"] = b
+            if "fp64" in forms:
+                line["mixed_batched_over_fp64"] = (b["pair_interactions_per_s"]
+                                                   / forms["fp64"]["pair_interactions_per_s"])
+        ens = W.ensemble(args.batch_replicas)
+        b = run_batch(ens, args.batch_steps, args.warmup, args.nind)
+        b["workload"] = (f"batched ensemble: {len(ens)} replicas (seed 1003), N log-uniform "
+                         "[1024, 16384], pure/mixture p=0.5, Gamma U[150,220], kappa U[0.5,2], "
+                         "mixed precision")
+        b["tuning"] = str(TUNING_FILE.relative_to(ROOT)) if TUNING_FILE.exists() else "default"
+        line["batched_ensemble"] = b
+    if not args.no_serve:
+        line["serving"] = run_serving(args.serve_requests, args.serve_rate, args.serve_round)
     line["roofline"] = {
         "bound": "valu",
         "kernel": "k_pairs<MIX,FP64>",

@@ -208,8 +208,10 @@ int yk_batch_create(int64_t capacity, int32_t max_replicas, int32_t max_records,
                     yk_batch **out);
 void yk_batch_destroy(yk_batch *b);
 
-/* Arena slots a replica of n particles occupies. */
-int yk_batch_footprint(int32_t n, int64_t *slots);
+/* Footprint of a replica of n particles under b's current tuning: the arena
+ * slots it occupies and the device bytes it holds (its slots plus the
+ * per-item partial sums of its work items). */
+int yk_batch_footprint(const yk_batch *b, int32_t n, int64_t *slots, int64_t *bytes);
 
 /* Admit a replica: positions/velocities in (3,n) layout (v may be NULL),
  * zii/aii as in yk_create (NULL allowed for pure-ion), its timestep and its
@@ -220,10 +222,17 @@ int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *
                  int32_t steps, int32_t *id);
 int yk_batch_remove(yk_batch *b, int32_t id);
 
-/* Source tiles per work item for replicas with n <= n_upper[k] (first match),
- * and the waves per workgroup of the persistent launch. */
+/* Tuning per N bucket (first k with n <= n_upper[k]): source tiles per work
+ * item, and waves per workgroup (1, 2, 4 or 8).  The persistent launch takes
+ * the waves of the bucket that holds the largest share of the pair work
+ * (sum of n^2) among the active replicas. */
 int yk_batch_set_tuning(yk_batch *b, int32_t nbuckets, const int32_t *n_upper,
-                        const int32_t *chunk_tiles, int32_t waves_per_block);
+                        const int32_t *chunk_tiles, const int32_t *waves_per_block);
+
+/* Re-sort a replica's slots into Morton order every `every` of its own steps
+ * (0: never; default 10), keeping tiles compact as particles move.  The
+ * re-sort points depend only on the replica's own step count. */
+int yk_batch_set_resort(yk_batch *b, int32_t every);
 
 /* Advance every replica by up to nsteps (a replica stops when its budget is
  * used up), measuring every nind-th step (src/md.F:129 cadence). */

@@ -110,3 +110,44 @@ def test_capacity_limit():
         assert err.value.code == 7
         e.remove(rid)
         e.add_workload(w, steps=1)
+
+
+def test_resort_changes_only_rounding():
+    w = W.replica(61, 3000, True, 180.0, 0.8, "fp64")
+    outs = []
+    for every in (0, 7):
+        with Ensemble(capacity=4096, max_replicas=1, max_records=4) as e:
+            e.set_resort(every)
+            rid = e.add_workload(w, steps=30)
+            e.run(30, nind=10)
+            outs.append(e.get(rid))
+    a, b = outs
+    L = w.params.xl[0]
+    dx = np.abs(a["x"] - b["x"])
+    dx = np.minimum(dx, L - dx)
+    assert dx.max() <= 1e-9 * L
+    assert len(a["records"]) == len(b["records"]) == 3
+    for r1, r2 in zip(a["records"], b["records"]):
+        assert r1["ev"] == pytest.approx(r2["ev"], rel=1e-10)
+        assert r1["ek"] == pytest.approx(r2["ek"], rel=1e-10)
+
+
+def test_staggered_admission_with_resort_is_bitwise():
+    target = W.replica(71, 1800, True, 170.0, 0.9, "mixed")
+    other = W.replica(72, 2500, False, 200.0, 1.5, "mixed")
+    with Ensemble(capacity=8192, max_replicas=4, max_records=8) as e:
+        e.set_resort(10)
+        e.add_workload(other, steps=100)
+        e.run(13, nind=0)                 # the other replica is 13 steps ahead
+        tid = e.add_workload(target, steps=45)
+        e.run(17, nind=0)
+        e.run(40, nind=0)
+        assert e.status(tid)[0] == 0
+        mixed_in = e.get(tid)
+    with Ensemble(capacity=4096, max_replicas=1, max_records=8) as e:
+        e.set_resort(10)
+        rid = e.add_workload(target, steps=45)
+        e.run(45, nind=0)
+        alone = e.get(rid)
+    for key in ("x", "v", "a"):
+        assert np.array_equal(mixed_in[key], alone[key]), key

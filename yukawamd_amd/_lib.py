@@ -79,12 +79,14 @@ SIGNATURES = {
     "yk_batch_last_error": (C.c_char_p, []),
     "yk_batch_create": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(P)]),
     "yk_batch_destroy": (None, [P]),
-    "yk_batch_footprint": (C.c_int, [C.c_int32, C.POINTER(C.c_int64)]),
+    "yk_batch_footprint": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_int64)]),
     "yk_batch_add": (C.c_int, [P, C.POINTER(YkParams), C.c_int32, D, D, D, D, C.c_double,
                                C.c_int32, C.POINTER(C.c_int32)]),
     "yk_batch_remove": (C.c_int, [P, C.c_int32]),
     "yk_batch_set_tuning": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int32),
-                                      C.POINTER(C.c_int32), C.c_int32]),
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "yk_batch_set_resort": (C.c_int, [P, C.c_int32]),
     "yk_batch_run": (C.c_int, [P, C.c_int32, C.c_int32]),
     "yk_batch_status": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "yk_batch_get": (C.c_int, [P, C.c_int32, D, D, D, C.POINTER(YkMeas), C.POINTER(YkMeas),

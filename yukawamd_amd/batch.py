@@ -31,10 +31,11 @@ class Ensemble:
                                                   int(max_records), C.byref(self._b)))
         self.max_records = int(max_records)
         self.replicas = {}
+        self.tuning = []
         if tuning is None and TUNING_FILE.exists():
-            tuning = json.loads(TUNING_FILE.read_text())
+            tuning = json.loads(TUNING_FILE.read_text())["buckets"]
         if tuning:
-            self.set_tuning(tuning["n_upper"], tuning["chunk_tiles"], tuning["waves_per_block"])
+            self.set_tuning(tuning)
 
     def close(self):
         if self._b:
@@ -53,17 +54,27 @@ class Ensemble:
         except Exception:
             pass
 
-    @staticmethod
-    def footprint(n: int) -> int:
-        out = C.c_int64()
-        _lib.check_batch(_lib.load().yk_batch_footprint(int(n), C.byref(out)))
-        return out.value
+    def footprint(self, n: int):
+        """(arena slots, device bytes) a replica of n particles would hold."""
+        slots, nbytes = C.c_int64(), C.c_int64()
+        _lib.check_batch(self.lib.yk_batch_footprint(self._b, int(n), C.byref(slots),
+                                                     C.byref(nbytes)))
+        return slots.value, nbytes.value
 
-    def set_tuning(self, n_upper, chunk_tiles, waves_per_block=4):
-        nb = len(n_upper)
-        a = (C.c_int32 * nb)(*[int(v) for v in n_upper])
-        b = (C.c_int32 * nb)(*[int(v) for v in chunk_tiles])
-        _lib.check_batch(self.lib.yk_batch_set_tuning(self._b, nb, a, b, int(waves_per_block)))
+    def set_tuning(self, buckets):
+        """buckets: [{"n_upper": N, "chunk_tiles": C, "waves_per_block": W}, ...]
+        in increasing n_upper."""
+        nb = len(buckets)
+        a = (C.c_int32 * nb)(*[int(b["n_upper"]) for b in buckets])
+        c = (C.c_int32 * nb)(*[int(b["chunk_tiles"]) for b in buckets])
+        w = (C.c_int32 * nb)(*[int(b["waves_per_block"]) for b in buckets])
+        _lib.check_batch(self.lib.yk_batch_set_tuning(self._b, nb, a, c, w))
+        self.tuning = list(buckets)
+
+    def set_resort(self, every: int):
+        """Re-sort each replica into Morton order every `every` of its own steps
+        (0: never)."""
+        _lib.check_batch(self.lib.yk_batch_set_resort(self._b, int(every)))
 
     def add(self, params, x, v, Z, A, dt: float, steps: int) -> int:
         n = x.shape[0]

@@ -22,6 +22,8 @@
 #include <string>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "../../include/yk.h"
 #include "yk_batch_kernels.hpp"
 
@@ -56,7 +58,9 @@ struct HostReplica {
     int n = 0, off = 0, nblocks = 0;   // slots [off, off + n), 256-slot blocks used
     yk_params p{};
     double dt = 0.0;
-    std::vector<int> orig;             // sorted slot k -> caller index
+    // host mirror of the device step countdown (the device skips a replica
+    // whose budget is used up; the host knows when, without reading back)
+    int steps_left = 0, steps_done = 0, sorted_at = 0;
 };
 
 }  // namespace
@@ -81,13 +85,23 @@ struct yk_batch {
     size_t part_items_cap = 0;
     double *blockpart = nullptr;
     unsigned *queue = nullptr;
+    // arena re-sort: keys, slot values, radix-sort scratch, gather targets
+    unsigned long long *keys = nullptr, *keys2 = nullptr;
+    int *vals = nullptr, *vals2 = nullptr, *due = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int sort_end_bit = 64;
+    double4 *pos2 = nullptr;
+    double *vel2 = nullptr, *vold2 = nullptr, *acc2 = nullptr, *aii2 = nullptr, *den2 = nullptr;
+    int *orig2 = nullptr;
+    int resort_every = 10;   // a replica is re-sorted every this many of its own steps
     // host
     std::vector<HostReplica> table;
     std::vector<char> used;  // per 256-slot block
     bool dirty = true;       // items / constants must be rebuilt
     bool need_accel = true;  // accelerations of new replicas are missing
     int nitems = 0;
-    std::vector<int> tune_nupper, tune_chunk;
+    std::vector<int> tune_nupper, tune_chunk, tune_waves;
     int waves = 4;           // waves per workgroup of the persistent launch
     // captured step graph (valid while !dirty)
     hipGraphExec_t gexec = nullptr;
@@ -112,13 +126,35 @@ void drop_graph(yk_batch *b)
     b->g_nind = -1;
 }
 
+int bucket_of(const yk_batch *b, int n)
+{
+    for (size_t k = 0; k < b->tune_nupper.size(); ++k)
+        if (n <= b->tune_nupper[k]) return (int)k;
+    return -1;
+}
+
 int chunk_tiles_for(const yk_batch *b, int n)
 {
     const int nt = (n + TILE - 1) / TILE;
-    for (size_t k = 0; k < b->tune_nupper.size(); ++k)
-        if (n <= b->tune_nupper[k]) return std::max(1, std::min(nt, b->tune_chunk[k]));
+    const int k = bucket_of(b, n);
+    if (k >= 0) return std::max(1, std::min(nt, b->tune_chunk[k]));
     return std::max(1, (nt + 3) / 4);   // default: four source ranges per target tile
 }
+
+// Work items of a replica of n particles and their device bytes.
+int64_t items_of(const yk_batch *b, int n)
+{
+    const int64_t nt = (n + TILE - 1) / TILE, C = chunk_tiles_for(b, n);
+    return nt * ((nt + C - 1) / C);
+}
+
+// device bytes per arena slot (pos, vel/vold/acc, aii, den, orig, rep), per
+// 64-slot tile (box, tile_rep, tile_items), per 256-slot block (measurement
+// partials) and per work item (partial sums, item record)
+constexpr int64_t SLOT_BYTES = sizeof(double4) + 11 * sizeof(double) + 2 * sizeof(int);
+constexpr int64_t TILE_BYTES = sizeof(TileBox) + sizeof(int) + sizeof(int2);
+constexpr int64_t BLOCK_BYTES = NQ * sizeof(double);
+constexpr int64_t ITEM_BYTES = 11 * TILE * sizeof(double) + sizeof(BatchItem);
 
 RepConsts rep_consts(const HostReplica &h)
 {
@@ -156,6 +192,9 @@ RepConsts rep_consts(const HostReplica &h)
     c.blk0 = h.off / BLOCK;
     c.nblk = (h.n + BLOCK - 1) / BLOCK;
     c.ref_f1 = (p.flags & YK_FLAG_REF_F1) ? 1 : 0;
+    int g = 1;   // about 2 particles per Morton cell, as at admission
+    while (g < 10 && (double)(1 << (3 * g)) < 0.5 * h.n) ++g;
+    c.grid_bits = g;
     return c;
 }
 
@@ -204,6 +243,22 @@ int rebuild(yk_batch *b)
         }
     }
     b->nitems = (int)items.size();
+    // launch shape: the waves per workgroup of the bucket carrying the most
+    // pair work
+    {
+        std::vector<double> work(b->tune_nupper.size(), 0.0);
+        for (int r : order) {
+            const int k = bucket_of(b, b->table[r].n);
+            if (k >= 0) work[k] += (double)b->table[r].n * b->table[r].n;
+        }
+        b->waves = 4;
+        double best = 0.0;
+        for (size_t k = 0; k < work.size(); ++k)
+            if (work[k] > best) {
+                best = work[k];
+                b->waves = b->tune_waves[k];
+            }
+    }
     if ((size_t)b->nitems > b->part_items_cap) {
         if (b->part) YB_HIP(hipFree(b->part));
         if (b->items) YB_HIP(hipFree(b->items));
@@ -263,7 +318,8 @@ int enqueue_step(yk_batch *b)
     return YK_OK;
 }
 
-// Accelerations (and the initial measurement) of every active replica.
+// Accelerations (and the initial measurement) of the replicas admitted since
+// the last pass; the others keep theirs.
 int accel_pass(yk_batch *b)
 {
     // the queue counter is zero here: it starts zeroed and every step's
@@ -297,6 +353,92 @@ int capture(yk_batch *b, int steps, int nind)
     b->g_steps = steps;
     b->g_nind = nind;
     return YK_OK;
+}
+
+// Re-sort the replicas whose own step count reached a multiple of
+// resort_every (and that still have steps to do) back into Morton order:
+// keys, one stable radix sort over the arena, gather into the second buffer
+// set, copy back (the captured step graph keeps its buffer addresses).  A
+// replica's re-sort points depend only on its own step count, so its
+// trajectory is the same whatever else is in the arena and whenever it was
+// admitted.
+int resort_due(yk_batch *b)
+{
+    if (b->resort_every <= 0) return YK_OK;
+    std::vector<int> due(b->max_rep, 0);
+    bool any = false;
+    for (int r = 0; r < b->max_rep; ++r) {
+        HostReplica &h = b->table[r];
+        if (h.active && h.steps_left > 0 && h.steps_done > 0 &&
+            h.steps_done % b->resort_every == 0 && h.sorted_at != h.steps_done) {
+            due[r] = 1;
+            h.sorted_at = h.steps_done;
+            any = true;
+        }
+    }
+    if (!any) return YK_OK;
+    const size_t nn = (size_t)b->cap;
+    const dim3 slots((b->cap + BLOCK - 1) / BLOCK);
+    YB_HIP(hipMemcpyAsync(b->due, due.data(), sizeof(int) * b->max_rep, hipMemcpyHostToDevice,
+                          b->stream));
+    hipLaunchKernelGGL(k_batch_sort_keys, slots, dim3(BLOCK), 0, b->stream, b->pos, b->rep,
+                       b->reps, b->due, b->cap, b->keys, b->vals);
+    YB_HIP(hipGetLastError());
+    size_t bytes = b->sort_tmp_bytes;
+    YB_HIP(hipcub::DeviceRadixSort::SortPairs(b->sort_tmp, bytes, b->keys, b->keys2, b->vals,
+                                              b->vals2, b->cap, 0, b->sort_end_bit, b->stream));
+    PermArrays a{b->pos, b->vel, b->vold, b->acc, b->aii, b->den, b->orig,
+                 b->pos2, b->vel2, b->vold2, b->acc2, b->aii2, b->den2, b->orig2};
+    hipLaunchKernelGGL(k_permute, slots, dim3(BLOCK), 0, b->stream, b->vals2, b->cap, a);
+    YB_HIP(hipGetLastError());
+    YB_HIP(hipMemcpyAsync(b->pos, b->pos2, sizeof(double4) * nn, hipMemcpyDeviceToDevice,
+                          b->stream));
+    for (auto pr : {std::make_pair(b->vel, b->vel2), std::make_pair(b->vold, b->vold2),
+                    std::make_pair(b->acc, b->acc2)})
+        YB_HIP(hipMemcpyAsync(pr.first, pr.second, sizeof(double) * 3 * nn,
+                              hipMemcpyDeviceToDevice, b->stream));
+    YB_HIP(hipMemcpyAsync(b->aii, b->aii2, sizeof(double) * nn, hipMemcpyDeviceToDevice,
+                          b->stream));
+    YB_HIP(hipMemcpyAsync(b->den, b->den2, sizeof(double) * nn, hipMemcpyDeviceToDevice,
+                          b->stream));
+    YB_HIP(hipMemcpyAsync(b->orig, b->orig2, sizeof(int) * nn, hipMemcpyDeviceToDevice,
+                          b->stream));
+    return YK_OK;
+}
+
+// Run-relative steps [from, to): whole graph blocks where aligned, single
+// steps otherwise; step j (1-based in the run) measures when j % nind == 0.
+int run_steps(yk_batch *b, int from, int to, int nind, int block)
+{
+    int j = from;
+    while (j < to) {
+        if (j % block == 0 && j + block <= to) {
+            YB_HIP(hipGraphLaunch(b->gexec, b->stream));
+            j += block;
+        } else {
+            const bool meas = nind > 0 && (j + 1) % nind == 0;
+            const int rc = meas ? enqueue_step<true, 1>(b) : enqueue_step<false, 1>(b);
+            if (rc) return rc;
+            j += 1;
+        }
+    }
+    for (HostReplica &h : b->table)
+        if (h.active) {
+            const int k = std::min(to - from, h.steps_left);
+            h.steps_left -= k;
+            h.steps_done += k;
+        }
+    return YK_OK;
+}
+
+// Box wrap of a downloaded position: the reference's one-shift rule
+// (src/vel_verlet_2.f:39-40), then a floor for positions several boxes out.
+double wrap_into_box(double x, double L)
+{
+    if (x < 0.0) x = x + L;
+    if (x > L) x = x - L;
+    if (x < 0.0 || x > L) x = x - L * std::floor(x / L);
+    return x;
 }
 
 // Morton order of one replica's particles (host; done once per admission).
@@ -406,6 +548,24 @@ int yk_batch_create(int64_t capacity, int32_t max_replicas, int32_t max_records,
     YB_HIPB(hipMalloc(&b->records, sizeof(MeasOut) * (size_t)max_replicas * b->rec_cap));
     YB_HIPB(hipMalloc(&b->init_records, sizeof(MeasOut) * (size_t)max_replicas));
     YB_HIPB(hipMalloc(&b->queue, sizeof(unsigned)));
+    YB_HIPB(hipMalloc(&b->keys, sizeof(unsigned long long) * nn));
+    YB_HIPB(hipMalloc(&b->keys2, sizeof(unsigned long long) * nn));
+    YB_HIPB(hipMalloc(&b->vals, sizeof(int) * nn));
+    YB_HIPB(hipMalloc(&b->vals2, sizeof(int) * nn));
+    YB_HIPB(hipMalloc(&b->due, sizeof(int) * (size_t)max_replicas));
+    YB_HIPB(hipMalloc(&b->pos2, sizeof(double4) * nn));
+    for (double **pp : {&b->vel2, &b->vold2, &b->acc2})
+        YB_HIPB(hipMalloc(pp, sizeof(double) * 3 * nn));
+    YB_HIPB(hipMalloc(&b->aii2, sizeof(double) * nn));
+    YB_HIPB(hipMalloc(&b->den2, sizeof(double) * nn));
+    YB_HIPB(hipMalloc(&b->orig2, sizeof(int) * nn));
+    b->sort_end_bit = 33;
+    while (b->sort_end_bit < 64 && ((unsigned long long)1 << (b->sort_end_bit - 32)) < nn)
+        ++b->sort_end_bit;
+    YB_HIPB(hipcub::DeviceRadixSort::SortPairs(nullptr, b->sort_tmp_bytes, b->keys, b->keys2,
+                                               b->vals, b->vals2, b->cap, 0, b->sort_end_bit,
+                                               b->stream));
+    YB_HIPB(hipMalloc(&b->sort_tmp, b->sort_tmp_bytes > 0 ? b->sort_tmp_bytes : 16));
     YB_HIPB(hipEventCreate(&b->e0));
     YB_HIPB(hipEventCreate(&b->e1));
     // empty arena: every slot unused, every replica idle
@@ -438,7 +598,10 @@ void yk_batch_destroy(yk_batch *b)
                     (void *)b->tile_rep, (void *)b->box, (void *)b->tile_items,
                     (void *)b->blockpart, (void *)b->reps, (void *)b->state,
                     (void *)b->records, (void *)b->init_records, (void *)b->queue,
-                    (void *)b->part, (void *)b->items})
+                    (void *)b->part, (void *)b->items, (void *)b->keys, (void *)b->keys2,
+                    (void *)b->vals, (void *)b->vals2, (void *)b->due, b->sort_tmp,
+                    (void *)b->pos2, (void *)b->vel2, (void *)b->vold2, (void *)b->acc2,
+                    (void *)b->aii2, (void *)b->den2, (void *)b->orig2})
         if (p) (void)hipFree(p);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
@@ -446,10 +609,14 @@ void yk_batch_destroy(yk_batch *b)
     delete b;
 }
 
-int yk_batch_footprint(int32_t n, int64_t *slots_out)
+int yk_batch_footprint(const yk_batch *b, int32_t n, int64_t *slots_out, int64_t *bytes_out)
 {
-    if (!slots_out || n < 1) return bfail(YK_ERR_ARG, "n must be >= 1");
-    *slots_out = ((int64_t)n + BLOCK - 1) / BLOCK * BLOCK;
+    if (!b || n < 1) return bfail(YK_ERR_ARG, "batch is NULL or n < 1");
+    const int64_t slots = ((int64_t)n + BLOCK - 1) / BLOCK * BLOCK;
+    if (slots_out) *slots_out = slots;
+    if (bytes_out)
+        *bytes_out = slots * SLOT_BYTES + slots / TILE * TILE_BYTES + slots / BLOCK * BLOCK_BYTES +
+                     items_of(b, n) * ITEM_BYTES;
     return YK_OK;
 }
 
@@ -491,14 +658,15 @@ int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *
     h.off = start * BLOCK;
     h.nblocks = nblk;
     h.dt = dt;
-    h.orig = morton_order(x, n, params->xl);
+    h.steps_left = steps;
+    const std::vector<int> order = morton_order(x, n, params->xl);
     // sorted host staging
     const size_t nb = (size_t)nblk * BLOCK;
     std::vector<double4> p4(nb, make_double4(0, 0, 0, 0));
     std::vector<double> vv(3 * nb, 0.0), aa(nb, 1.0), dd(nb, 1.0);
     std::vector<int> oo(nb, -1);
     for (int k = 0; k < n; ++k) {
-        const int i = h.orig[k];
+        const int i = order[k];
         const double zi = (params->sim_type == YK_PURE_ION) ? (zii ? zii[i] : params->zi) : zii[i];
         const double ai = (params->sim_type == YK_PURE_ION) ? (aii ? aii[i] : params->ai) : aii[i];
         if (!(ai > 0.0) || !std::isfinite(zi))
@@ -527,7 +695,7 @@ int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *
                           b->stream));
     YB_HIP(hipMemcpyAsync(b->orig + o, oo.data(), sizeof(int) * nb, hipMemcpyHostToDevice,
                           b->stream));
-    RepState st{steps, 0, 0, 0};
+    RepState st{steps, 0, 1, 0};
     YB_HIP(hipMemcpyAsync(b->state + id, &st, sizeof st, hipMemcpyHostToDevice, b->stream));
     YB_HIP(hipStreamSynchronize(b->stream));
     for (int s = start; s < start + nblk; ++s) b->used[s] = 1;
@@ -553,19 +721,29 @@ int yk_batch_remove(yk_batch *b, int32_t id)
 }
 
 int yk_batch_set_tuning(yk_batch *b, int32_t nbuckets, const int32_t *n_upper,
-                        const int32_t *chunk_tiles, int32_t waves_per_block)
+                        const int32_t *chunk_tiles, const int32_t *waves_per_block)
 {
-    if (!b || nbuckets < 0 || (nbuckets > 0 && (!n_upper || !chunk_tiles)))
+    if (!b || nbuckets < 0 || (nbuckets > 0 && (!n_upper || !chunk_tiles || !waves_per_block)))
         return bfail(YK_ERR_ARG, "invalid tuning table");
-    if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4 &&
-        waves_per_block != 8)
-        return bfail(YK_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
+    for (int k = 0; k < nbuckets; ++k) {
+        const int w = waves_per_block[k];
+        if (chunk_tiles[k] < 1) return bfail(YK_ERR_ARG, "chunk_tiles must be >= 1");
+        if (w != 1 && w != 2 && w != 4 && w != 8)
+            return bfail(YK_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
+        if (k > 0 && n_upper[k] <= n_upper[k - 1])
+            return bfail(YK_ERR_ARG, "n_upper must increase");
+    }
     b->tune_nupper.assign(n_upper, n_upper + nbuckets);
     b->tune_chunk.assign(chunk_tiles, chunk_tiles + nbuckets);
-    for (int k = 0; k < nbuckets; ++k)
-        if (b->tune_chunk[k] < 1) return bfail(YK_ERR_ARG, "chunk_tiles must be >= 1");
-    b->waves = waves_per_block;
+    b->tune_waves.assign(waves_per_block, waves_per_block + nbuckets);
     b->dirty = true;
+    return YK_OK;
+}
+
+int yk_batch_set_resort(yk_batch *b, int32_t every)
+{
+    if (!b || every < 0) return bfail(YK_ERR_ARG, "resort interval must be >= 0");
+    b->resort_every = every;
     return YK_OK;
 }
 
@@ -583,10 +761,21 @@ int yk_batch_run(yk_batch *b, int32_t nsteps, int32_t nind)
     if (!b->gexec || b->g_steps != block || b->g_nind != nind)
         if ((rc = capture(b, block, nind))) return rc;
     YB_HIP(hipEventRecord(b->e0, b->stream));
+    // segments end where some replica's own step count reaches a re-sort point
     int done = 0;
-    for (; done + block <= nsteps; done += block) YB_HIP(hipGraphLaunch(b->gexec, b->stream));
-    for (; done < nsteps; ++done)   // remainder, never a measuring step when nind > 0
-        if ((rc = enqueue_step<false, 1>(b))) return rc;
+    for (;;) {
+        if ((rc = resort_due(b))) return rc;
+        if (done >= nsteps) break;
+        int seg = nsteps - done;
+        if (b->resort_every > 0)
+            for (const HostReplica &h : b->table)
+                if (h.active && h.steps_left > 0) {
+                    const int d = b->resort_every - h.steps_done % b->resort_every;
+                    if (d <= h.steps_left) seg = std::min(seg, d);
+                }
+        if ((rc = run_steps(b, done, done + seg, nind, block))) return rc;
+        done += seg;
+    }
     YB_HIP(hipEventRecord(b->e1, b->stream));
     YB_HIP(hipEventSynchronize(b->e1));
     float ms = 0.f;
@@ -616,14 +805,16 @@ int yk_batch_get(yk_batch *b, int32_t id, double *x, double *v, double *a, yk_me
     const HostReplica &h = b->table[id];
     YB_HIP(hipStreamSynchronize(b->stream));
     const size_t o = (size_t)h.off, n = (size_t)h.n, cap = (size_t)b->cap;
+    std::vector<int> orig(n);   // slot -> caller index (re-sorts move slots)
+    YB_HIP(hipMemcpy(orig.data(), b->orig + o, sizeof(int) * n, hipMemcpyDeviceToHost));
     if (x) {
         std::vector<double4> p4(n);
         YB_HIP(hipMemcpy(p4.data(), b->pos + o, sizeof(double4) * n, hipMemcpyDeviceToHost));
         for (size_t k = 0; k < n; ++k) {
-            double *xi = x + 3 * (size_t)h.orig[k];
-            xi[0] = p4[k].x;
-            xi[1] = p4[k].y;
-            xi[2] = p4[k].z;
+            double *xi = x + 3 * (size_t)orig[k];
+            xi[0] = wrap_into_box(p4[k].x, h.p.xl[0]);
+            xi[1] = wrap_into_box(p4[k].y, h.p.xl[1]);
+            xi[2] = wrap_into_box(p4[k].z, h.p.xl[2]);
         }
     }
     std::vector<double> t(n);
@@ -634,7 +825,7 @@ int yk_batch_get(yk_batch *b, int32_t id, double *x, double *v, double *a, yk_me
         for (int c = 0; c < 3; ++c) {
             YB_HIP(hipMemcpy(t.data(), src + c * cap + o, sizeof(double) * n,
                              hipMemcpyDeviceToHost));
-            for (size_t k = 0; k < n; ++k) dst[3 * (size_t)h.orig[k] + c] = t[k];
+            for (size_t k = 0; k < n; ++k) dst[3 * (size_t)orig[k] + c] = t[k];
         }
     }
     auto conv = [](const MeasOut &m, yk_meas *d) {

@@ -29,13 +29,14 @@ struct RepConsts {
     float three_n, n_f;
     int kind, prec, off, n;
     int tile0, ntiles, blk0, nblk;
-    int ref_f1, pad0;
+    int ref_f1, grid_bits;   // grid_bits: Morton cells per axis = 2^grid_bits
 };
 
 struct RepState {            // device-updated per replica
     int steps_left;          // newton steps still to do (0 = idle)
     int rec_count;           // measurement records written
-    int pad0, pad1;
+    int fresh;               // 1 until the accel pass has evaluated the admitted state
+    int pad0;
 };
 
 struct BatchItem {
@@ -129,14 +130,14 @@ __device__ __forceinline__ void batch_item(const RepConsts &rc, const ExpCoeffs 
 }
 
 // Persistent: every wave loops pulling items until the queue is empty.
-// all_reps != 0 (the initial accel pass) evaluates every listed replica;
-// otherwise replicas whose step budget is used up are skipped.
+// accel_pass != 0 evaluates only replicas admitted since the last pass
+// (fresh); otherwise replicas whose step budget is used up are skipped.
 template <bool MEAS>
 __global__ void k_batch_pairs(const double4 *__restrict__ pos, const TileBox *__restrict__ box,
                               const RepConsts *__restrict__ reps,
                               const RepState *__restrict__ state,
                               const BatchItem *__restrict__ items, int nitems,
-                              unsigned *__restrict__ queue, ExpCoeffs ec, int all_reps,
+                              unsigned *__restrict__ queue, ExpCoeffs ec, int accel_pass,
                               double *__restrict__ part)
 {
     extern __shared__ double4 sj_dyn[];
@@ -151,8 +152,8 @@ __global__ void k_batch_pairs(const double4 *__restrict__ pos, const TileBox *__
         const BatchItem it = items[k];
         double *out = part + (size_t)k * NC * TILE;
         const RepConsts &rc = reps[it.rep];
-        if (!all_reps && state[it.rep].steps_left <= 0) {
-            // idle replica: leave zero partials
+        if (accel_pass ? state[it.rep].fresh == 0 : state[it.rep].steps_left <= 0) {
+            // replica not part of this pass: leave zero partials
             out[lane] = 0.0; out[TILE + lane] = 0.0; out[2 * TILE + lane] = 0.0;
             if constexpr (MEAS)
                 for (int q = 3; q < NC; ++q) out[q * TILE + lane] = 0.0;
@@ -171,7 +172,11 @@ __global__ void k_batch_pairs(const double4 *__restrict__ pos, const TileBox *__
 }
 
 // ---------------------------------------------------------------------------
-// Per-slot drift (vel_verlet_2 order, reference wrap) + tile AABBs.
+// Per-slot drift (vel_verlet_2 order) + tile AABBs.  Positions stay
+// continuous on the device (no wrap into the box): forces see only minimum-
+// image differences, and a tile whose particle crosses a box face keeps a
+// compact AABB instead of one spanning the box.  Positions are wrapped into
+// the box when they are downloaded.
 // MOVE = false only recomputes the tile AABBs (before the initial accel pass).
 template <bool MOVE>
 __global__ __launch_bounds__(BLOCK) void k_batch_drift(double4 *__restrict__ pos,
@@ -196,18 +201,56 @@ __global__ __launch_bounds__(BLOCK) void k_batch_drift(double4 *__restrict__ pos
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 double v = vel[k * nn + i], a = acc[k * nn + i];
-                double xk = x[k] + v * rc.dt + rc.afac * a;
                 vold[k * nn + i] = v;
                 vel[k * nn + i] = v + rc.dt * a;
-                if (xk < 0.0) xk = xk + rc.L[k];        // src/vel_verlet_2.f:39-40
-                if (xk > rc.L[k]) xk = xk - rc.L[k];
-                x[k] = xk;
+                x[k] = x[k] + v * rc.dt + rc.afac * a;   // src/vel_verlet_2.f:38
             }
             pos[i] = make_double4(x[0], x[1], x[2], p.w);
         }
     }
     const int tile = i / TILE;
     if (tile < cap / TILE) tile_box_store(x, valid, tile, box);
+}
+
+// Keys of an arena re-sort (a stable radix sort of 64-bit keys over every
+// slot).  A slot of a replica in `due` gets (replica offset << 32 | Morton
+// cell of its position), and the position is wrapped into the box (a
+// particle that crossed a face joins a tile on the side it is now on, so
+// tiles stay compact); every other slot, unused ones included, gets
+// (slot << 32), which keeps it where it is.  So the sort reorders each due
+// replica inside its own slot range (equal cells keep their slot order) and
+// leaves the rest of the arena alone.
+__global__ __launch_bounds__(BLOCK) void k_batch_sort_keys(double4 *__restrict__ pos,
+                                                           const int *__restrict__ rep,
+                                                           const RepConsts *__restrict__ reps,
+                                                           const int *__restrict__ due, int cap,
+                                                           unsigned long long *__restrict__ key,
+                                                           int *__restrict__ val)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= cap) return;
+    const int r = rep[i];
+    unsigned long long k = (unsigned long long)i << 32;
+    if (r >= 0 && due[r]) {
+        const RepConsts &rc = reps[r];
+        const int g = rc.grid_bits;
+        const int cmax = (1 << g) - 1;
+        const double4 p = pos[i];
+        double xs[3] = {p.x, p.y, p.z};
+        unsigned c[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            xs[d] = xs[d] - rc.L[d] * __builtin_floor(xs[d] * rc.invL[d]);
+            int ci = (int)(xs[d] * rc.invL[d] * (double)(1 << g));
+            ci = ci < 0 ? 0 : (ci > cmax ? cmax : ci);
+            c[d] = (unsigned)ci;
+        }
+        pos[i] = make_double4(xs[0], xs[1], xs[2], p.w);
+        k = ((unsigned long long)rc.off << 32) |
+            (unsigned long long)(spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2));
+    }
+    key[i] = k;
+    val[i] = i;
 }
 
 // Per-slot finalize: sum the tile's item partials in item order, scale,
@@ -228,7 +271,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_finalize(
 #pragma unroll
     for (int k = 0; k < NQ; ++k) q[k] = 0.0;
     const int r = (i < cap) ? rep[i] : -1;
-    if (r >= 0 && (MODE == 0 || state[r].steps_left > 0)) {
+    if (r >= 0 && (MODE == 0 ? state[r].fresh != 0 : state[r].steps_left > 0)) {
         const RepConsts &rc = reps[r];
         const int t = i / TILE, lane = i % TILE;
         const int2 ti = tile_items[t];
@@ -281,8 +324,9 @@ __global__ __launch_bounds__(BLOCK) void k_batch_finalize(
 // One workgroup per replica: optional measurement record (fixed-order sum of
 // the replica's finalize blocks), step countdown, and the queue reset for
 // the next step (block 0).
-// MODE 0 (initial accel) writes its record to init_records[r] and keeps the
-// step budget; MODE 1 appends to records[r][rec_count] and counts a step.
+// MODE 0 (accel pass of freshly admitted replicas) writes their record to
+// init_records[r] and clears fresh; MODE 1 appends to records[r][rec_count]
+// and counts a step.
 template <bool MEAS, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_batch_replica(const double *__restrict__ blockpart,
                                                          const RepConsts *__restrict__ reps,
@@ -296,7 +340,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_replica(const double *__restric
     if (r == 0 && threadIdx.x == 0) *queue = 0u;
     if (r >= nrep) return;
     const RepConsts &rc = reps[r];
-    const bool live = rc.n > 0 && ((MODE == 0) || state[r].steps_left > 0);
+    const bool live = rc.n > 0 && (MODE == 0 ? state[r].fresh != 0 : state[r].steps_left > 0);
     if constexpr (MEAS) {
         double q[NQ];
 #pragma unroll
@@ -338,7 +382,11 @@ __global__ __launch_bounds__(BLOCK) void k_batch_replica(const double *__restric
             }
         }
     }
-    if (MODE != 0 && threadIdx.x == 0 && state[r].steps_left > 0) state[r].steps_left -= 1;
+    __syncthreads();   // every thread has read state[r] before it changes
+    if (threadIdx.x == 0) {
+        if (MODE == 0) state[r].fresh = 0;
+        else if (state[r].steps_left > 0) state[r].steps_left -= 1;
+    }
 }
 
 }  // namespace yk
