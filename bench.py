@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-rate", type=float, default=0.35, help="Poisson arrivals per s")
     ap.add_argument("--serve-round", type=int, default=50, help="steps per serving round")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (reference "
+                         "kernel for scaling HBM byte counters) and exit")
     return ap.parse_args()
 
 
@@ -142,8 +145,22 @@ def cpu_baseline(calls):
     return json.loads(out.stdout.strip().splitlines()[-1])
 
 
+def copy_calibration(mib):
+    import torch
+    n = mib * 2 ** 20 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_mib": mib, "bytes_read": n * 8, "bytes_written": n * 8}))
+    return 0
+
+
 def main():
     args = parse()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(independent systems shard one per GPU; see the sharding design)",

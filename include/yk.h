@@ -65,7 +65,11 @@ enum yk_flags {
     YK_FLAG_TIMING = 1u << 2,
     /* Evaluate all tile pairs (AABB-culled) instead of the cluster-pair
      * lists; fixed at yk_create. */
-    YK_FLAG_ALLPAIRS = 1u << 3
+    YK_FLAG_ALLPAIRS = 1u << 3,
+    /* Spreiter-Walter velocity Verlet for the uniform field bfield along z
+     * (src/vel_verlet_3.f:20-100) instead of the predictor/corrector form;
+     * excludes YK_FLAG_KDK. */
+    YK_FLAG_SW = 1u << 4
 };
 
 enum yk_status {
@@ -167,7 +171,8 @@ int yk_accel(yk_system *sys, int32_t do_measurements, yk_meas *out);
 
 /* newton(do_measurements): one velocity-Verlet step of length dt.  Needs
  * accelerations of the current positions (a previous yk_accel/yk_newton).
- * src/vel_verlet_2.f:18-120 (src/vel_verlet_1.f with YK_FLAG_KDK). */
+ * src/vel_verlet_2.f:18-120 (src/vel_verlet_1.f:16-89 with YK_FLAG_KDK,
+ * src/vel_verlet_3.f:20-100 with YK_FLAG_SW). */
 int yk_newton(yk_system *sys, double dt, int32_t do_measurements, yk_meas *out);
 
 /* nsteps newton steps with state kept on the device.  Step s (1-based)

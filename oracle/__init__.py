@@ -57,6 +57,7 @@ def load():
         lib.yko_accel.argtypes = [P, C.c_int, _D, _D, _D, C.c_int, _D, _D]
         lib.yko_newton.argtypes = [P, C.c_int, C.c_double, _D, _D, _D, _D, _D, _D, C.c_int, _D]
         lib.yko_newton_kdk.argtypes = [P, C.c_int, C.c_double, _D, _D, _D, _D, _D, C.c_int, _D]
+        lib.yko_newton_sw.argtypes = [P, C.c_int, C.c_double, _D, _D, _D, _D, _D, C.c_int, _D]
         lib.yko_ttot.argtypes = [P, C.c_int, _D, _D]
         lib.yko_ttot.restype = C.c_double
         lib.yko_accel_truth.argtypes = [P, C.c_int, _D, _D, _D, _D]
@@ -79,6 +80,7 @@ def load_ref():
         lib.ykref_setup.restype = C.c_int
         lib.ykref_accel.argtypes = [C.c_int, _D, _D]
         lib.ykref_newton.argtypes = [C.c_int, C.c_double, C.c_int, _D]
+        lib.ykref_newton_alt.argtypes = [C.c_int, C.c_int, C.c_double, _D]
         lib.ykref_set_accel.argtypes = [_D]
         lib.ykref_get.argtypes = [_D, _D, _D]
         lib.ykref_ttot.restype = C.c_double
@@ -127,8 +129,9 @@ def accel(p, x, Z, A, do_meas=True, ref_f1=None):
     return a[:n].copy(), _meas(m)
 
 
-def newton(p, dt, x, v, vold, a, Z, A, do_meas=False, ref_f1=None, kdk=False):
-    """One step on copies of the state; returns (x, v, vold, a, meas)."""
+def newton(p, dt, x, v, vold, a, Z, A, do_meas=False, ref_f1=None, kdk=False, sw=False):
+    """One step on copies of the state; returns (x, v, vold, a, meas).
+    Integrator: vel_verlet_2 (default), vel_verlet_1 (kdk) or vel_verlet_3 (sw)."""
     n = x.shape[0]
     x = _arr(x, (n, 3)).copy()
     v = _arr(v, (n, 3)).copy()
@@ -140,9 +143,10 @@ def newton(p, dt, x, v, vold, a, Z, A, do_meas=False, ref_f1=None, kdk=False):
     m = np.zeros(NMEAS)
     ps = params_struct(p, ref_f1)
     lib = load()
-    if kdk:
-        rc = lib.yko_newton_kdk(C.byref(ps), n, dt, _ptr(x), _ptr(v), _ptr(ab), _ptr(Z), _ptr(A),
-                                int(do_meas), _ptr(m))
+    if kdk or sw:
+        fn = lib.yko_newton_sw if sw else lib.yko_newton_kdk
+        rc = fn(C.byref(ps), n, dt, _ptr(x), _ptr(v), _ptr(ab), _ptr(Z), _ptr(A),
+                int(do_meas), _ptr(m))
     else:
         rc = lib.yko_newton(C.byref(ps), n, dt, _ptr(x), _ptr(v), _ptr(vold), _ptr(ab), _ptr(Z),
                             _ptr(A), int(do_meas), _ptr(m))
@@ -217,6 +221,21 @@ def ref_newton(p, dt, x, v, a, Z, A, nsteps=1, meas_last=True):
     lib.ykref_set_accel(_ptr(a))
     m = np.zeros(NMEAS)
     lib.ykref_newton(int(nsteps), float(dt), int(meas_last), _ptr(m))
+    xo, vo, ao = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+    lib.ykref_get(_ptr(xo), _ptr(vo), _ptr(ao))
+    return xo, vo, ao, _meas(m)
+
+
+def ref_newton_alt(p, dt, x, v, a, Z, A, which, nsteps=1):
+    """nsteps of the reference's vel_verlet_1 (which='kdk') or vel_verlet_3
+    (which='sw') newton from (x, v, a).  Both call accel without an argument
+    in the reference; the harness gives them accel(.true.) (measuring)."""
+    n = x.shape[0]
+    ref_setup(p, x, v, Z, A)
+    lib = load_ref()
+    lib.ykref_set_accel(_ptr(_arr(a, (n, 3))))
+    m = np.zeros(NMEAS)
+    lib.ykref_newton_alt({"kdk": 1, "sw": 3}[which], int(nsteps), float(dt), _ptr(m))
     xo, vo, ao = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
     lib.ykref_get(_ptr(xo), _ptr(vo), _ptr(ao))
     return xo, vo, ao, _meas(m)

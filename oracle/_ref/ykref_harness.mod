@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:dfc1899c4db8c8cf
+﻿!mod$ v1 sum:3c7f01c474468e05
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ 49f872476a916429 n md_comm
 !need$ 770f10bc665f50df n md_globals
@@ -278,6 +278,12 @@ subroutine ykref_newton(nsteps,dtin,meas_last,meas) bind(c,name="ykref_newton")
 integer(4),value::nsteps
 real(8),value::dtin
 integer(4),value::meas_last
+real(8),intent(out)::meas(1_8:10_8)
+end
+subroutine ykref_newton_alt(which,nsteps,dtin,meas) bind(c,name="ykref_newton_alt")
+integer(4),value::which
+integer(4),value::nsteps
+real(8),value::dtin
 real(8),intent(out)::meas(1_8:10_8)
 end
 subroutine ykref_set_accel(ain_) bind(c,name="ykref_set_accel")

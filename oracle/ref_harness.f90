@@ -138,6 +138,29 @@ contains
     call pack_meas(meas)
   end subroutine ykref_newton
 
+  ! nsteps of the reference's alternate integrators, vel_verlet_1 (which = 1)
+  ! or vel_verlet_3 (which = 3).  Their sources are compiled with newton
+  ! renamed to ykref_newton_vv1 / ykref_newton_vv3 (both define a newton
+  ! without arguments, which would clash with vel_verlet_2's) and their
+  ! argument-less call of accel renamed to ykref_accel_measuring below.
+  subroutine ykref_newton_alt(which, nsteps, dtin, meas) bind(C, name='ykref_newton_alt')
+    integer(c_int), value :: which, nsteps
+    real(c_double), value :: dtin
+    real(c_double), intent(out) :: meas(10)
+    integer :: s
+    external :: ykref_newton_vv1, ykref_newton_vv3
+    dt = dtin
+    do s = 1, nsteps
+      pp = 0.0d0
+      if (which == 1) then
+        call ykref_newton_vv1()
+      else
+        call ykref_newton_vv3()
+      end if
+    end do
+    call pack_meas(meas)
+  end subroutine ykref_newton_alt
+
   subroutine ykref_set_accel(ain_) bind(C, name='ykref_set_accel')
     real(c_double), intent(in) :: ain_(3, n)
     a(:, 0:n-1) = ain_
@@ -174,3 +197,11 @@ contains
   end subroutine ykref_reset_timers
 
 end module ykref_harness
+
+
+! The accel the alternate integrators call (they pass no argument): the
+! reference's accel with measurements on, so ev, px and pp are formed.
+subroutine ykref_accel_measuring()
+  implicit none
+  call accel(.true.)
+end subroutine ykref_accel_measuring

@@ -426,6 +426,72 @@ int yko_newton_kdk(const yko_params *p, int n, double dt, double *x, double *v,
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* newton, src/vel_verlet_3.f:20-100: Spreiter-Walter velocity Verlet for a  */
+/* uniform B field along z ("inversion of ez x v").  Drift + wrap (:31-40),  */
+/* first velocity stage from v(t), a(t) (:45-54; the y update reads the x    */
+/* component already updated on the line before, as the reference does),    */
+/* force at t+dt (:58), second stage (:64-79).  The field constant is the    */
+/* reference's single-precision literal 3.218404e-20 (:48, :67) widened to   */
+/* double.  (Not link-compatible in the reference; kept as algorithm spec.)  */
+int yko_newton_sw(const yko_params *p, int n, double dt, double *x, double *v,
+                  double *a, const double *zii, const double *aii, int do_meas,
+                  double *meas)
+{
+    const double *xl = p->xl;
+    const double cb = (double)3.218404e-20f;
+    double afac = 0.5 * (dt * dt);                                /* :31 */
+    double halfdt = 0.5 * dt;                                     /* :45 */
+    for (int i = 0; i < n; ++i)                                   /* :33-39 */
+        for (int k = 0; k < 3; ++k) {
+            double xk = x[3 * i + k] + v[3 * i + k] * dt + afac * a[3 * i + k];
+            if (xk < 0.0) xk = xk + xl[k];
+            if (xk > xl[k]) xk = xk - xl[k];
+            x[3 * i + k] = xk;
+        }
+    for (int i = 0; i < n; ++i) {                                 /* :47-53 */
+        double *vi = v + 3 * i;
+        const double *ai = a + 3 * i;
+        double omega = cb * zii[i] * p->bfield / aii[i];
+        double omegap = halfdt * omega;
+        vi[0] = (1.0 - omegap * omegap) * vi[0] + 2.0 * omegap * vi[1] +
+                halfdt * (ai[0] + omegap * ai[1]);
+        vi[1] = (1.0 - omegap * omegap) * vi[1] - 2.0 * omegap * vi[0] +
+                halfdt * (ai[1] - omegap * ai[0]);
+        vi[2] = vi[2] + halfdt * ai[2];
+    }
+    if (yko_accel(p, n, x, zii, aii, do_meas, a, meas) != 0) return -1;
+    double vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
+    for (int i = 0; i < n; ++i) {                                 /* :66-78 */
+        double *vi = v + 3 * i;
+        const double *ai = a + 3 * i;
+        double omega = cb * zii[i] * p->bfield / aii[i];
+        double omegap = halfdt * omega;
+        vi[0] = (vi[0] + halfdt * (ai[0] + omegap * ai[1])) / (1.0 + omegap * omegap);
+        vi[1] = (vi[1] + halfdt * (ai[1] - omegap * ai[0])) / (1.0 + omegap * omegap);
+        vi[2] = vi[2] + halfdt * ai[2];
+        vxx = vxx + aii[i] * vi[0] * vi[0];
+        vxy = vxy + aii[i] * vi[0] * vi[1];
+        vxz = vxz + aii[i] * vi[0] * vi[2];
+        vyy = vyy + aii[i] * vi[1] * vi[1];
+        vyz = vyz + aii[i] * vi[1] * vi[2];
+        vzz = vzz + aii[i] * vi[2] * vi[2];
+    }
+    if (do_meas) {                                                /* :80-90 */
+        double vol = p->xl[0] * p->xl[1] * p->xl[2];
+        meas[9] = (0.5 * p->xmass) * (vxx + vyy + vzz) / (double)(float)n;
+        meas[3] = (meas[3] + p->xmass * vxx) / vol;
+        meas[4] = (meas[4] + p->xmass * vxy) / vol;
+        meas[5] = (meas[5] + p->xmass * vxz) / vol;
+        meas[6] = (meas[6] + p->xmass * vyy) / vol;
+        meas[7] = (meas[7] + p->xmass * vyz) / vol;
+        meas[8] = (meas[8] + p->xmass * vzz) / vol;
+    } else {
+        for (int k = 3; k < YKO_NMEAS; ++k) meas[k] = 0.0;
+    }
+    return 0;
+}
+
 /* ttot, src/gen_utils.f:40-55 */
 double yko_ttot(const yko_params *p, int n, const double *v, const double *aii)
 {

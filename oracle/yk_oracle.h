@@ -14,6 +14,7 @@
  *   yko_accel           <- accel            src/int_control.f:52-81
  *   yko_newton          <- newton           src/vel_verlet_2.f:18-120
  *   yko_newton_kdk      <- newton           src/vel_verlet_1.f:16-89 (algorithm spec)
+ *   yko_newton_sw       <- newton           src/vel_verlet_3.f:20-100 (algorithm spec)
  *   yko_ttot            <- ttot             src/gen_utils.f:40-55
  *   yko_accel_truth     <- same physics, full N x N sum in long double
  *
@@ -67,6 +68,11 @@ int yko_newton(const yko_params *p, int n, double dt, double *x, double *v,
 int yko_newton_kdk(const yko_params *p, int n, double dt, double *x, double *v,
                    double *a, const double *zii, const double *aii, int do_meas,
                    double *meas);
+
+/* One vel_verlet_3 (Spreiter-Walter, uniform B field along z) step. */
+int yko_newton_sw(const yko_params *p, int n, double dt, double *x, double *v,
+                  double *a, const double *zii, const double *aii, int do_meas,
+                  double *meas);
 
 double yko_ttot(const yko_params *p, int n, const double *v, const double *aii);
 

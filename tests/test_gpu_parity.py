@@ -205,6 +205,36 @@ def test_kdk_against_oracle():
     assert mg["ek"] == pytest.approx(m["ek"], rel=1e-12)
 
 
+@pytest.mark.parametrize("integrator", ["sw", "vv2"])
+@pytest.mark.parametrize("make", [lambda: W.ion_mixture(n=900, seed=72),
+                                  lambda: W.pure_ion(n=500, seed=73)])
+def test_bfield_integrators_against_oracle(make, integrator):
+    """Uniform B field along z: the Spreiter-Walter integrator
+    (src/vel_verlet_3.f) and vel_verlet_2's Lorentz term
+    (src/vel_verlet_2.f:61-69), three steps, against the oracle."""
+    w = make()
+    p = w.params.replace(integrator=integrator, bfield=2.0e17)
+    a, _ = oracle.accel(p, w.x, w.Z, w.A, True, ref_f1=0)
+    x, v, vold = w.x, w.v, np.zeros_like(w.v)
+    for st in range(3):
+        x, v, vold, a, m = oracle.newton(p, w.dt, x, v, vold, a, w.Z, w.A, st == 2,
+                                         sw=integrator == "sw")
+    with YukawaSystem(p, Z=w.Z, A=w.A, dt=w.dt) as s:
+        s.put_state(w.x, w.v)
+        s.accel(True)
+        s.newton(False)
+        s.newton(False)
+        mg = s.newton(True)
+        xg, vg, ag = s.get_state()
+    L = p.xl[0]
+    dx = np.abs(xg - x)
+    dx = np.minimum(dx, L - dx)
+    assert dx.max() <= 1e-12 * L
+    assert np.abs(vg - v).max() <= 1e-11 * np.abs(v).max()
+    assert mg["ek"] == pytest.approx(m["ek"], rel=1e-12)
+    assert mg["pp"][0] == pytest.approx(m["pp11"], rel=1e-11)
+
+
 def test_error_paths():
     from yukawamd_amd._lib import YkError
     w = W.ion_mixture(n=64, seed=81)

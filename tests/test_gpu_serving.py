@@ -56,14 +56,16 @@ def test_served_equals_alone():
 
 def test_admission_by_footprint():
     reqs = requests()
-    probe = Server(hbm_budget_bytes=2 ** 30, capacity_slots=16384, max_runs=8)
+    # the engine's default work shape (footprints then do not depend on the
+    # tuning table)
+    probe = Server(hbm_budget_bytes=2 ** 30, capacity_slots=16384, max_runs=8, tuning=[])
     sizes = {r.id: probe.footprint(r) for r in reqs}
     probe.close()
     budget = max(sizes.values())      # the largest run alone, or a few small ones
     first = [r for r in reqs if r.arrival_s == 0.0]
     assert sum(sizes[r.id] for r in first) > budget
     with Server(hbm_budget_bytes=budget, capacity_slots=16384, max_runs=8,
-                round_steps=10) as srv:
+                round_steps=10, tuning=[]) as srv:
         results = srv.serve(reqs)
         assert srv.peak_bytes <= budget
         s = summarize(results, 1.0, srv.device_ms)
@@ -71,7 +73,7 @@ def test_admission_by_footprint():
     first_done = min(r.done_s for r in results)
     waited = [r for r in results if r.request.arrival_s == 0.0 and r.admitted_s >= first_done]
     assert waited                      # some request was held back until a run finished
-    with Server(hbm_budget_bytes=budget, capacity_slots=16384, max_runs=8) as srv:
+    with Server(hbm_budget_bytes=budget, capacity_slots=16384, max_runs=8, tuning=[]) as srv:
         big = Request(id=99, arrival_s=0.0, n=16000, steps=1, gamma=170.0, kappa=1.0,
                       precision="mixed", seed=1)
         with pytest.raises(ValueError):

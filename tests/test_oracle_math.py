@@ -225,3 +225,46 @@ def test_auto_rncut_matches_closed_form_loop():
     r = auto_rncut()
     # the nuclear force at rncut is below 2^-53 * 1e-4 of the packet scale
     assert 10.0 < r < 13.0
+
+
+def test_spreiter_walter_reduces_to_kdk_without_field():
+    # vel_verlet_3 with B = 0: both stages become plain half kicks, so the step
+    # is vel_verlet_1's kick-drift-kick, bit for bit
+    w = W.ion_mixture(n=150, seed=17)
+    p = w.params
+    a0, _ = oracle.accel(p, w.x, w.Z, w.A, True, ref_f1=0)
+    z = np.zeros_like(w.v)
+    r1 = oracle.newton(p, w.dt, w.x, w.v, z, a0, w.Z, w.A, True, ref_f1=0, kdk=True)
+    r3 = oracle.newton(p, w.dt, w.x, w.v, z, a0, w.Z, w.A, True, ref_f1=0, sw=True)
+    for u, v in zip(r1[:4], r3[:4]):
+        np.testing.assert_array_equal(u, v)
+    assert r1[4] == r3[4]
+
+
+def test_spreiter_walter_free_particle_gyrates():
+    # no pair force ('B-field' sim type): a charge in B along z keeps v_z, drifts
+    # exactly, and its transverse velocity turns by the cyclotron angle
+    # -omega*dt (to first order in omega*dt) in the sense of F = q v x B
+    L = 1.0e6
+    dt = 2.0
+    p = SimParams(sim_type="B-field", xl=(L, L, L), rccut=10.0, bfield=1.0e16)
+    Z = np.array([8.0, 26.0])
+    A = np.array([16.0, 56.0])
+    x = np.array([[100.0, 200.0, 300.0], [400.0, 500.0, 600.0]])
+    v = np.array([[0.01, -0.02, 0.005], [-0.03, 0.01, -0.004]])
+    a = np.zeros((2, 3))
+    xo, vo, _, ao, _ = oracle.newton(p, dt, x, v, np.zeros_like(v), a, Z, A, sw=True)
+    np.testing.assert_array_equal(xo, x + v * dt)
+    np.testing.assert_array_equal(vo[:, 2], v[:, 2])
+    omega = np.float64(np.float32(3.218404e-20)) * Z * p.bfield / A
+    th = omega * dt
+    assert np.all(np.abs(th) < 1e-2) and np.all(np.abs(th) > 1e-4)
+    exp_vx = v[:, 0] + th * v[:, 1]
+    exp_vy = v[:, 1] - th * v[:, 0]
+    scale = np.abs(v[:, :2]).max(axis=1)
+    assert np.all(np.abs(vo[:, 0] - exp_vx) <= 2 * th ** 2 * scale)
+    assert np.all(np.abs(vo[:, 1] - exp_vy) <= 2 * th ** 2 * scale)
+    # transverse speed is kept to second order
+    s0 = np.hypot(v[:, 0], v[:, 1])
+    s1 = np.hypot(vo[:, 0], vo[:, 1])
+    assert np.all(np.abs(s1 - s0) <= 2 * th ** 2 * s0)

@@ -74,3 +74,22 @@ def test_reference_f1_last_particle():
     assert np.array_equal(ar[:-1], af[:-1])
     np.testing.assert_allclose(ar[-1] * w.Z[-1], af[-1], rtol=1e-15, atol=0)
     assert not np.allclose(ar[-1], af[-1])
+
+
+@pytest.mark.parametrize("which", ["kdk", "sw"])
+@pytest.mark.parametrize("name,make", CASES[:2])
+def test_alternate_integrators_bitexact(which, name, make):
+    """vel_verlet_1 (kick-drift-kick) and vel_verlet_3 (Spreiter-Walter, with a
+    field strong enough that the rotation terms matter) against the
+    reference's own sources, three measured steps."""
+    w = make()
+    p = w.params.replace(bfield=2.0e17) if which == "sw" else w.params
+    a0, _ = oracle.accel(p, w.x, w.Z, w.A, True, ref_f1=1)
+    x, v, vold, a = w.x, w.v, np.zeros_like(w.v), a0
+    for _ in range(3):
+        x, v, vold, a, m = oracle.newton(p, w.dt, x, v, vold, a, w.Z, w.A, True, ref_f1=1,
+                                         kdk=which == "kdk", sw=which == "sw")
+    xr, vr, ar, mr = oracle.ref_newton_alt(p, w.dt, w.x, w.v, a0, w.Z, w.A, which, nsteps=3)
+    assert np.array_equal(x, xr) and np.array_equal(v, vr) and np.array_equal(a, ar)
+    for k in ("ev", "ek", "px", "pp11", "pp12", "pp13", "pp22", "pp23", "pp33"):
+        assert m[k] == mr[k], k

@@ -480,7 +480,7 @@ int check_replica_params(const yk_params *p, int n)
         return bfail(YK_ERR_UNSUPPORTED, "batched replicas are ion systems (pure or mixture)");
     if (p->precision != YK_FP64 && p->precision != YK_MIXED)
         return bfail(YK_ERR_ARG, "precision=%d", p->precision);
-    if (p->bfield != 0.0 || (p->flags & YK_FLAG_KDK))
+    if (p->bfield != 0.0 || (p->flags & (YK_FLAG_KDK | YK_FLAG_SW)))
         return bfail(YK_ERR_UNSUPPORTED, "batched replicas use vel_verlet_2 without a B field");
     for (int k = 0; k < 3; ++k)
         if (!(p->xl[k] > 0.0)) return bfail(YK_ERR_ARG, "xl(%d)=%g", k + 1, p->xl[k]);

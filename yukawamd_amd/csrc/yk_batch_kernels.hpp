@@ -94,22 +94,25 @@ __device__ __forceinline__ void batch_item(const RepConsts &rc, const ExpCoeffs 
         gap2 = uniform(gap2);
         if (gap2 > rc.cull2) continue;
         const int j = st * TILE + lane;
-        sw[lane] = (j - rc.off < rc.n) ? pos[j] : make_double4(nanv, nanv, nanv, 0.0);
+        double4 pjs = (j - rc.off < rc.n) ? pos[j] : make_double4(nanv, nanv, nanv, 0.0);
+        pjs.w = stage_w<KIND, PREC>(pjs.w);
+        sw[lane] = pjs;
         __builtin_amdgcn_wave_barrier();
         const int self_jj = (st == t) ? lane : -1;
+        const int jcount = min(TILE, rc.n - (st * TILE - rc.off));   // padded sources skipped
         if (uniform(fast ? 1.0 : 0.0) != 0.0) {
             const double xs = pi.x - uniform(shift[0]);
             const double ys = pi.y - uniform(shift[1]);
             const double zs = pi.z - uniform(shift[2]);
 #pragma unroll 4
-            for (int jj = 0; jj < TILE; ++jj) {
+            for (int jj = 0; jj < jcount; ++jj) {
                 const double4 pj = sw[jj];
                 interact<KIND, PREC, MEAS>(c, pi.w, pj.w, xs - pj.x, ys - pj.y, zs - pj.z,
                                            jj == self_jj, acc);
             }
         } else {
 #pragma unroll 2
-            for (int jj = 0; jj < TILE; ++jj) {
+            for (int jj = 0; jj < jcount; ++jj) {
                 const double4 pj = sw[jj];
                 interact<KIND, PREC, MEAS>(c, pi.w, pj.w,
                                            min_image(pi.x - pj.x, c.L[0], c.invL[0]),

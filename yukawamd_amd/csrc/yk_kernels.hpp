@@ -55,7 +55,8 @@ struct ForceConsts {
 };
 
 struct StepConsts {
-    double dt, afac, hdt, bfield_c;  // bfield_c = 3.218404e-20 * bfield
+    double dt, afac, hdt;
+    double bconst, bfield;           // omega_i = bconst * Z_i * bfield / A_i
     double L[3];
     double cnum;                     // frp*vc (mix), frp*zi*zi*vc (pure), 1 (nn)
     int n, ref_f1, kind, has_bfield;
@@ -99,9 +100,29 @@ __device__ __forceinline__ double min_image(double d, double L, double invL)
     return __builtin_fma(-L, __builtin_rint(d * invL), d);
 }
 
+// Source charge as staged in LDS.  The mixed-precision ion forms read it as
+// a float: the staging copy keeps (float)Z in the low 32 bits of the w slot,
+// one conversion per staged source instead of one per pair.  Every other form
+// keeps the double.
+template <int KIND, int PREC>
+__device__ __forceinline__ double stage_w(double z)
+{
+    if constexpr (PREC == 1 && KIND != NN)
+        return __builtin_bit_cast(double,
+                                  (unsigned long long)__builtin_bit_cast(unsigned, (float)z));
+    else
+        return z;
+}
+
+__device__ __forceinline__ float staged_zf(double w)
+{
+    return __builtin_bit_cast(float, (unsigned)__builtin_bit_cast(unsigned long long, w));
+}
+
 // Pair interaction given the minimum-image displacement d = x_i - x_j.
 // PREC 0 = FP64, 1 = mixed (FP64 displacement, r^2, cutoff test and
-// accumulation; FP32 1/r, exp and Gaussians).
+// accumulation; FP32 1/r, exp and Gaussians).  zj is the source's staged w
+// (stage_w).  Callers never pass padded sources.
 // SELF_BY_R2: the self pair is recognised by r^2 == 0 instead of the caller's
 // flag (pair-list kernel: a source is its own target exactly when d = 0).
 template <int KIND, int PREC, bool MEAS, bool SELF_BY_R2 = false>
@@ -112,6 +133,9 @@ __device__ __forceinline__ void interact(const ForceConsts &c, double zi, double
     double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
     if constexpr (SELF_BY_R2) self = !(r2 > 0.0);
     if constexpr (KIND != NN) {
+        // Guarded: a wave whose lanes are all outside the cutoff skips the
+        // evaluation (measured faster than a branch-free select form, since
+        // culled tiles and list entries leave many such waves).
         if (r2 <= c.rc2 && !self) {
             double s, phi = 0.0;
             if constexpr (PREC == 0) {
@@ -127,16 +151,17 @@ __device__ __forceinline__ void interact(const ForceConsts &c, double zi, double
                     if (KIND == MIX) phi *= zj;
                 }
             } else {
-                float r2f = (float)r2;
-                float rinv = rsqrt32(r2f);
-                float r = r2f * rinv;
-                float ex = exp2_32(-c.mul2ef * r);
-                float exr = ex * rinv;
+                const float r2f = (float)r2;
+                const float rinv = rsqrt32(r2f);
+                const float r = r2f * rinv;
+                const float ex = exp2_32(-c.mul2ef * r);
+                const float exr = ex * rinv;
                 float fc = __builtin_fmaf(c.muf, r, 1.0f) * exr * (rinv * rinv);
-                s = (double)((KIND == MIX) ? (float)zj * fc : fc);
+                if constexpr (KIND == MIX) fc = staged_zf(zj) * fc;
+                s = (double)fc;
                 if constexpr (MEAS) {
                     phi = (double)exr - c.evcut;
-                    if (KIND == MIX) phi *= zj;
+                    if constexpr (KIND == MIX) phi *= (double)staged_zf(zj);
                 }
             }
             accumulate<MEAS>(acc, s, dx, dy, dz, r2, phi);
@@ -230,8 +255,10 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const double4 *__restrict__ pos
     for (int st = s0; st < s1; ++st) {
         const int jb = st * STAGE;
         const int jl = jb + threadIdx.x;
-        // padded sources are NaN: every comparison with them is false
-        sj[threadIdx.x] = (jl < n) ? pos[jl] : make_double4(nanv, nanv, nanv, 0.0);
+        // padded sources are NaN and never evaluated (loops stop at n)
+        double4 pjs = (jl < n) ? pos[jl] : make_double4(nanv, nanv, nanv, 0.0);
+        pjs.w = stage_w<KIND, PREC>(pjs.w);
+        sj[threadIdx.x] = pjs;
         __syncthreads();
         if (wave_live) {
 #pragma unroll 1
@@ -258,19 +285,20 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const double4 *__restrict__ pos
                 ++nswept;
                 const int self_jj = (jt == itile) ? lane : -1;
                 const double4 *src = sj + q * TILE;
+                const int jcount = min(TILE, n - jt * TILE);
                 if (uniform(fast ? 1.0 : 0.0) != 0.0) {
                     const double xs = pi.x - uniform(shift[0]);
                     const double ys = pi.y - uniform(shift[1]);
                     const double zs = pi.z - uniform(shift[2]);
 #pragma unroll 4
-                    for (int jj = 0; jj < TILE; ++jj) {
+                    for (int jj = 0; jj < jcount; ++jj) {
                         const double4 pj = src[jj];
                         interact<KIND, PREC, MEAS>(c, pi.w, pj.w, xs - pj.x, ys - pj.y,
                                                    zs - pj.z, jj == self_jj, acc);
                     }
                 } else {
 #pragma unroll 2
-                    for (int jj = 0; jj < TILE; ++jj) {
+                    for (int jj = 0; jj < jcount; ++jj) {
                         const double4 pj = src[jj];
                         interact<KIND, PREC, MEAS>(
                             c, pi.w, pj.w, min_image(pi.x - pj.x, c.L[0], c.invL[0]),
@@ -448,6 +476,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_list(const double4 *__restrict_
                 pj.y += c.L[1] * (double)((int)((e_cur >> 28) & 3u) - 1);
                 pj.z += c.L[2] * (double)((int)((e_cur >> 30) & 3u) - 1);
             }
+            pj.w = stage_w<KIND, PREC>(pj.w);
             sw[lane] = pj;
         }
         __builtin_amdgcn_wave_barrier();
@@ -659,7 +688,7 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(
         for (int k = 0; k < 3; ++k) a[k] = (sc.cnum * (fz_i * f[k])) / d;
         if constexpr (MODE == 1) {
             if (sc.has_bfield) {   // src/vel_verlet_2.f:61-69
-                double omega = sc.bfield_c * zi / aii[i];
+                double omega = sc.bconst * zi * sc.bfield / aii[i];
                 a[0] = a[0] + vel[nn + i] * omega;
                 a[1] = a[1] - vel[i] * omega;
             }
@@ -680,6 +709,14 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(
                 v[k] = vel[k * nn + i] + sc.hdt * a[k];
                 vel[k * nn + i] = v[k];
             }
+        } else if constexpr (MODE == 3) {   // src/vel_verlet_3.f:66-71
+            const double omega = sc.bconst * zi * sc.bfield / aii[i];
+            const double omegap = sc.hdt * omega;
+            v[0] = (vel[i] + sc.hdt * (a[0] + omegap * a[1])) / (1.0 + omegap * omegap);
+            v[1] = (vel[nn + i] + sc.hdt * (a[1] - omegap * a[0])) / (1.0 + omegap * omegap);
+            v[2] = vel[2 * nn + i] + sc.hdt * a[2];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) vel[k * nn + i] = v[k];
         }
         if constexpr (MEAS) {
             double zs = (KIND == MIX) ? zi : 1.0;   // per-i zii factor (:199-207)
@@ -704,12 +741,14 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(
 // Drift (+ velocity predictor or opening half kick), then the tile AABBs of
 // the new positions and the largest displacement since the pair lists were
 // built.  MODE 1: vel_verlet_2 (src/vel_verlet_2.f:33-53); MODE 2: drift +
-// half kick (src/vel_verlet_1.f:28-48).
+// half kick (src/vel_verlet_1.f:28-48); MODE 3: drift + first Spreiter-Walter
+// velocity stage (src/vel_verlet_3.f:31-54).
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_drift(double4 *__restrict__ pos,
                                                  double *__restrict__ vel,
                                                  double *__restrict__ vold,
                                                  const double *__restrict__ acc,
+                                                 const double *__restrict__ aii,
                                                  StepConsts sc, int ntiles,
                                                  TileBox *__restrict__ box,
                                                  const double4 *__restrict__ xref,
@@ -730,6 +769,8 @@ __global__ __launch_bounds__(BLOCK) void k_drift(double4 *__restrict__ pos,
                 xk = x[k] + v * sc.dt + sc.afac * a;   // contraction is off in this TU
                 vold[k * nn + i] = v;
                 vel[k * nn + i] = v + sc.dt * a;
+            } else if constexpr (MODE == 3) {   // src/vel_verlet_3.f:35
+                xk = x[k] + v * sc.dt + sc.afac * a;
             } else {   // src/vel_verlet_1.f:30-48
                 xk = x[k] + v * sc.dt + a * sc.afac;
                 vel[k * nn + i] = v + sc.hdt * a;
@@ -742,6 +783,19 @@ __global__ __launch_bounds__(BLOCK) void k_drift(double4 *__restrict__ pos,
             x[k] = xk;
         }
         pos[i] = make_double4(x[0], x[1], x[2], p.w);
+        if constexpr (MODE == 3) {   // src/vel_verlet_3.f:47-53
+            const double a0 = acc[i], a1 = acc[nn + i], a2 = acc[2 * nn + i];
+            const double omega = sc.bconst * p.w * sc.bfield / aii[i];
+            const double omegap = sc.hdt * omega;
+            const double c1 = 1.0 - omegap * omegap;
+            // the y update reads the x component updated just before it
+            const double v0 = c1 * vel[i] + 2.0 * omegap * vel[nn + i] +
+                              sc.hdt * (a0 + omegap * a1);
+            const double v1 = c1 * vel[nn + i] - 2.0 * omegap * v0 + sc.hdt * (a1 - omegap * a0);
+            vel[i] = v0;
+            vel[nn + i] = v1;
+            vel[2 * nn + i] = vel[2 * nn + i] + sc.hdt * a2;
+        }
     }
     const int tile = i / TILE;
     if (tile < ntiles) tile_box_store(x, valid, tile, box);
@@ -791,8 +845,11 @@ struct MeasConsts {
     double cnum, rho, kT, xmass, vol;
     float three_n;   // 3.0*n as the reference's real(4) product
     float n_f;       // float(n)
-    int mode;        // 0 accel, 1/2 newton, 3 ttot
+    int mode;        // MEAS_ACCEL, an integrator mode (1, 2, 3), or MEAS_TTOT
 };
+
+constexpr int MEAS_ACCEL = 0;
+constexpr int MEAS_TTOT = 16;
 
 static __global__ __launch_bounds__(BLOCK) void k_meas_final(const double *__restrict__ blockpart,
                                                       int nblocks, MeasConsts mc,
@@ -823,7 +880,7 @@ static __global__ __launch_bounds__(BLOCK) void k_meas_final(const double *__res
     }
     if (threadIdx.x != 0) return;
     MeasOut m;
-    if (mc.mode == 3) {
+    if (mc.mode == MEAS_TTOT) {
         m.ev = 0; m.px = 0; m.vir = 0;
         for (int k = 0; k < 6; ++k) m.pp[k] = 0;
         m.ek = (0.5 * mc.xmass * tot[0]) / (double)mc.n_f;
@@ -837,7 +894,7 @@ static __global__ __launch_bounds__(BLOCK) void k_meas_final(const double *__res
     m.px = mc.rho * (mc.kT + vir / (double)mc.three_n);
     for (int k = 0; k < 6; ++k) m.pp[k] = mc.cnum * tot[2 + k];
     m.ek = 0.0;
-    if (mc.mode != 0) {   // src/vel_verlet_2.f:96-106
+    if (mc.mode != MEAS_ACCEL) {   // src/vel_verlet_2.f:96-106, src/vel_verlet_3.f:80-90
         m.ek = (0.5 * mc.xmass) * (tot[8] + tot[11] + tot[13]) / (double)mc.n_f;
         for (int k = 0; k < 6; ++k) m.pp[k] = (m.pp[k] + mc.xmass * tot[8 + k]) / mc.vol;
     }

@@ -138,6 +138,9 @@ int check_params(const yk_params *p)
         return fail(YK_ERR_ARG, "pure-ion needs ai > 0 (got %g)", p->ai);
     if (p->sim_type == YK_NUCLEON && p->nuclear_on && !(p->xpacket > 0.0))
         return fail(YK_ERR_ARG, "xpacket=%g must be positive", p->xpacket);
+    if ((p->flags & YK_FLAG_KDK) && (p->flags & YK_FLAG_SW))
+        return fail(YK_ERR_ARG, "YK_FLAG_KDK and YK_FLAG_SW select different integrators");
+    if (!std::isfinite(p->bfield)) return fail(YK_ERR_ARG, "bfield must be finite");
     return YK_OK;
 }
 
@@ -255,7 +258,10 @@ StepConsts step_consts(const yk_system *s, double dt)
     c.dt = dt;
     c.afac = kdk ? 0.5 * dt * dt : 0.5 * (dt * dt);   // vel_verlet_1.f:28 / vel_verlet_2.f:33
     c.hdt = 0.5 * dt;
-    c.bfield_c = 3.218404e-20 * p.bfield;
+    // field constant: vel_verlet_2's double literal (src/vel_verlet_2.f:65),
+    // vel_verlet_3's single-precision literal widened (src/vel_verlet_3.f:48)
+    c.bconst = (p.flags & YK_FLAG_SW) ? (double)3.218404e-20f : 3.218404e-20;
+    c.bfield = p.bfield;
     c.has_bfield = (p.bfield != 0.0) ? 1 : 0;
     for (int k = 0; k < 3; ++k) c.L[k] = p.xl[k];
     switch (p.sim_type) {
@@ -448,11 +454,13 @@ void launch_fin_kind(yk_system *s, const StepConsts &sc, bool meas, int mode)
     if (meas) {
         if (mode == 0) launch_fin_k<KIND, true, 0>(s, sc);
         else if (mode == 1) launch_fin_k<KIND, true, 1>(s, sc);
-        else launch_fin_k<KIND, true, 2>(s, sc);
+        else if (mode == 2) launch_fin_k<KIND, true, 2>(s, sc);
+        else launch_fin_k<KIND, true, 3>(s, sc);
     } else {
         if (mode == 0) launch_fin_k<KIND, false, 0>(s, sc);
         else if (mode == 1) launch_fin_k<KIND, false, 1>(s, sc);
-        else launch_fin_k<KIND, false, 2>(s, sc);
+        else if (mode == 2) launch_fin_k<KIND, false, 2>(s, sc);
+        else launch_fin_k<KIND, false, 3>(s, sc);
     }
 }
 
@@ -479,12 +487,18 @@ int launch_drift(yk_system *s, double dt)
 {
     StepConsts sc = step_consts(s, dt);
     const double4 *xr = s->use_list ? s->xref : nullptr;
-    if (s->p.flags & YK_FLAG_KDK)
+    if (s->p.flags & YK_FLAG_SW)
+        hipLaunchKernelGGL(k_drift<3>, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos,
+                           s->vel, s->vold, s->acc, s->aii, sc, s->ntiles, s->box, xr,
+                           s->maxdisp);
+    else if (s->p.flags & YK_FLAG_KDK)
         hipLaunchKernelGGL(k_drift<2>, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos,
-                           s->vel, s->vold, s->acc, sc, s->ntiles, s->box, xr, s->maxdisp);
+                           s->vel, s->vold, s->acc, s->aii, sc, s->ntiles, s->box, xr,
+                           s->maxdisp);
     else
         hipLaunchKernelGGL(k_drift<1>, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos,
-                           s->vel, s->vold, s->acc, sc, s->ntiles, s->box, xr, s->maxdisp);
+                           s->vel, s->vold, s->acc, s->aii, sc, s->ntiles, s->box, xr,
+                           s->maxdisp);
     YK_HIP(hipGetLastError());
     return YK_OK;
 }
@@ -545,7 +559,7 @@ int device_step(yk_system *s, double dt, bool meas, int slot)
     rc = launch_drift(s, dt);
     if (rc) return rc;
     if ((rc = launch_pairs(s, meas))) return rc;
-    const int mode = (s->p.flags & YK_FLAG_KDK) ? 2 : 1;
+    const int mode = (s->p.flags & YK_FLAG_SW) ? 3 : (s->p.flags & YK_FLAG_KDK) ? 2 : 1;
     return launch_finalize(s, dt, meas, mode, slot);
 }
 
@@ -879,7 +893,7 @@ int yk_ttot(yk_system *s, double *eka)
     hipLaunchKernelGGL(k_ttot, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->vel, s->aii,
                        s->n, s->blockpart);
     YK_HIP(hipGetLastError());
-    MeasConsts mc = meas_consts(s, 3);
+    MeasConsts mc = meas_consts(s, MEAS_TTOT);
     hipLaunchKernelGGL(k_meas_final, dim3(1), dim3(BLOCK), 0, s->stream, s->blockpart,
                        s->nblocks, mc, s->d_meas);
     YK_HIP(hipGetLastError());

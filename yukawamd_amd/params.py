@@ -27,6 +27,7 @@ FLAG_REF_F1 = 1 << 0
 FLAG_KDK = 1 << 1
 FLAG_TIMING = 1 << 2
 FLAG_ALLPAIRS = 1 << 3
+FLAG_SW = 1 << 4
 
 
 @dataclass
@@ -55,6 +56,7 @@ class SimParams:
     precision: str = "fp64"             # 'fp64' | 'mixed'
     ref_compat_f1: bool = False         # keep the mixture kernel's last-particle defect
     integrator: str = "vv2"             # 'vv2' (vel_verlet_2.f) | 'kdk' (vel_verlet_1.f)
+                                        # | 'sw' (vel_verlet_3.f, Spreiter-Walter B field)
     timing: bool = False
     resort_every: int = 0               # device re-sort cadence (0 = library default)
     allpairs: bool = False              # AABB-culled tile pairs instead of pair lists
@@ -65,8 +67,8 @@ class SimParams:
                              "simulation type")
         if self.precision not in PRECISIONS:
             raise ValueError(f"precision={self.precision!r} is not 'fp64' or 'mixed'")
-        if self.integrator not in ("vv2", "kdk"):
-            raise ValueError(f"integrator={self.integrator!r} is not 'vv2' or 'kdk'")
+        if self.integrator not in ("vv2", "kdk", "sw"):
+            raise ValueError(f"integrator={self.integrator!r} is not 'vv2', 'kdk' or 'sw'")
         self.xl = tuple(float(v) for v in self.xl)
 
     @property
@@ -84,6 +86,8 @@ class SimParams:
             f |= FLAG_REF_F1
         if self.integrator == "kdk":
             f |= FLAG_KDK
+        if self.integrator == "sw":
+            f |= FLAG_SW
         if self.timing:
             f |= FLAG_TIMING
         if self.allpairs:
