@@ -1,20 +1,32 @@
 """Benchmark of the Yukawa pair-force + velocity-Verlet hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
+                    [--no-serve] [--forms fp64,mixed]
 
-Workload: the 27,648-ion crust mixture (0.6 Z=8 A=16, 0.3 Z=26 A=56,
+Headline workload: the 27,648-ion crust mixture (0.6 Z=8 A=16, 0.3 Z=26 A=56,
 0.1 Z=34 A=80; n = 7.18e-5 fm^-3, kappa = 0.7, Gamma(Z=8) = 100, cutoff
 min(L/2, 12 lambda), dt = 0.05/omega_p), positions uniform random from seed
 1001, state resident in HBM before timing.  A step is one full velocity-Verlet
-step (drift + all-pairs force + kick), with energies/pressure measured every
-10th step.  Both precision forms run; the headline ``value`` is the FP64
-drop-in form, the mixed form is reported beside it.
+step (drift + pair forces from the device-built pair lists + corrector), with
+energies/pressure measured every 10th step.  ``value`` is the FP64 drop-in
+form; the mixed-precision form is reported beside it.
 
-Prints ONE JSON line.  ``value`` = pair interactions/s = N(N-1)/2 x steps /
-wall seconds.  ``roofline`` is the FP64 pair kernel against the FP64 vector
-peak, with the kernel's average duration from HIP events recorded on the
-stream it runs on.  ``cpu_baseline`` times the reference's own Fortran OpenMP
-build (oracle/_ref) on this host's cores on a bounded sample.
+Further sections of the same JSON line:
+  forms.mixed_batched  4 replicas of the same mixture (seeds 2001..) stepped
+                       together by the batched engine in mixed precision;
+  batched_ensemble     256 replicas (seed 1003: N log-uniform [1024, 16384],
+                       pure or mixture, Gamma U[150,220], kappa U[0.5,2]),
+                       500 mixed steps in one persistent launch per step;
+  serving              the first requests of the seeded serving stream served
+                       with continuous batching: p50/p99 completion latency and
+                       aggregate pair interactions/s;
+  roofline             the FP64 pair-list kernel against the FP64 vector peak,
+                       its average duration from HIP events recorded on the
+                       stream it runs on, and its profiled HBM bytes;
+  cpu_baseline         the reference's own Fortran OpenMP build (oracle/_ref)
+                       on this host's cores, on a bounded sample.
+
+``value`` = pair interactions/s = N(N-1)/2 x steps / wall seconds.
 """
 from __future__ import annotations
 
@@ -34,6 +46,12 @@ PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector (packed)
 FLOPS_PER_PAIR_FP64 = 90     # directed pair, force-only step, special functions costed
 FLOPS_PER_PAIR_MEAS = 109    # measurement step
 FM_C_SECONDS = 3.33564e-24
+# HBM bytes per launch of the FP64 force-only pair-list kernel at N = 27,648,
+# from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scale factors from a plain
+# 256 MiB copy); counters cannot be read inside a timed run, so the figure is
+# the profiled one, with its commands, in the file named here.
+TRAFFIC_BYTES_PER_LAUNCH = 7.772e7
+TRAFFIC_SOURCE = "profiles/hbm_traffic_k_pairs_list_ion_mixture_27648.md"
 
 
 def parse():
@@ -238,13 +256,16 @@ def main():
         line["serving"] = run_serving(args.serve_requests, args.serve_rate, args.serve_round)
     line["roofline"] = {
         "bound": "valu",
-        "kernel": "k_pairs<MIX,FP64>",
+        "peak_kind": "FP64 vector",
+        "kernel": "k_pairs_list<MIX,FP64>",
         "achieved": f64["force_kernel_tflops"],
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
         "frac": f64["force_kernel_tflops"] / PEAK_FP64_TFLOPS,
-        "traffic": None,
+        "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+        "traffic_source": TRAFFIC_SOURCE,
         "flops_per_directed_pair": FLOPS_PER_PAIR_FP64,
+        "directed_pairs_per_launch": f64["directed_pairs_evaluated_per_launch"],
     }
     if not args.no_cpu:
         cb = cpu_baseline(args.cpu_calls)

@@ -1,5 +1,5 @@
-﻿!mod$ v1 sum:dbef63210b5b7c92
-!need$ fead504c5b658531 n yk_bind
+﻿!mod$ v1 sum:48e667c7f08623b3
+!need$ d08c980de4d7a69b n yk_bind
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins
 module md_yk_shim_state
@@ -88,6 +88,7 @@ use yk_bind,only:yk_flag_ref_f1
 use yk_bind,only:yk_flag_kdk
 use yk_bind,only:yk_flag_timing
 use yk_bind,only:yk_flag_allpairs
+use yk_bind,only:yk_flag_sw
 use yk_bind,only:yk_ok
 use yk_bind,only:yk_params_t
 use yk_bind,only:yk_meas_t
@@ -105,7 +106,19 @@ use yk_bind,only:yk_run
 use yk_bind,only:yk_ttot
 use yk_bind,only:yk_get_timing
 use yk_bind,only:yk_reset_timing
+use yk_bind,only:yk_batch_create
+use yk_bind,only:yk_batch_destroy
+use yk_bind,only:yk_batch_footprint
+use yk_bind,only:yk_batch_add
+use yk_bind,only:yk_batch_remove
+use yk_bind,only:yk_batch_set_tuning
+use yk_bind,only:yk_batch_set_resort
+use yk_bind,only:yk_batch_run
+use yk_bind,only:yk_batch_status
+use yk_bind,only:yk_batch_get
+use yk_bind,only:yk_batch_last_run
 use yk_bind,only:yk_error_message
+use yk_bind,only:yk_batch_error_message
 use,intrinsic::__fortran_builtins,only:iso_c_binding$__fortran_builtins$c_associated_c_ptr=>c_associated_c_ptr
 private::iso_c_binding$__fortran_builtins$c_associated_c_ptr
 type(c_ptr),save::handle

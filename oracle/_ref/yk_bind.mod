@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:fead504c5b658531
+﻿!mod$ v1 sum:d08c980de4d7a69b
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins
 module yk_bind
@@ -164,6 +164,7 @@ integer(4),parameter::yk_flag_ref_f1=1_4
 integer(4),parameter::yk_flag_kdk=2_4
 integer(4),parameter::yk_flag_timing=4_4
 integer(4),parameter::yk_flag_allpairs=8_4
+integer(4),parameter::yk_flag_sw=16_4
 integer(4),parameter::yk_ok=0_4
 type,bind(c)::yk_params_t
 integer(4)::sim_type=0_4
@@ -326,6 +327,128 @@ type(c_ptr),value::sys
 integer(4)::yk_reset_timing
 end
 end interface
+private::yk_batch_last_error_c
+interface
+function yk_batch_last_error_c() bind(c,name="yk_batch_last_error")
+import::c_ptr
+type(c_ptr)::yk_batch_last_error_c
+end
+end interface
+interface
+function yk_batch_create(capacity,max_replicas,max_records,b) bind(c,name="yk_batch_create")
+import::c_ptr
+integer(8),value::capacity
+integer(4),value::max_replicas
+integer(4),value::max_records
+type(c_ptr),intent(out)::b
+integer(4)::yk_batch_create
+end
+end interface
+interface
+subroutine yk_batch_destroy(b) bind(c,name="yk_batch_destroy")
+import::c_ptr
+type(c_ptr),value::b
+end
+end interface
+interface
+function yk_batch_footprint(b,n,slots,bytes) bind(c,name="yk_batch_footprint")
+import::c_ptr
+type(c_ptr),value::b
+integer(4),value::n
+integer(8),intent(out)::slots
+integer(8),intent(out)::bytes
+integer(4)::yk_batch_footprint
+end
+end interface
+interface
+function yk_batch_add(b,params,n,zii,aii,x,v,dt,steps,id) bind(c,name="yk_batch_add")
+import::c_ptr
+import::yk_params_t
+type(c_ptr),value::b
+type(yk_params_t),intent(in)::params
+integer(4),value::n
+real(8),intent(in)::zii(1_8:*)
+real(8),intent(in)::aii(1_8:*)
+real(8),intent(in)::x(1_8:*)
+real(8),intent(in)::v(1_8:*)
+real(8),value::dt
+integer(4),value::steps
+integer(4),intent(out)::id
+integer(4)::yk_batch_add
+end
+end interface
+interface
+function yk_batch_remove(b,id) bind(c,name="yk_batch_remove")
+import::c_ptr
+type(c_ptr),value::b
+integer(4),value::id
+integer(4)::yk_batch_remove
+end
+end interface
+interface
+function yk_batch_set_tuning(b,nbuckets,n_upper,chunk_tiles,waves) bind(c,name="yk_batch_set_tuning")
+import::c_ptr
+type(c_ptr),value::b
+integer(4),value::nbuckets
+integer(4),intent(in)::n_upper(1_8:*)
+integer(4),intent(in)::chunk_tiles(1_8:*)
+integer(4),intent(in)::waves(1_8:*)
+integer(4)::yk_batch_set_tuning
+end
+end interface
+interface
+function yk_batch_set_resort(b,every) bind(c,name="yk_batch_set_resort")
+import::c_ptr
+type(c_ptr),value::b
+integer(4),value::every
+integer(4)::yk_batch_set_resort
+end
+end interface
+interface
+function yk_batch_run(b,nsteps,nind) bind(c,name="yk_batch_run")
+import::c_ptr
+type(c_ptr),value::b
+integer(4),value::nsteps
+integer(4),value::nind
+integer(4)::yk_batch_run
+end
+end interface
+interface
+function yk_batch_status(b,id,steps_left,nrec) bind(c,name="yk_batch_status")
+import::c_ptr
+type(c_ptr),value::b
+integer(4),value::id
+integer(4),intent(out)::steps_left
+integer(4),intent(out)::nrec
+integer(4)::yk_batch_status
+end
+end interface
+interface
+function yk_batch_get(b,id,x,v,a,init,records,rec_cap,nrec) bind(c,name="yk_batch_get")
+import::c_ptr
+import::yk_meas_t
+type(c_ptr),value::b
+integer(4),value::id
+real(8),intent(inout)::x(1_8:*)
+real(8),intent(inout)::v(1_8:*)
+real(8),intent(inout)::a(1_8:*)
+type(yk_meas_t),intent(inout)::init
+type(yk_meas_t),intent(inout)::records(1_8:*)
+integer(4),value::rec_cap
+integer(4),intent(out)::nrec
+integer(4)::yk_batch_get
+end
+end interface
+interface
+function yk_batch_last_run(b,ms,steps,items) bind(c,name="yk_batch_last_run")
+import::c_ptr
+type(c_ptr),value::b
+real(8),intent(out)::ms
+integer(8),intent(out)::steps
+integer(8),intent(out)::items
+integer(4)::yk_batch_last_run
+end
+end interface
 private::c_strlen
 interface
 function c_strlen(s) bind(c,name="strlen")
@@ -334,8 +457,16 @@ type(c_ptr),value::s
 integer(8)::c_strlen
 end
 end interface
+private::c_string
 contains
 function yk_error_message() result(msg)
+character(:,1),allocatable::msg
+end
+function yk_batch_error_message() result(msg)
+character(:,1),allocatable::msg
+end
+function c_string(p) result(msg)
+type(c_ptr),intent(in)::p
 character(:,1),allocatable::msg
 end
 end

@@ -40,6 +40,13 @@ def test_python_binding_covers_the_header():
     assert set(declared_functions()) <= set(_lib.SIGNATURES)
 
 
+def test_fortran_module_binds_the_header():
+    """yk_bind.f90 (the Fortran host's interface module) binds every C entry."""
+    f90 = (HEADER.parent.parent / "yukawamd_amd" / "fortran" / "yk_bind.f90").read_text()
+    bound = set(re.findall(r"bind\(C,\s*name='(yk_[a-z_0-9]+)'\)", f90))
+    assert set(declared_functions()) <= bound, set(declared_functions()) - bound
+
+
 def test_abi_version_and_error_string():
     lib = _lib.load()
     assert lib.yk_abi_version() == 1

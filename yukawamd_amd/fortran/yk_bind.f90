@@ -10,6 +10,8 @@
 !   yk_accel   accel(do_measurements)   src/int_control.f:52-81
 !   yk_newton  newton(do_measurements)  src/vel_verlet_2.f:18-120
 !   yk_run     nsteps of newton with the md.F measurement cadence (src/md.F:129)
+!   yk_batch_* many independent ion systems advanced together (no reference
+!              counterpart: the reference runs one system per process)
 !-------------------------------------------------------------------------------
 module yk_bind
   use iso_c_binding
@@ -65,6 +67,9 @@ module yk_bind
   public :: yk_abi_version, yk_create, yk_destroy, yk_set_params, yk_put_state, &
             yk_put_accel, yk_get_state, yk_accel, yk_newton, yk_run, yk_ttot, &
             yk_get_timing, yk_reset_timing, yk_error_message
+  public :: yk_batch_create, yk_batch_destroy, yk_batch_footprint, yk_batch_add, &
+            yk_batch_remove, yk_batch_set_tuning, yk_batch_set_resort, yk_batch_run, &
+            yk_batch_status, yk_batch_get, yk_batch_last_run, yk_batch_error_message
 
   interface
     integer(c_int32_t) function yk_abi_version() bind(C, name='yk_abi_version')
@@ -152,6 +157,94 @@ module yk_bind
       type(c_ptr), value :: sys
     end function
 
+    type(c_ptr) function yk_batch_last_error_c() bind(C, name='yk_batch_last_error')
+      import :: c_ptr
+    end function
+
+    integer(c_int) function yk_batch_create(capacity, max_replicas, max_records, b) &
+        bind(C, name='yk_batch_create')
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      integer(c_int64_t), value :: capacity
+      integer(c_int32_t), value :: max_replicas, max_records
+      type(c_ptr), intent(out) :: b
+    end function
+
+    subroutine yk_batch_destroy(b) bind(C, name='yk_batch_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: b
+    end subroutine
+
+    integer(c_int) function yk_batch_footprint(b, n, slots, bytes) &
+        bind(C, name='yk_batch_footprint')
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: n
+      integer(c_int64_t), intent(out) :: slots, bytes
+    end function
+
+    integer(c_int) function yk_batch_add(b, params, n, zii, aii, x, v, dt, steps, id) &
+        bind(C, name='yk_batch_add')
+      import :: c_int, c_int32_t, c_double, c_ptr, yk_params_t
+      type(c_ptr), value :: b
+      type(yk_params_t), intent(in) :: params
+      integer(c_int32_t), value :: n, steps
+      real(c_double), intent(in) :: zii(*), aii(*), x(*), v(*)
+      real(c_double), value :: dt
+      integer(c_int32_t), intent(out) :: id
+    end function
+
+    integer(c_int) function yk_batch_remove(b, id) bind(C, name='yk_batch_remove')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: id
+    end function
+
+    integer(c_int) function yk_batch_set_tuning(b, nbuckets, n_upper, chunk_tiles, waves) &
+        bind(C, name='yk_batch_set_tuning')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: nbuckets
+      integer(c_int32_t), intent(in) :: n_upper(*), chunk_tiles(*), waves(*)
+    end function
+
+    integer(c_int) function yk_batch_set_resort(b, every) bind(C, name='yk_batch_set_resort')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: every
+    end function
+
+    integer(c_int) function yk_batch_run(b, nsteps, nind) bind(C, name='yk_batch_run')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: nsteps, nind
+    end function
+
+    integer(c_int) function yk_batch_status(b, id, steps_left, nrec) &
+        bind(C, name='yk_batch_status')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: id
+      integer(c_int32_t), intent(out) :: steps_left, nrec
+    end function
+
+    integer(c_int) function yk_batch_get(b, id, x, v, a, init, records, rec_cap, nrec) &
+        bind(C, name='yk_batch_get')
+      import :: c_int, c_int32_t, c_double, c_ptr, yk_meas_t
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: id, rec_cap
+      real(c_double), intent(inout) :: x(*), v(*), a(*)
+      type(yk_meas_t), intent(inout) :: init, records(*)
+      integer(c_int32_t), intent(out) :: nrec
+    end function
+
+    integer(c_int) function yk_batch_last_run(b, ms, steps, items) &
+        bind(C, name='yk_batch_last_run')
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: b
+      real(c_double), intent(out) :: ms
+      integer(c_int64_t), intent(out) :: steps, items
+    end function
+
     integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
       import :: c_size_t, c_ptr
       type(c_ptr), value :: s
@@ -163,10 +256,20 @@ contains
   ! yk_last_error() as a Fortran string.
   function yk_error_message() result(msg)
     character(len=:), allocatable :: msg
-    type(c_ptr) :: p
+    msg = c_string(yk_last_error_c())
+  end function yk_error_message
+
+  ! yk_batch_last_error() as a Fortran string.
+  function yk_batch_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    msg = c_string(yk_batch_last_error_c())
+  end function yk_batch_error_message
+
+  function c_string(p) result(msg)
+    type(c_ptr), intent(in) :: p
+    character(len=:), allocatable :: msg
     character(kind=c_char), pointer :: chars(:)
     integer :: k, m
-    p = yk_last_error_c()
     if (.not. c_associated(p)) then
       msg = ''
       return
@@ -177,6 +280,6 @@ contains
     do k = 1, m
       msg(k:k) = chars(k)
     end do
-  end function yk_error_message
+  end function c_string
 
 end module yk_bind
