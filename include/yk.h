@@ -69,7 +69,11 @@ enum yk_flags {
     /* Spreiter-Walter velocity Verlet for the uniform field bfield along z
      * (src/vel_verlet_3.f:20-100) instead of the predictor/corrector form;
      * excludes YK_FLAG_KDK. */
-    YK_FLAG_SW = 1u << 4
+    YK_FLAG_SW = 1u << 4,
+    /* Pair lists per 64-particle tile only, without the 16-particle cluster
+     * lists built from them (which systems up to 65,536 particles use by
+     * default); fixed at yk_create. */
+    YK_FLAG_TILE_LISTS = 1u << 5
 };
 
 enum yk_status {

@@ -53,8 +53,10 @@ def abs_pair_force_sum(p, x, Z, A):
     return out
 
 
-def gpu_accel(p, x, Z, A, do_meas=True, allpairs=False):
-    p = p.replace(allpairs=allpairs)
+def gpu_accel(p, x, Z, A, do_meas=True, path="clist"):
+    """path: 'clist' (cluster lists, the default for N <= 65,536), 'tilelist'
+    (64-particle tile lists only) or 'allpairs' (culled tile pairs)."""
+    p = p.replace(allpairs=path == "allpairs", tile_lists=path == "tilelist")
     with YukawaSystem(p, Z=Z, A=A) as s:
         s.put_state(x)
         m = s.accel(do_meas)
@@ -120,15 +122,15 @@ def test_golden_fp64_three_steps(name):
     lambda: W.nucleon(n=4000, seed=52),
     lambda: W.ion_mixture(n=1024, seed=53, kappa=2.0),
 ])
-@pytest.mark.parametrize("allpairs", [False, True], ids=["pairlist", "allpairs"])
-def test_fp64_against_oracle(make, allpairs):
+@pytest.mark.parametrize("path", ["clist", "tilelist", "allpairs"])
+def test_fp64_against_oracle(make, path):
     w = make()
     p = w.params
     for dm in (True, False):
         a_o, m_o = oracle.accel(p, w.x, w.Z, w.A, dm, ref_f1=0)
-        a_g, m_g = gpu_accel(p, w.x, w.Z, w.A, dm, allpairs)
+        a_g, m_g = gpu_accel(p, w.x, w.Z, w.A, dm, path)
         rel, cond = force_errors(p, a_g, a_o, w.x, w.Z, w.A)
-        print(f"{w.name} allpairs={allpairs} meas={dm}: max rel {rel:.2e} cond {cond:.2e}")
+        print(f"{w.name} {path} meas={dm}: max rel {rel:.2e} cond {cond:.2e}")
         assert cond <= 1e-12 and rel <= 1e-10
         if dm:
             check_meas(m_g, m_o)

@@ -131,7 +131,7 @@ __device__ __forceinline__ void interact(const ForceConsts &c, double zi, double
                                          Acc<MEAS> &acc)
 {
     double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
-    if constexpr (SELF_BY_R2) self = !(r2 > 0.0);
+    if constexpr (SELF_BY_R2) self = self || !(r2 > 0.0);
     if constexpr (KIND != NN) {
         // Guarded: a wave whose lanes are all outside the cutoff skips the
         // evaluation (measured faster than a branch-free select form, since
@@ -342,6 +342,10 @@ struct ListConsts {
     double thresh2;      // (skin/2)^2: rebuild when a displacement exceeds it
     int n, ntiles, cap;  // cap = entries per tile region (= n)
     int pad;
+    // FP32 copies for the cluster-list candidate test, widened so that the
+    // float test keeps every candidate and only calls an entry fast when the
+    // double test would: rl2f = rl^2 (1 + 1e-5); halfLf_m = halfL_m - 1e-5 L
+    float Lf[3], invLf[3], halfLf_m[3], rl2f, skinf;
 };
 
 constexpr unsigned SLOT_MASK = (1u << 26) - 1u;
@@ -565,6 +569,328 @@ static __global__ __launch_bounds__(BLOCK) void k_tile_boxes(const double4 *__re
 }
 
 // ---------------------------------------------------------------------------
+// Cluster lists.  A cluster is 16 consecutive sorted slots; its list holds
+// the source slots whose minimum-image distance to the cluster's AABB is
+// within the list radius rl = cutoff + skin, with the same entry encoding as
+// the tile lists (slot | per-axis image code).  A 16-particle cluster's AABB
+// is much smaller than a 64-particle tile's: about 30 % fewer candidate
+// pairs.  Fast entries (one image for every target of the cluster while no
+// particle has moved more than skin/2) fill the cluster's region from the
+// front, general ones from the back, both in slot order.
+constexpr int CL = 16;              // slots per cluster
+constexpr int CPW = TILE / CL;      // clusters per wave in the pair kernel
+constexpr int CLIST_MAX_N = 65536;  // above this the tile lists are used
+// A cluster's list is stored as NSEG segments: segment s holds the entries
+// whose source tile index is s mod NSEG (each built by its own workgroup, for
+// parallelism), each with its fast entries from the front and general ones
+// from the back of a region of segcap entries; the list is the segments in
+// order.
+constexpr int NSEG = 4;
+
+// One workgroup per tile, wave w builds the list of the tile's cluster w.
+// Source tiles are culled by the minimum-image gap between their AABB
+// (current positions, from k_drift / k_tile_boxes) and the tile's; the four
+// waves then load the surviving source tiles four at a time into LDS (double
+// buffered, the loads of the next two rounds in flight) and each tests its
+// own cluster against every staged slot, so one read of a source tile serves
+// the tile's four clusters.  Also records the positions the list was built
+// from (xref).
+static __global__ __launch_bounds__(BLOCK) void k_clist_build(
+    const double4 *__restrict__ pos, const TileBox *__restrict__ box, ListConsts lc,
+    int segcap, const unsigned long long *__restrict__ maxdisp, unsigned *__restrict__ clist,
+    int2 *__restrict__ ccounts, double4 *__restrict__ xref)
+{
+    if (*maxdisp <= as_bits(lc.thresh2)) return;   // lists still valid (uniform)
+    constexpr int W = BLOCK / TILE;
+    __shared__ double4 stage[2][W][TILE];
+    __shared__ int ctiles[CLIST_MAX_N / TILE / NSEG + 1];
+    const int t = blockIdx.x;
+    const int seg = blockIdx.y;   // source tiles st with st % NSEG == seg
+    const int lane = threadIdx.x & (TILE - 1), wv = threadIdx.x / TILE;
+    const int cl = t * CPW + wv;
+    const bool cvalid = cl * CL < lc.n;
+    // cluster AABB (lanes 0..15 hold its slots)
+    double c[3], h[3];
+    {
+        const int i = cl * CL + lane;
+        const bool v = lane < CL && i < lc.n;
+        const double4 p = v ? pos[i] : make_double4(0, 0, 0, 0);
+        if (v && seg == 0) xref[i] = p;
+        const double xi[3] = {p.x, p.y, p.z};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double lo = v ? xi[k] : __builtin_inf(), hi = v ? xi[k] : -__builtin_inf();
+            wave_minmax(lo, hi);
+            c[k] = 0.5 * (lo + hi);
+            h[k] = 0.5 * (hi - lo);
+        }
+    }
+    const bool live = cvalid && h[0] >= 0.0;   // the cluster holds a particle
+    // half extents in float, widened by 1e-6 (more than one rounding), so the
+    // float box contains the double one
+    float hf[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) hf[k] = live ? (float)h[k] * 1.000001f : 0.0f;
+    // candidate source tiles of the whole tile, in ascending order
+    int ncand = 0;
+    {
+        const double rl = __builtin_sqrt(lc.rl2);
+        double tc[3], th[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { tc[k] = box[t].c[k]; th[k] = box[t].h[k]; }
+        const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        for (int tb = 0; tb < lc.ntiles; tb += TILE * NSEG) {
+            const int st = tb + lane * NSEG + seg;
+            bool tcand = false;
+            if (st < lc.ntiles) {
+                double gap2 = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    double d = tc[k] - box[st].c[k];
+                    d = __builtin_fma(-lc.L[k], __builtin_rint(d * lc.invL[k]), d);
+                    double gk = __builtin_fabs(d) - th[k] - box[st].h[k];
+                    gk = gk > 0.0 ? gk : 0.0;
+                    gap2 = __builtin_fma(gk, gk, gap2);
+                }
+                tcand = gap2 <= rl * rl;
+            }
+            const unsigned long long tm = __ballot(tcand);
+            if (wv == 0 && tcand) ctiles[ncand + __popcll(tm & lt)] = st;
+            ncand += __popcll(tm);
+        }
+    }
+    __syncthreads();
+    unsigned *dst = clist + ((size_t)(cvalid ? cl : 0) * NSEG + seg) * segcap;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    auto load_round = [&](int r) -> double4 {
+        const int idx = r * W + wv;
+        const int j = (idx < ncand) ? ctiles[idx] * TILE + lane : lc.n;
+        return (j < lc.n) ? pos[j] : make_double4(0, 0, 0, 0);
+    };
+    const int nrounds = (ncand + W - 1) / W;
+    double4 pa = load_round(0), pb = load_round(1);
+    int nf = 0, ng = 0;
+    for (int r = 0; r < nrounds; ++r) {
+        stage[r & 1][wv][lane] = pa;
+        pa = pb;
+        pb = load_round(r + 2);
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const int idx = r * W + w;
+            const int j = (idx < ncand) ? ctiles[idx] * TILE + lane : lc.n;
+            bool cand = false, fast = true;
+            unsigned code = 0;
+            if (live && j < lc.n) {
+                // offset from the cluster centre formed in FP64, tested in
+                // FP32 with the widened bounds of ListConsts
+                const double4 pj = stage[r & 1][w][lane];
+                const double xj[3] = {pj.x, pj.y, pj.z};
+                float d2 = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float cd = (float)(c[k] - xj[k]);
+                    const float m = __builtin_rintf(cd * lc.invLf[k]);
+                    const float ev = __builtin_fabsf(__builtin_fmaf(-lc.Lf[k], m, cd));
+                    float gk = ev - hf[k];
+                    gk = gk > 0.0f ? gk : 0.0f;
+                    d2 = __builtin_fmaf(gk, gk, d2);
+                    // positions outside [0, L) can give |m| > 1: those take the
+                    // per-pair minimum image and keep a zero image code
+                    const bool m_ok = (m >= -1.0f) && (m <= 1.0f);
+                    fast = fast && m_ok && (ev + hf[k] + lc.skinf < lc.halfLf_m[k]);
+                    code |= (unsigned)(m_ok ? (int)m + 1 : 1) << (26 + 2 * k);
+                }
+                cand = d2 <= lc.rl2f;
+            }
+            const unsigned long long bf = __ballot(cand && fast);
+            const unsigned long long bg = __ballot(cand && !fast);
+            if (cand && fast) dst[nf + __popcll(bf & lt)] = (unsigned)j | code;
+            if (cand && !fast) dst[segcap - 1 - (ng + __popcll(bg & lt))] = (unsigned)j | code;
+            nf += __popcll(bf);
+            ng += __popcll(bg);
+        }
+    }
+    if (cvalid && lane == 0) ccounts[cl * NSEG + seg] = make_int2(nf, ng);
+}
+
+// Cluster order for the pair kernel: by list length, longest first (ties by
+// index), so the four clusters a wave takes have nearly equal lists.  One
+// workgroup of 1024 threads, bitonic sort of up to 4096 keys in LDS.
+constexpr int CORDER_THREADS = 1024;
+constexpr int CORDER_MAX = 4096;
+static __global__ __launch_bounds__(CORDER_THREADS) void k_corder(
+    const int2 *__restrict__ ccounts, int ncl, ListConsts lc,
+    const unsigned long long *__restrict__ maxdisp, int *__restrict__ order)
+{
+    if (*maxdisp <= as_bits(lc.thresh2)) return;
+    __shared__ unsigned key[CORDER_MAX];
+    int np = 1;
+    while (np < ncl) np <<= 1;
+    for (int k = threadIdx.x; k < np; k += CORDER_THREADS) {
+        unsigned kk = 0xffffffffu;
+        if (k < ncl) {
+            unsigned len = 0;
+            for (int sg = 0; sg < NSEG; ++sg) {
+                const int2 cc = ccounts[k * NSEG + sg];
+                len += (unsigned)(cc.x + cc.y);
+            }
+            kk = ((0x1ffffu - (len > 0x1ffffu ? 0x1ffffu : len)) << 12) | (unsigned)k;
+        }
+        key[k] = kk;
+    }
+    __syncthreads();
+    for (int size = 2; size <= np; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int k = threadIdx.x; k < np; k += CORDER_THREADS) {
+                const int j = k ^ stride;
+                if (j > k) {
+                    const bool up = (k & size) == 0;
+                    const unsigned a = key[k], b = key[j];
+                    if ((a > b) == up) { key[k] = b; key[j] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int k = threadIdx.x; k < ncl; k += CORDER_THREADS) order[k] = (int)(key[k] & 0xfffu);
+}
+
+// Pair sums over the cluster lists.  A wave takes four clusters (order[4s ..
+// 4s+3]), 16 lanes each, one target per lane, and item m of M: an equal share
+// of every cluster's list.  Each 16-lane group stages 16 of its own sources at
+// a time (already shifted to their image) in its LDS slice; the wave then
+// steps through the four slices together, lanes of shorter shares idle.
+template <int KIND, int PREC, bool MEAS>
+__global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict__ pos,
+                                                       const unsigned *__restrict__ clist,
+                                                       const int2 *__restrict__ ccounts,
+                                                       const int *__restrict__ order,
+                                                       ForceConsts c, int segcap, int ncl, int M,
+                                                       double *__restrict__ part,
+                                                       unsigned long long *__restrict__ maxdisp,
+                                                       unsigned long long thresh_bits,
+                                                       unsigned long long *__restrict__ swept)
+{
+    __shared__ double4 sj[BLOCK / TILE][TILE];
+    const int n = c.n;
+    // the build kernels (previous launches) have consumed the displacement flag
+    if (blockIdx.x == 0 && threadIdx.x == 0 && *maxdisp > thresh_bits) *maxdisp = 0ull;
+    const int lane = threadIdx.x & (TILE - 1);
+    const int wv = threadIdx.x / TILE;
+    const int g = lane / CL, gl = lane % CL;
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / TILE) + wv);
+    const int set = item / M, m = item % M;
+    const int nsets = (ncl + CPW - 1) / CPW;
+    if (set >= nsets) return;   // whole wave; this kernel has no block barrier
+    const int ci = set * CPW + g;
+    const int cl = (ci < ncl) ? order[ci] : -1;
+    const int i = cl * CL + gl;
+    const bool ivalid = cl >= 0 && i < n;
+    const double nanv = __builtin_nan("");
+    const double4 pi = ivalid ? pos[i] : make_double4(nanv, nanv, nanv, 0.0);
+    // this group's share of its cluster's list (NSEG segments in order)
+    int2 sc[NSEG];
+    int sbeg[NSEG + 1];
+    sbeg[0] = 0;
+#pragma unroll
+    for (int sg = 0; sg < NSEG; ++sg) {
+        sc[sg] = (cl >= 0) ? ccounts[cl * NSEG + sg] : make_int2(0, 0);
+        sbeg[sg + 1] = sbeg[sg] + sc[sg].x + sc[sg].y;
+    }
+    const int total = sbeg[NSEG];
+    const int per = (total + M - 1) / M;
+    const int lo = min(m * per, total), hi = min(lo + per, total);
+    const int share = hi - lo;
+    int iters = share;
+#pragma unroll
+    for (int off = CL; off < TILE; off <<= 1) {
+        const int o = __shfl_xor(iters, off, 64);
+        iters = o > iters ? o : iters;
+    }
+    iters = __builtin_amdgcn_readfirstlane(iters);
+    const unsigned *mine = clist + (size_t)(cl >= 0 ? cl : 0) * NSEG * segcap;
+    // entry v of the list; bit 31 of the result's companion flag: general
+    auto entry_at = [&](int v, bool &general) -> unsigned {
+        general = false;
+        if (v >= hi) return 0u;
+        int sg = 0;
+#pragma unroll
+        for (int q = 1; q < NSEG; ++q) sg += (v >= sbeg[q]) ? 1 : 0;
+        const int o = v - sbeg[sg];
+        const unsigned *r = mine + (size_t)sg * segcap;
+        if (o < sc[sg].x) return r[o];
+        general = true;
+        return r[segcap - 1 - (o - sc[sg].x)];
+    };
+    Acc<MEAS> acc;
+    double4 *sw = sj[wv];
+    // software pipeline over batches of 16 entries per group
+    bool g_cur, g_nxt;
+    unsigned e_cur = entry_at(lo + gl, g_cur);
+    double4 pj_cur = pos[e_cur & SLOT_MASK];
+    unsigned e_nxt = entry_at(lo + CL + gl, g_nxt);
+    for (int b0 = 0; b0 < iters; b0 += CL) {
+        const int v = lo + b0 + gl;
+        bool genl = false;
+        if (v < hi) {
+            double4 pj = pj_cur;
+            if (!g_cur) {   // apply the entry's periodic image
+                pj.x += c.L[0] * (double)((int)((e_cur >> 26) & 3u) - 1);
+                pj.y += c.L[1] * (double)((int)((e_cur >> 28) & 3u) - 1);
+                pj.z += c.L[2] * (double)((int)((e_cur >> 30) & 3u) - 1);
+            } else {
+                genl = true;
+            }
+            pj.w = stage_w<KIND, PREC>(pj.w);
+            sw[lane] = pj;
+        }
+        // a batch holding a general entry takes the per-pair minimum image for
+        // all of its pairs (a no-op for the already shifted ones)
+        const bool gen = __ballot(genl) != 0ull;
+        __builtin_amdgcn_wave_barrier();
+        e_cur = e_nxt;
+        g_cur = g_nxt;
+        pj_cur = pos[e_cur & SLOT_MASK];
+        e_nxt = entry_at(lo + b0 + 2 * CL + gl, g_nxt);
+        const int kb = min(CL, iters - b0);
+        const int mycount = share - b0;   // entries of this group in the batch (may be <= 0)
+        const double4 *gs = sw + g * CL;
+        if (!gen) {
+            double4 pn = gs[0];
+#pragma unroll 4
+            for (int k = 0; k < kb; ++k) {
+                const double4 pj = pn;
+                pn = gs[min(k + 1, CL - 1)];
+                interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w, pi.x - pj.x, pi.y - pj.y,
+                                                 pi.z - pj.z, k >= mycount, acc);
+            }
+        } else {
+#pragma unroll 2
+            for (int k = 0; k < kb; ++k) {
+                const double4 pj = gs[k];
+                interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w,
+                                                 min_image(pi.x - pj.x, c.L[0], c.invL[0]),
+                                                 min_image(pi.y - pj.y, c.L[1], c.invL[1]),
+                                                 min_image(pi.z - pj.z, c.L[2], c.invL[2]),
+                                                 k >= mycount, acc);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0 && iters > 0) atomicAdd(swept, (unsigned long long)iters * TILE);
+    if (cl < 0) return;
+    constexpr int NC = MEAS ? 11 : 3;
+    double *o = part + ((size_t)cl * M + m) * NC * CL + gl;
+    o[0] = acc.fx; o[CL] = acc.fy; o[2 * CL] = acc.fz;
+    if constexpr (MEAS) {
+        o[3 * CL] = acc.e; o[4 * CL] = acc.w;
+        o[5 * CL] = acc.pxx; o[6 * CL] = acc.pxy; o[7 * CL] = acc.pxz;
+        o[8 * CL] = acc.pyy; o[9 * CL] = acc.pyz; o[10 * CL] = acc.pzz;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Morton keys of the current slots (cell grid 2^g per axis) and the gather
 // that applies a sort permutation to every per-particle array.
 __device__ __forceinline__ unsigned spread3(unsigned v)
@@ -645,7 +971,8 @@ __device__ __forceinline__ void block_reduce_store(double (&q)[NQ], double *__re
 // MODE 0: accel only.  MODE 1: vel_verlet_2 corrector (src/vel_verlet_2.f:61-79).
 // MODE 2: kick-drift-kick closing half kick (src/vel_verlet_1.f:52-56).
 // LAYOUT 0: partials part[c][NC][n] from k_pairs (nparts chunks);
-// LAYOUT 1: part[t][m][NC][TILE] from k_pairs_list (nparts items per tile).
+// LAYOUT 1: part[t][m][NC][TILE] from k_pairs_list (nparts items per tile);
+// LAYOUT 2: part[cl][m][NC][CL] from k_pairs_clist (nparts items per cluster).
 template <int KIND, bool MEAS, int MODE, int LAYOUT>
 __global__ __launch_bounds__(BLOCK) void k_finalize(
     const double *__restrict__ part, int nparts, const double4 *__restrict__ pos,
@@ -670,12 +997,19 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(
 #pragma unroll
                 for (int k = 0; k < NC; ++k) f[k] += p[k * nn];
             }
-        } else {
+        } else if constexpr (LAYOUT == 1) {
             const int t = i / TILE, lane = i % TILE;
             for (int m = 0; m < nparts; ++m) {
                 const double *p = part + ((size_t)t * nparts + m) * NC * TILE + lane;
 #pragma unroll
                 for (int k = 0; k < NC; ++k) f[k] += p[k * TILE];
+            }
+        } else {
+            const int cl = i / CL, gl = i % CL;
+            for (int m = 0; m < nparts; ++m) {
+                const double *p = part + ((size_t)cl * nparts + m) * NC * CL + gl;
+#pragma unroll
+                for (int k = 0; k < NC; ++k) f[k] += p[k * CL];
             }
         }
         double zi = pos[i].w;

@@ -67,6 +67,13 @@ struct yk_system {
     double skin = 0.0;
     unsigned *list = nullptr;                // [ntiles][n]
     int2 *counts = nullptr;                  // [ntiles] (fast, general)
+    bool use_clist = false;                  // 16-slot cluster lists on top of the tile lists
+    int ncl = 0;                             // clusters
+    unsigned *clist = nullptr;               // [ncl][n]
+    int2 *ccounts = nullptr;                 // [ncl] (fast, general)
+    int *corder = nullptr;                   // [ncl] clusters by list length
+    int segcap = 0;                          // entries per cluster-list segment
+    bool corder_stale = true;                // slots moved: order to be recomputed
     double4 *xref = nullptr;                 // positions at the last build
     unsigned long long *maxdisp = nullptr;   // max displacement^2 since build (bits)
     unsigned long long *swept = nullptr;     // directed pairs evaluated (counter)
@@ -230,6 +237,13 @@ ListConsts list_consts(const yk_system *s)
     l.n = s->n;
     l.ntiles = s->ntiles;
     l.cap = s->n;
+    for (int k = 0; k < 3; ++k) {
+        l.Lf[k] = (float)l.L[k];
+        l.invLf[k] = (float)l.invL[k];
+        l.halfLf_m[k] = (float)(l.halfL_m[k] - 1e-5 * p.xl[k]);
+    }
+    l.rl2f = (float)(l.rl2 * (1.0 + 1e-5));
+    l.skinf = (float)s->skin;
     return l;
 }
 
@@ -246,6 +260,7 @@ int invalidate_list(yk_system *s)
     if (!s->use_list) return YK_OK;
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s->stream, s->maxdisp,
                        bits_of(__builtin_inf()));
+    s->corder_stale = true;
     YK_HIP(hipGetLastError());
     return YK_OK;
 }
@@ -369,6 +384,21 @@ int read_swept(yk_system *s, unsigned long long *out)
 template <int KIND, int PREC>
 void launch_pairs_k(yk_system *s, const ForceConsts &c, bool meas)
 {
+    if (s->use_clist) {
+        const ListConsts lc = list_consts(s);
+        const unsigned long long tb = bits_of(lc.thresh2);
+        const int items = (s->ncl + CPW - 1) / CPW * s->list_M;
+        dim3 grid((items + BLOCK / TILE - 1) / (BLOCK / TILE));
+        if (meas)
+            hipLaunchKernelGGL((k_pairs_clist<KIND, PREC, true>), grid, dim3(BLOCK), 0,
+                               s->stream, s->pos, s->clist, s->ccounts, s->corder, c,
+                               s->segcap, s->ncl, s->list_M, s->part, s->maxdisp, tb, s->swept);
+        else
+            hipLaunchKernelGGL((k_pairs_clist<KIND, PREC, false>), grid, dim3(BLOCK), 0,
+                               s->stream, s->pos, s->clist, s->ccounts, s->corder, c,
+                               s->segcap, s->ncl, s->list_M, s->part, s->maxdisp, tb, s->swept);
+        return;
+    }
     if (s->use_list) {
         const ListConsts lc = list_consts(s);
         const unsigned long long tb = bits_of(lc.thresh2);
@@ -399,8 +429,22 @@ int launch_list_build(yk_system *s)
 {
     if (!s->use_list) return YK_OK;
     const ListConsts lc = list_consts(s);
-    hipLaunchKernelGGL(k_list_build, dim3(s->ntiles), dim3(BLOCK), 0, s->stream, s->pos,
-                       s->box, lc, s->maxdisp, s->list, s->counts, s->xref);
+    if (s->use_clist) {
+        hipLaunchKernelGGL(k_clist_build, dim3(s->ntiles, NSEG), dim3(BLOCK), 0, s->stream,
+                           s->pos, s->box, lc, s->segcap, s->maxdisp, s->clist, s->ccounts,
+                           s->xref);
+        // the cluster order is refreshed with the build that follows a re-sort
+        // (or a new state); between re-sorts clusters keep their particles and
+        // their list lengths change slowly, so the order stays balanced
+        if (s->corder_stale) {
+            hipLaunchKernelGGL(k_corder, dim3(1), dim3(CORDER_THREADS), 0, s->stream,
+                               s->ccounts, s->ncl, lc, s->maxdisp, s->corder);
+            s->corder_stale = false;
+        }
+    } else {
+        hipLaunchKernelGGL(k_list_build, dim3(s->ntiles), dim3(BLOCK), 0, s->stream, s->pos,
+                           s->box, lc, s->maxdisp, s->list, s->counts, s->xref);
+    }
     YK_HIP(hipGetLastError());
     return YK_OK;
 }
@@ -438,7 +482,11 @@ int launch_pairs(yk_system *s, bool meas)
 template <int KIND, bool MEAS, int MODE>
 void launch_fin_k(yk_system *s, const StepConsts &sc)
 {
-    if (s->use_list)
+    if (s->use_clist)
+        hipLaunchKernelGGL((k_finalize<KIND, MEAS, MODE, 2>), dim3(s->nblocks), dim3(BLOCK), 0,
+                           s->stream, s->part, s->list_M, s->pos, s->den, s->aii, s->orig,
+                           s->acc, s->vel, s->vold, sc, s->blockpart);
+    else if (s->use_list)
         hipLaunchKernelGGL((k_finalize<KIND, MEAS, MODE, 1>), dim3(s->nblocks), dim3(BLOCK), 0,
                            s->stream, s->part, s->list_M, s->pos, s->den, s->aii, s->orig,
                            s->acc, s->vel, s->vold, sc, s->blockpart);
@@ -577,6 +625,7 @@ void release(yk_system *s)
                       (void *)s->blockpart, (void *)s->d_meas, (void *)s->keys,
                       (void *)s->keys2, (void *)s->vals, (void *)s->vals2, s->sort_tmp,
                       (void *)s->list, (void *)s->counts, (void *)s->xref,
+                      (void *)s->clist, (void *)s->ccounts, (void *)s->corder,
                       (void *)s->maxdisp, (void *)s->swept})
         if (ptr) (void)hipFree(ptr);
     if (s->h_stage) (void)hipHostFree(s->h_stage);
@@ -662,9 +711,19 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
     if (s->use_list) {
         s->part_doubles = std::max(s->part_doubles,
                                    (size_t)s->ntiles * s->list_M * 11 * TILE);
-        YK_HIPB(hipMalloc(&s->list, sizeof(unsigned) * (size_t)s->ntiles * nn));
-        YK_HIPB(hipMalloc(&s->counts, sizeof(int2) * (size_t)s->ntiles));
         YK_HIPB(hipMalloc(&s->xref, sizeof(double4) * nn));
+        s->use_clist = !(params->flags & YK_FLAG_TILE_LISTS) && n <= CLIST_MAX_N;
+        if (!s->use_clist) {
+            YK_HIPB(hipMalloc(&s->list, sizeof(unsigned) * (size_t)s->ntiles * nn));
+            YK_HIPB(hipMalloc(&s->counts, sizeof(int2) * (size_t)s->ntiles));
+        } else {
+            s->ncl = (n + CL - 1) / CL;
+            s->segcap = (s->ntiles + NSEG - 1) / NSEG * TILE;   // slots of a segment's tiles
+            YK_HIPB(hipMalloc(&s->clist,
+                              sizeof(unsigned) * (size_t)s->ncl * NSEG * s->segcap));
+            YK_HIPB(hipMalloc(&s->ccounts, sizeof(int2) * (size_t)s->ncl * NSEG));
+            YK_HIPB(hipMalloc(&s->corder, sizeof(int) * (size_t)s->ncl));
+        }
     }
     YK_HIPB(hipMalloc(&s->maxdisp, sizeof(unsigned long long)));
     YK_HIPB(hipMalloc(&s->swept, sizeof(unsigned long long)));
@@ -704,8 +763,9 @@ int yk_set_params(yk_system *s, const yk_params *params)
         return fail(YK_ERR_UNSUPPORTED, "pure-ion ai/xmass are fixed at yk_create");
     if (params->xmass != s->p.xmass)
         return fail(YK_ERR_UNSUPPORTED, "xmass is fixed at yk_create");
-    if ((params->flags & YK_FLAG_ALLPAIRS) != (s->p.flags & YK_FLAG_ALLPAIRS))
-        return fail(YK_ERR_UNSUPPORTED, "YK_FLAG_ALLPAIRS is fixed at yk_create");
+    if ((params->flags & YK_FLAG_ALLPAIRS) != (s->p.flags & YK_FLAG_ALLPAIRS) ||
+        (params->flags & YK_FLAG_TILE_LISTS) != (s->p.flags & YK_FLAG_TILE_LISTS))
+        return fail(YK_ERR_UNSUPPORTED, "YK_FLAG_ALLPAIRS / YK_FLAG_TILE_LISTS are fixed at yk_create");
     if (list_cutoff(*params) + 0.0 != list_cutoff(s->p) && s->use_list)
         s->skin = 0.05 * list_cutoff(*params);
     s->p = *params;

@@ -23,7 +23,7 @@ module yk_bind
   integer(c_int32_t), parameter, public :: YK_FP64 = 0, YK_MIXED = 1
   integer(c_int32_t), parameter, public :: YK_FLAG_REF_F1 = 1, YK_FLAG_KDK = 2, &
                                            YK_FLAG_TIMING = 4, YK_FLAG_ALLPAIRS = 8, &
-                                           YK_FLAG_SW = 16
+                                           YK_FLAG_SW = 16, YK_FLAG_TILE_LISTS = 32
   integer(c_int), parameter, public :: YK_OK = 0
 
   type, bind(C), public :: yk_params_t

@@ -28,6 +28,7 @@ FLAG_KDK = 1 << 1
 FLAG_TIMING = 1 << 2
 FLAG_ALLPAIRS = 1 << 3
 FLAG_SW = 1 << 4
+FLAG_TILE_LISTS = 1 << 5
 
 
 @dataclass
@@ -60,6 +61,7 @@ class SimParams:
     timing: bool = False
     resort_every: int = 0               # device re-sort cadence (0 = library default)
     allpairs: bool = False              # AABB-culled tile pairs instead of pair lists
+    tile_lists: bool = False            # 64-particle tile lists only (no cluster lists)
 
     def __post_init__(self):
         if self.sim_type not in SIM_TYPES:
@@ -92,6 +94,8 @@ class SimParams:
             f |= FLAG_TIMING
         if self.allpairs:
             f |= FLAG_ALLPAIRS
+        if self.tile_lists:
+            f |= FLAG_TILE_LISTS
         return f
 
     def replace(self, **kw) -> "SimParams":
