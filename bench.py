@@ -46,12 +46,12 @@ PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector (packed)
 FLOPS_PER_PAIR_FP64 = 90     # directed pair, force-only step, special functions costed
 FLOPS_PER_PAIR_MEAS = 109    # measurement step
 FM_C_SECONDS = 3.33564e-24
-# HBM bytes per launch of the FP64 force-only pair-list kernel at N = 27,648,
+# HBM bytes per launch of the FP64 force-only cluster-list kernel at N = 27,648,
 # from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scale factors from a plain
 # 256 MiB copy); counters cannot be read inside a timed run, so the figure is
 # the profiled one, with its commands, in the file named here.
-TRAFFIC_BYTES_PER_LAUNCH = 7.772e7
-TRAFFIC_SOURCE = "profiles/hbm_traffic_k_pairs_list_ion_mixture_27648.md"
+TRAFFIC_BYTES_PER_LAUNCH = 1.308e8
+TRAFFIC_SOURCE = "profiles/hbm_traffic_k_pairs_clist_ion_mixture_27648.md"
 
 
 def parse():
@@ -257,7 +257,7 @@ def main():
     line["roofline"] = {
         "bound": "valu",
         "peak_kind": "FP64 vector",
-        "kernel": "k_pairs_list<MIX,FP64>",
+        "kernel": "k_pairs_clist<MIX,FP64>",
         "achieved": f64["force_kernel_tflops"],
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
