@@ -231,12 +231,17 @@ int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *
                  int32_t steps, int32_t *id);
 int yk_batch_remove(yk_batch *b, int32_t id);
 
-/* Tuning per N bucket (first k with n <= n_upper[k]): source tiles per work
- * item, and waves per workgroup (1, 2, 4 or 8).  The persistent launch takes
- * the waves of the bucket that holds the largest share of the pair work
- * (sum of n^2) among the active replicas. */
+/* Tuning per N bucket (first k with n <= n_upper[k]): list_items > 0 gives
+ * the bucket's replicas (up to 65,536 particles) 16-slot cluster pair lists
+ * with that many work items per 64-slot tile; list_items = 0 gives them tile
+ * items of chunk_tiles source tiles each.  waves_per_block (1, 2, 4 or 8):
+ * the persistent launch takes the waves of the bucket that holds the largest
+ * share of the pair work (sum of n^2) among the active replicas.  A replica
+ * keeps the list / tile choice it was admitted with.  Without a table:
+ * lists with 16 items per tile up to 65,536 particles. */
 int yk_batch_set_tuning(yk_batch *b, int32_t nbuckets, const int32_t *n_upper,
-                        const int32_t *chunk_tiles, const int32_t *waves_per_block);
+                        const int32_t *chunk_tiles, const int32_t *waves_per_block,
+                        const int32_t *list_items);
 
 /* Re-sort a replica's slots into Morton order every `every` of its own steps
  * (0: never; default 10), keeping tiles compact as particles move.  The

@@ -18,6 +18,13 @@ from yukawamd_amd.system import YukawaSystem
 
 pytestmark = pytest.mark.gpu
 
+# the two pair paths of the engine: cluster pair lists, and tile items with
+# AABB culling
+MODES = {
+    "lists": [{"n_upper": 1 << 20, "chunk_tiles": 8, "waves_per_block": 4, "list_items": 16}],
+    "tiles": [{"n_upper": 1 << 20, "chunk_tiles": 4, "waves_per_block": 4, "list_items": 0}],
+}
+
 
 def small_set():
     return [W.replica(11, 300, False, 175.0, 1.0, "fp64"),
@@ -25,9 +32,10 @@ def small_set():
             W.replica(13, 777, True, 210.0, 1.8, "fp64")]
 
 
-def test_batch_fp64_accel_matches_oracle():
+@pytest.mark.parametrize("mode", list(MODES))
+def test_batch_fp64_accel_matches_oracle(mode):
     reps = small_set()
-    with Ensemble(capacity=4096, max_replicas=8) as e:
+    with Ensemble(capacity=4096, max_replicas=8, tuning=MODES[mode]) as e:
         ids = [e.add_workload(w, steps=0) for w in reps]
         e.run(0)
         for rid, w in zip(ids, reps):
@@ -41,13 +49,14 @@ def test_batch_fp64_accel_matches_oracle():
             assert g["init"]["px"] == pytest.approx(m_o["px"], rel=1e-13)
 
 
-def test_replica_alone_equals_in_mix():
+@pytest.mark.parametrize("mode", list(MODES))
+def test_replica_alone_equals_in_mix(mode):
     target = W.replica(21, 1500, True, 180.0, 0.9, "mixed")
     others = [W.replica(22 + k, n, k % 2 == 0, 170.0, 1.3, "mixed")
               for k, n in enumerate([3000, 700, 2100, 1100])]
     results = []
     for company in ([], others):
-        with Ensemble(capacity=16384, max_replicas=8, max_records=8) as e:
+        with Ensemble(capacity=16384, max_replicas=8, max_records=8, tuning=MODES[mode]) as e:
             ids = [e.add_workload(w, steps=20) for w in company[:2]]
             tid = e.add_workload(target, steps=20)
             ids += [e.add_workload(w, steps=20) for w in company[2:]]
@@ -58,9 +67,10 @@ def test_replica_alone_equals_in_mix():
     assert a["records"] == b["records"] and len(a["records"]) == 2
 
 
-def test_batch_matches_single_system():
+@pytest.mark.parametrize("mode", list(MODES))
+def test_batch_matches_single_system(mode):
     w = W.replica(31, 2000, True, 190.0, 1.2, "fp64")
-    with Ensemble(capacity=4096, max_replicas=2, max_records=4) as e:
+    with Ensemble(capacity=4096, max_replicas=2, max_records=4, tuning=MODES[mode]) as e:
         rid = e.add_workload(w, steps=10)
         e.run(10, nind=5)
         g = e.get(rid)
@@ -132,10 +142,11 @@ def test_resort_changes_only_rounding():
         assert r1["ek"] == pytest.approx(r2["ek"], rel=1e-10)
 
 
-def test_staggered_admission_with_resort_is_bitwise():
+@pytest.mark.parametrize("mode", list(MODES))
+def test_staggered_admission_with_resort_is_bitwise(mode):
     target = W.replica(71, 1800, True, 170.0, 0.9, "mixed")
     other = W.replica(72, 2500, False, 200.0, 1.5, "mixed")
-    with Ensemble(capacity=8192, max_replicas=4, max_records=8) as e:
+    with Ensemble(capacity=8192, max_replicas=4, max_records=8, tuning=MODES[mode]) as e:
         e.set_resort(10)
         e.add_workload(other, steps=100)
         e.run(13, nind=0)                 # the other replica is 13 steps ahead
@@ -144,7 +155,7 @@ def test_staggered_admission_with_resort_is_bitwise():
         e.run(40, nind=0)
         assert e.status(tid)[0] == 0
         mixed_in = e.get(tid)
-    with Ensemble(capacity=4096, max_replicas=1, max_records=8) as e:
+    with Ensemble(capacity=4096, max_replicas=1, max_records=8, tuning=MODES[mode]) as e:
         e.set_resort(10)
         rid = e.add_workload(target, steps=45)
         e.run(45, nind=0)

@@ -85,7 +85,8 @@ SIGNATURES = {
                                C.c_int32, C.POINTER(C.c_int32)]),
     "yk_batch_remove": (C.c_int, [P, C.c_int32]),
     "yk_batch_set_tuning": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int32),
-                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_int32)]),
     "yk_batch_set_resort": (C.c_int, [P, C.c_int32]),
     "yk_batch_run": (C.c_int, [P, C.c_int32, C.c_int32]),
     "yk_batch_status": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),

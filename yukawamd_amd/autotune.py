@@ -5,23 +5,25 @@
 For every N bucket an ensemble of replicas drawn from that bucket alone
 (N log-uniform inside the bucket, pure Z=8 or the three-species mixture with
 probability 1/2, Gamma U[150, 220], kappa U[0.5, 2], mixed precision: the
-batched-ensemble parameter ranges) is stepped under every candidate shape,
-and the shortest device time per step wins.  The two knobs are
+batched-ensemble parameter ranges) is admitted afresh under every candidate
+shape and stepped, and the shortest device time per step wins.  The knobs:
 
-  chunk_tiles      source tiles per work item: one wave sweeps one 64-particle
-                   target tile against chunk_tiles source tiles, so this sets
-                   the item grain (load balance against per-item overhead:
-                   partial-sum traffic and finalize work grow with the number
-                   of items per tile);
+  list_items       > 0: the replicas get 16-slot cluster pair lists and this
+                   many work items per 64-slot target tile (item = an equal
+                   share of each of the tile's four cluster lists); 0: tile
+                   items (replicas above 65,536 particles always use them);
+  chunk_tiles      tile items: source tiles per item, the item grain (load
+                   balance against per-item overhead: partial-sum traffic and
+                   finalize work grow with the items per tile);
   waves_per_block  waves per workgroup of the persistent launch (LDS per
                    workgroup = waves x 64 x 32 B; occupancy and queue
                    contention).
 
-Candidates keep at most 64 items per target tile.  The table is written as
-JSON ({"device": ..., "buckets": [{"n_upper", "chunk_tiles",
-"waves_per_block", "ms_per_step", "pairs_per_s"}, ...], "candidates": ...});
-yukawamd_amd/tuning/batch_gfx950.json is the committed result for MI355X,
-which ``Ensemble`` loads by default.
+Tile-item candidates keep at most 64 items per target tile.  The table is
+written as JSON ({"device": ..., "buckets": [{"n_upper", "list_items",
+"chunk_tiles", "waves_per_block", "ms_per_step", "pairs_per_s"}, ...],
+"candidates": ...}); yukawamd_amd/tuning/batch_gfx950.json is the committed
+result for MI355X, which ``Ensemble`` loads by default.
 """
 from __future__ import annotations
 
@@ -34,8 +36,10 @@ import time
 import numpy as np
 
 DEFAULT_BUCKETS = [2048, 4096, 8192, 16384, 65536, 262144]
-WAVES = [1, 2, 4, 8]
-CHUNKS = [1, 2, 4, 8, 16, 32, 64, 128, 256]
+WAVES = [1, 2, 4]
+CHUNKS = [2, 4, 8, 16, 32, 64, 128]
+LIST_ITEMS = [2, 4, 8, 16]
+CLIST_MAX_N = 65536
 MAX_ITEMS_PER_TILE = 64
 BUCKET_SLOTS = 1 << 20          # particles per tuning ensemble (at least one replica)
 
@@ -65,8 +69,12 @@ def candidates(n_max: int):
         if (nt + c - 1) // c > MAX_ITEMS_PER_TILE:
             continue
         for w in WAVES:
-            out.append((min(c, nt), w))
-    return sorted(set(out))
+            out.append({"list_items": 0, "chunk_tiles": min(c, nt), "waves_per_block": w})
+    if n_max <= CLIST_MAX_N:
+        for mm in LIST_ITEMS:
+            for w in WAVES:
+                out.append({"list_items": mm, "chunk_tiles": 8, "waves_per_block": w})
+    return out
 
 
 def tune_bucket(lo, hi, steps, warmup, seed, log):
@@ -75,19 +83,18 @@ def tune_bucket(lo, hi, steps, warmup, seed, log):
     cap = sum(((w.n + 255) // 256) * 256 for w in reps)
     pairs = sum(w.n * (w.n - 1) // 2 for w in reps)
     results = []
-    with Ensemble(capacity=cap, max_replicas=len(reps), max_records=1, tuning=[]) as e:
-        big = steps * 1000 + warmup * 1000
-        for w in reps:
-            e.add_workload(w, steps=big)
-        for c, wv in candidates(hi):
-            e.set_tuning([{"n_upper": hi, "chunk_tiles": c, "waves_per_block": wv}])
+    for cand in candidates(hi):
+        tuning = [dict(cand, n_upper=hi)]
+        # the list / tile choice is made at admission: a fresh ensemble each
+        with Ensemble(capacity=cap, max_replicas=len(reps), max_records=1, tuning=tuning) as e:
+            for w in reps:
+                e.add_workload(w, steps=steps + warmup)
             e.run(warmup)
             e.run(steps)
             ms = e.last_run()["ms"] / steps
-            results.append({"chunk_tiles": c, "waves_per_block": wv, "ms_per_step": ms,
-                            "pairs_per_s": pairs / (ms * 1e-3)})
-            log(f"  n<={hi} chunk={c} waves={wv}: {ms:.4f} ms/step "
-                f"{pairs / (ms * 1e-3):.3e} pairs/s")
+        r = dict(cand, ms_per_step=ms, pairs_per_s=pairs / (ms * 1e-3))
+        results.append(r)
+        log(f"  n<={hi} {cand}: {ms:.4f} ms/step {pairs / (ms * 1e-3):.3e} pairs/s")
     best = min(results, key=lambda r: r["ms_per_step"])
     return best, results, len(reps), pairs
 
@@ -95,7 +102,7 @@ def tune_bucket(lo, hi, steps, warmup, seed, log):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=None)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--buckets", default=",".join(map(str, DEFAULT_BUCKETS)))
     ap.add_argument("--seed", type=int, default=1005)
@@ -114,7 +121,8 @@ def main(argv=None):
         log(f"bucket ({lo}, {hi}]")
         best, results, nrep, pairs = tune_bucket(lo, hi, args.steps, args.warmup,
                                                   args.seed + k, log)
-        table.append({"n_upper": hi, "chunk_tiles": best["chunk_tiles"],
+        table.append({"n_upper": hi, "list_items": best["list_items"],
+                      "chunk_tiles": best["chunk_tiles"],
                       "waves_per_block": best["waves_per_block"],
                       "ms_per_step": best["ms_per_step"], "pairs_per_s": best["pairs_per_s"]})
         detail[str(hi)] = {"replicas": nrep, "pairs_per_step": pairs, "results": results}

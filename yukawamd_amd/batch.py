@@ -62,13 +62,16 @@ class Ensemble:
         return slots.value, nbytes.value
 
     def set_tuning(self, buckets):
-        """buckets: [{"n_upper": N, "chunk_tiles": C, "waves_per_block": W}, ...]
-        in increasing n_upper."""
+        """buckets: [{"n_upper": N, "chunk_tiles": C, "waves_per_block": W,
+        "list_items": M}, ...] in increasing n_upper; list_items > 0 selects
+        cluster pair lists with M work items per tile, 0 tile items of C source
+        tiles."""
         nb = len(buckets)
         a = (C.c_int32 * nb)(*[int(b["n_upper"]) for b in buckets])
-        c = (C.c_int32 * nb)(*[int(b["chunk_tiles"]) for b in buckets])
+        c = (C.c_int32 * nb)(*[int(b.get("chunk_tiles", 8)) for b in buckets])
         w = (C.c_int32 * nb)(*[int(b["waves_per_block"]) for b in buckets])
-        _lib.check_batch(self.lib.yk_batch_set_tuning(self._b, nb, a, c, w))
+        m = (C.c_int32 * nb)(*[int(b.get("list_items", 0)) for b in buckets])
+        _lib.check_batch(self.lib.yk_batch_set_tuning(self._b, nb, a, c, w, m))
         self.tuning = list(buckets)
 
     def set_resort(self, every: int):

@@ -61,7 +61,13 @@ struct HostReplica {
     // host mirror of the device step countdown (the device skips a replica
     // whose budget is used up; the host knows when, without reading back)
     int steps_left = 0, steps_done = 0, sorted_at = 0;
+    // cluster lists (list_items > 0 work items per tile): region of the list
+    // arena, fixed for the replica's lifetime
+    int list_items = 0, segcap = 0;
+    int64_t lists_off = 0, lists_len = 0;
 };
+
+constexpr int64_t LIST_UNIT = 1 << 20;   // list-arena allocation unit (entries)
 
 }  // namespace
 
@@ -101,7 +107,20 @@ struct yk_batch {
     bool dirty = true;       // items / constants must be rebuilt
     bool need_accel = true;  // accelerations of new replicas are missing
     int nitems = 0;
-    std::vector<int> tune_nupper, tune_chunk, tune_waves;
+    std::vector<int> tune_nupper, tune_chunk, tune_waves, tune_list;
+    // cluster-list arena (entries) with first-fit unit bookkeeping
+    unsigned *clist = nullptr;
+    int64_t clist_cap = 0;
+    std::vector<char> list_used;   // per LIST_UNIT
+    int2 *ccounts = nullptr;       // [cap/16][NSEG]
+    double4 *xref = nullptr;       // positions at each replica's last list build
+    unsigned long long *rep_maxdisp = nullptr;   // [max_rep] largest displacement^2 (bits)
+    int nlist = 0;                 // active list replicas
+    int span = 0;                  // used extent of the arena (slots)
+    int *list_tiles = nullptr;     // tiles of list replicas (build launch)
+    int n_list_tiles = 0;
+    size_t list_tiles_cap = 0;
+    unsigned long long md_stage = 0;   // host source of an async maxdisp upload
     int waves = 4;           // waves per workgroup of the persistent launch
     // captured step graph (valid while !dirty)
     hipGraphExec_t gexec = nullptr;
@@ -141,17 +160,37 @@ int chunk_tiles_for(const yk_batch *b, int n)
     return std::max(1, (nt + 3) / 4);   // default: four source ranges per target tile
 }
 
+// Work items per tile of a list replica (0: tile items).
+int list_items_for(const yk_batch *b, int n)
+{
+    if (n > CLIST_MAX_N) return 0;
+    const int k = bucket_of(b, n);
+    if (k >= 0) return b->tune_list[k];
+    return 16;
+}
+
+int segcap_of(int n) { return ((n + TILE - 1) / TILE + NSEG - 1) / NSEG * TILE; }
+
+int64_t list_entries_of(int n)
+{
+    return (int64_t)((n + CL - 1) / CL) * NSEG * segcap_of(n);
+}
+
 // Work items of a replica of n particles and their device bytes.
 int64_t items_of(const yk_batch *b, int n)
 {
-    const int64_t nt = (n + TILE - 1) / TILE, C = chunk_tiles_for(b, n);
+    const int64_t nt = (n + TILE - 1) / TILE;
+    const int M = list_items_for(b, n);
+    if (M > 0) return nt * M;
+    const int64_t C = chunk_tiles_for(b, n);
     return nt * ((nt + C - 1) / C);
 }
 
 // device bytes per arena slot (pos, vel/vold/acc, aii, den, orig, rep), per
 // 64-slot tile (box, tile_rep, tile_items), per 256-slot block (measurement
 // partials) and per work item (partial sums, item record)
-constexpr int64_t SLOT_BYTES = sizeof(double4) + 11 * sizeof(double) + 2 * sizeof(int);
+constexpr int64_t SLOT_BYTES = 2 * sizeof(double4) + 11 * sizeof(double) + 2 * sizeof(int) +
+                               NSEG * sizeof(int2) / CL;
 constexpr int64_t TILE_BYTES = sizeof(TileBox) + sizeof(int) + sizeof(int2);
 constexpr int64_t BLOCK_BYTES = NQ * sizeof(double);
 constexpr int64_t ITEM_BYTES = 11 * TILE * sizeof(double) + sizeof(BatchItem);
@@ -195,7 +234,68 @@ RepConsts rep_consts(const HostReplica &h)
     int g = 1;   // about 2 particles per Morton cell, as at admission
     while (g < 10 && (double)(1 << (3 * g)) < 0.5 * h.n) ++g;
     c.grid_bits = g;
+    c.list_items = h.list_items;
+    if (h.list_items > 0) {   // as the single-system lists: skin 5 % of the cutoff
+        c.lists_off = h.lists_off;
+        c.segcap = h.segcap;
+        c.skin = 0.05 * p.rccut;
+        const double rl = p.rccut + c.skin + c.margin;
+        c.rl2 = rl * rl;
+        c.thresh2 = 0.25 * c.skin * c.skin;
+        for (int k = 0; k < 3; ++k) {
+            c.Lf[k] = (float)c.L[k];
+            c.invLf[k] = (float)c.invL[k];
+            c.halfLf_m[k] = (float)(c.halfL_m[k] - 1e-5 * p.xl[k]);
+        }
+        c.rl2f = (float)(c.rl2 * (1.0 + 1e-5));
+        c.skinf = (float)c.skin;
+    }
     return c;
+}
+
+// First-fit region of the list arena for `len` entries; grows the arena
+// (copying its contents, so existing regions keep their offsets) if needed.
+int list_alloc(yk_batch *b, int64_t len, int64_t *off)
+{
+    const int64_t units = (len + LIST_UNIT - 1) / LIST_UNIT;
+    int64_t start = -1;
+    for (int64_t u = 0, run = 0; u < (int64_t)b->list_used.size(); ++u) {
+        run = b->list_used[u] ? 0 : run + 1;
+        if (run == units) {
+            start = u - units + 1;
+            break;
+        }
+    }
+    if (start < 0) {
+        // grow: keep the used prefix, append enough free units
+        int64_t tail = 0;
+        while (tail < (int64_t)b->list_used.size() && !b->list_used[b->list_used.size() - 1 - tail])
+            ++tail;
+        const int64_t old_units = (int64_t)b->list_used.size();
+        const int64_t new_units = std::max(old_units + units - tail, 2 * old_units);
+        unsigned *nc = nullptr;
+        YB_HIP(hipMalloc(&nc, sizeof(unsigned) * (size_t)(new_units * LIST_UNIT)));
+        if (b->clist) {
+            YB_HIP(hipMemcpyAsync(nc, b->clist, sizeof(unsigned) * (size_t)b->clist_cap,
+                                  hipMemcpyDeviceToDevice, b->stream));
+            YB_HIP(hipStreamSynchronize(b->stream));
+            YB_HIP(hipFree(b->clist));
+        }
+        b->clist = nc;
+        b->clist_cap = new_units * LIST_UNIT;
+        b->list_used.resize((size_t)new_units, 0);
+        b->dirty = true;   // kernels and the captured graph take the new pointer
+        start = old_units - tail;
+    }
+    for (int64_t u = start; u < start + units; ++u) b->list_used[(size_t)u] = 1;
+    *off = start * LIST_UNIT;
+    return YK_OK;
+}
+
+void list_free(yk_batch *b, int64_t off, int64_t len)
+{
+    const int64_t u0 = off / LIST_UNIT, units = (len + LIST_UNIT - 1) / LIST_UNIT;
+    for (int64_t u = u0; u < u0 + units; ++u) b->list_used[(size_t)u] = 0;
 }
 
 __global__ void k_fill_rep(int *rep, const int *tile_rep, const RepConsts *reps, int cap)
@@ -230,19 +330,45 @@ int rebuild(yk_batch *b)
     // is made of small items
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int c) { return b->table[a].n > b->table[c].n; });
+    b->nlist = 0;
     for (int r : order) {
         const HostReplica &h = b->table[r];
         const int t0 = h.off / TILE, nt = (h.n + TILE - 1) / TILE;
         const int C = chunk_tiles_for(b, h.n);
+        const int M = h.list_items;
+        b->nlist += (M > 0) ? 1 : 0;
         for (int t = t0; t < t0 + nt; ++t) {
             tile_rep[t] = r;
             tile_items[t].x = (int)items.size();
-            for (int s = t0; s < t0 + nt; s += C)
-                items.push_back(BatchItem{r, t, s, std::min(s + C, t0 + nt)});
+            if (M > 0) {
+                for (int m = 0; m < M; ++m) items.push_back(BatchItem{r, t, m, -M});
+            } else {
+                for (int s = t0; s < t0 + nt; s += C)
+                    items.push_back(BatchItem{r, t, s, std::min(s + C, t0 + nt)});
+            }
             tile_items[t].y = (int)items.size() - tile_items[t].x;
         }
     }
     b->nitems = (int)items.size();
+    b->span = 0;
+    std::vector<int> ltiles;
+    for (int r : order) {
+        const HostReplica &h = b->table[r];
+        b->span = std::max(b->span, h.off + h.nblocks * BLOCK);
+        if (h.list_items > 0)
+            for (int t = h.off / TILE; t < h.off / TILE + (h.n + TILE - 1) / TILE; ++t)
+                ltiles.push_back(t);
+    }
+    b->n_list_tiles = (int)ltiles.size();
+    if (ltiles.size() > b->list_tiles_cap) {
+        if (b->list_tiles) YB_HIP(hipFree(b->list_tiles));
+        b->list_tiles = nullptr;
+        b->list_tiles_cap = ltiles.size() + ltiles.size() / 4 + 64;
+        YB_HIP(hipMalloc(&b->list_tiles, sizeof(int) * b->list_tiles_cap));
+    }
+    if (!ltiles.empty())
+        YB_HIP(hipMemcpyAsync(b->list_tiles, ltiles.data(), sizeof(int) * ltiles.size(),
+                              hipMemcpyHostToDevice, b->stream));
     // launch shape: the waves per workgroup of the bucket carrying the most
     // pair work
     {
@@ -300,20 +426,25 @@ int persistent_blocks(const yk_batch *b)
 template <bool MEAS, int MODE>
 int enqueue_step(yk_batch *b)
 {
-    const dim3 slots((b->cap + BLOCK - 1) / BLOCK);
+    const dim3 slots(std::max(1, (b->span + BLOCK - 1) / BLOCK));
     const ExpCoeffs ec = exp_coeffs();
     hipLaunchKernelGGL((k_batch_drift<MODE == 1>), slots, dim3(BLOCK), 0, b->stream, b->pos, b->vel,
-                           b->vold, b->acc, b->rep, b->reps, b->state, b->cap, b->box);
+                           b->vold, b->acc, b->rep, b->reps, b->state, b->span, b->cap,
+                           b->box, b->tile_rep, b->xref, b->rep_maxdisp);
+    if (b->n_list_tiles > 0)
+        hipLaunchKernelGGL(k_batch_clist_build, dim3(b->n_list_tiles, NSEG), dim3(BLOCK), 0,
+                           b->stream, b->pos, b->box, b->list_tiles, b->tile_rep, b->reps,
+                           b->rep_maxdisp, b->clist, b->ccounts, b->xref);
     const size_t lds = sizeof(double4) * TILE * b->waves;
     hipLaunchKernelGGL((k_batch_pairs<MEAS>), dim3(persistent_blocks(b)), dim3(TILE * b->waves),
                        lds, b->stream, b->pos, b->box, b->reps, b->state, b->items, b->nitems,
-                       b->queue, ec, MODE == 0 ? 1 : 0, b->part);
+                       b->queue, ec, MODE == 0 ? 1 : 0, b->clist, b->ccounts, b->part);
     hipLaunchKernelGGL((k_batch_finalize<MEAS, MODE>), slots, dim3(BLOCK), 0, b->stream, b->part,
                        b->tile_items, b->pos, b->den, b->aii, b->orig, b->rep, b->reps,
-                       b->state, b->cap, b->acc, b->vel, b->vold, b->blockpart);
+                       b->state, b->span, b->cap, b->acc, b->vel, b->vold, b->blockpart);
     hipLaunchKernelGGL((k_batch_replica<MEAS, MODE>), dim3(b->max_rep), dim3(BLOCK), 0,
                        b->stream, b->blockpart, b->reps, b->state, b->max_rep, b->records,
-                       b->rec_cap, b->init_records, b->queue);
+                       b->rec_cap, b->init_records, b->queue, b->rep_maxdisp);
     YB_HIP(hipGetLastError());
     return YK_OK;
 }
@@ -377,31 +508,34 @@ int resort_due(yk_batch *b)
         }
     }
     if (!any) return YK_OK;
-    const size_t nn = (size_t)b->cap;
-    const dim3 slots((b->cap + BLOCK - 1) / BLOCK);
+    // the used extent of the arena only; component-major arrays have stride cap
+    const size_t nn = (size_t)b->cap, ns = (size_t)b->span;
+    const dim3 slots(std::max(1, (b->span + BLOCK - 1) / BLOCK));
     YB_HIP(hipMemcpyAsync(b->due, due.data(), sizeof(int) * b->max_rep, hipMemcpyHostToDevice,
                           b->stream));
     hipLaunchKernelGGL(k_batch_sort_keys, slots, dim3(BLOCK), 0, b->stream, b->pos, b->rep,
-                       b->reps, b->due, b->cap, b->keys, b->vals);
+                       b->reps, b->due, b->span, b->keys, b->vals, b->rep_maxdisp);
     YB_HIP(hipGetLastError());
     size_t bytes = b->sort_tmp_bytes;
     YB_HIP(hipcub::DeviceRadixSort::SortPairs(b->sort_tmp, bytes, b->keys, b->keys2, b->vals,
-                                              b->vals2, b->cap, 0, b->sort_end_bit, b->stream));
+                                              b->vals2, b->span, 0, b->sort_end_bit, b->stream));
     PermArrays a{b->pos, b->vel, b->vold, b->acc, b->aii, b->den, b->orig,
                  b->pos2, b->vel2, b->vold2, b->acc2, b->aii2, b->den2, b->orig2};
-    hipLaunchKernelGGL(k_permute, slots, dim3(BLOCK), 0, b->stream, b->vals2, b->cap, a);
+    hipLaunchKernelGGL(k_batch_permute, slots, dim3(BLOCK), 0, b->stream, b->vals2, b->span,
+                       b->cap, a);
     YB_HIP(hipGetLastError());
-    YB_HIP(hipMemcpyAsync(b->pos, b->pos2, sizeof(double4) * nn, hipMemcpyDeviceToDevice,
+    YB_HIP(hipMemcpyAsync(b->pos, b->pos2, sizeof(double4) * ns, hipMemcpyDeviceToDevice,
                           b->stream));
     for (auto pr : {std::make_pair(b->vel, b->vel2), std::make_pair(b->vold, b->vold2),
                     std::make_pair(b->acc, b->acc2)})
-        YB_HIP(hipMemcpyAsync(pr.first, pr.second, sizeof(double) * 3 * nn,
-                              hipMemcpyDeviceToDevice, b->stream));
-    YB_HIP(hipMemcpyAsync(b->aii, b->aii2, sizeof(double) * nn, hipMemcpyDeviceToDevice,
+        for (int c = 0; c < 3; ++c)
+            YB_HIP(hipMemcpyAsync(pr.first + c * nn, pr.second + c * nn, sizeof(double) * ns,
+                                  hipMemcpyDeviceToDevice, b->stream));
+    YB_HIP(hipMemcpyAsync(b->aii, b->aii2, sizeof(double) * ns, hipMemcpyDeviceToDevice,
                           b->stream));
-    YB_HIP(hipMemcpyAsync(b->den, b->den2, sizeof(double) * nn, hipMemcpyDeviceToDevice,
+    YB_HIP(hipMemcpyAsync(b->den, b->den2, sizeof(double) * ns, hipMemcpyDeviceToDevice,
                           b->stream));
-    YB_HIP(hipMemcpyAsync(b->orig, b->orig2, sizeof(int) * nn, hipMemcpyDeviceToDevice,
+    YB_HIP(hipMemcpyAsync(b->orig, b->orig2, sizeof(int) * ns, hipMemcpyDeviceToDevice,
                           b->stream));
     return YK_OK;
 }
@@ -548,6 +682,9 @@ int yk_batch_create(int64_t capacity, int32_t max_replicas, int32_t max_records,
     YB_HIPB(hipMalloc(&b->records, sizeof(MeasOut) * (size_t)max_replicas * b->rec_cap));
     YB_HIPB(hipMalloc(&b->init_records, sizeof(MeasOut) * (size_t)max_replicas));
     YB_HIPB(hipMalloc(&b->queue, sizeof(unsigned)));
+    YB_HIPB(hipMalloc(&b->xref, sizeof(double4) * nn));
+    YB_HIPB(hipMalloc(&b->ccounts, sizeof(int2) * (nn / CL) * NSEG));
+    YB_HIPB(hipMalloc(&b->rep_maxdisp, sizeof(unsigned long long) * (size_t)max_replicas));
     YB_HIPB(hipMalloc(&b->keys, sizeof(unsigned long long) * nn));
     YB_HIPB(hipMalloc(&b->keys2, sizeof(unsigned long long) * nn));
     YB_HIPB(hipMalloc(&b->vals, sizeof(int) * nn));
@@ -601,7 +738,9 @@ void yk_batch_destroy(yk_batch *b)
                     (void *)b->part, (void *)b->items, (void *)b->keys, (void *)b->keys2,
                     (void *)b->vals, (void *)b->vals2, (void *)b->due, b->sort_tmp,
                     (void *)b->pos2, (void *)b->vel2, (void *)b->vold2, (void *)b->acc2,
-                    (void *)b->aii2, (void *)b->den2, (void *)b->orig2})
+                    (void *)b->aii2, (void *)b->den2, (void *)b->orig2, (void *)b->xref,
+                    (void *)b->ccounts, (void *)b->rep_maxdisp, (void *)b->clist,
+                    (void *)b->list_tiles})
         if (p) (void)hipFree(p);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
@@ -616,7 +755,11 @@ int yk_batch_footprint(const yk_batch *b, int32_t n, int64_t *slots_out, int64_t
     if (slots_out) *slots_out = slots;
     if (bytes_out)
         *bytes_out = slots * SLOT_BYTES + slots / TILE * TILE_BYTES + slots / BLOCK * BLOCK_BYTES +
-                     items_of(b, n) * ITEM_BYTES;
+                     items_of(b, n) * ITEM_BYTES +
+                     (list_items_for(b, n) > 0
+                          ? (list_entries_of(n) + LIST_UNIT - 1) / LIST_UNIT * LIST_UNIT *
+                                (int64_t)sizeof(unsigned)
+                          : 0);
     return YK_OK;
 }
 
@@ -659,6 +802,15 @@ int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *
     h.nblocks = nblk;
     h.dt = dt;
     h.steps_left = steps;
+    h.list_items = list_items_for(b, n);
+    if (h.list_items > 0) {
+        h.segcap = segcap_of(n);
+        h.lists_len = list_entries_of(n);
+        if ((rc = list_alloc(b, h.lists_len, &h.lists_off))) {
+            h = HostReplica{};
+            return rc;
+        }
+    }
     const std::vector<int> order = morton_order(x, n, params->xl);
     // sorted host staging
     const size_t nb = (size_t)nblk * BLOCK;
@@ -697,6 +849,12 @@ int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *
                           b->stream));
     RepState st{steps, 0, 1, 0};
     YB_HIP(hipMemcpyAsync(b->state + id, &st, sizeof st, hipMemcpyHostToDevice, b->stream));
+    const double inf = __builtin_inf();   // lists to be built at the accel pass
+    unsigned long long infbits;
+    std::memcpy(&infbits, &inf, sizeof infbits);
+    b->md_stage = infbits;
+    YB_HIP(hipMemcpyAsync(b->rep_maxdisp + id, &b->md_stage, sizeof infbits,
+                          hipMemcpyHostToDevice, b->stream));
     YB_HIP(hipStreamSynchronize(b->stream));
     for (int s = start; s < start + nblk; ++s) b->used[s] = 1;
     h.active = true;
@@ -715,19 +873,24 @@ int yk_batch_remove(yk_batch *b, int32_t id)
     RepState st{0, 0, 0, 0};
     YB_HIP(hipMemcpy(b->state + id, &st, sizeof st, hipMemcpyHostToDevice));
     for (int s = h.off / BLOCK; s < h.off / BLOCK + h.nblocks; ++s) b->used[s] = 0;
+    if (h.list_items > 0) list_free(b, h.lists_off, h.lists_len);
     h = HostReplica{};
     b->dirty = true;
     return YK_OK;
 }
 
 int yk_batch_set_tuning(yk_batch *b, int32_t nbuckets, const int32_t *n_upper,
-                        const int32_t *chunk_tiles, const int32_t *waves_per_block)
+                        const int32_t *chunk_tiles, const int32_t *waves_per_block,
+                        const int32_t *list_items)
 {
-    if (!b || nbuckets < 0 || (nbuckets > 0 && (!n_upper || !chunk_tiles || !waves_per_block)))
+    if (!b || nbuckets < 0 ||
+        (nbuckets > 0 && (!n_upper || !chunk_tiles || !waves_per_block || !list_items)))
         return bfail(YK_ERR_ARG, "invalid tuning table");
     for (int k = 0; k < nbuckets; ++k) {
         const int w = waves_per_block[k];
         if (chunk_tiles[k] < 1) return bfail(YK_ERR_ARG, "chunk_tiles must be >= 1");
+        if (list_items[k] < 0 || list_items[k] > 1024)
+            return bfail(YK_ERR_ARG, "list_items must be in [0, 1024]");
         if (w != 1 && w != 2 && w != 4 && w != 8)
             return bfail(YK_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
         if (k > 0 && n_upper[k] <= n_upper[k - 1])
@@ -736,6 +899,7 @@ int yk_batch_set_tuning(yk_batch *b, int32_t nbuckets, const int32_t *n_upper,
     b->tune_nupper.assign(n_upper, n_upper + nbuckets);
     b->tune_chunk.assign(chunk_tiles, chunk_tiles + nbuckets);
     b->tune_waves.assign(waves_per_block, waves_per_block + nbuckets);
+    b->tune_list.assign(list_items, list_items + nbuckets);
     b->dirty = true;
     return YK_OK;
 }

@@ -30,6 +30,13 @@ struct RepConsts {
     int kind, prec, off, n;
     int tile0, ntiles, blk0, nblk;
     int ref_f1, grid_bits;   // grid_bits: Morton cells per axis = 2^grid_bits
+    // cluster lists (list_items > 0): the replica's region of the list arena
+    // (entries), segment capacity, list radius^2, skin, rebuild threshold
+    // (skin/2)^2 and the FP32 bounds of the candidate test (as ListConsts)
+    long long lists_off;
+    int list_items, segcap;
+    double rl2, skin, thresh2;
+    float Lf[3], invLf[3], halfLf_m[3], rl2f, skinf;
 };
 
 struct RepState {            // device-updated per replica
@@ -40,7 +47,9 @@ struct RepState {            // device-updated per replica
 };
 
 struct BatchItem {
-    int rep, tile, s0, s1;   // target tile and source tiles [s0, s1), global indices
+    // tile item: target tile and source tiles [s0, s1), global indices;
+    // list item (s1 < 0): target tile's four clusters, share s0 of -s1
+    int rep, tile, s0, s1;
 };
 
 // ---------------------------------------------------------------------------
@@ -132,16 +141,157 @@ __device__ __forceinline__ void batch_item(const RepConsts &rc, const ExpCoeffs 
     }
 }
 
+// A cluster's list as four segments: entry v (< hi) of the concatenation.
+struct SegList {
+    const unsigned *base;
+    int segcap, hi, b1, b2, b3, nf0, nf1, nf2, nf3;
+    __device__ __forceinline__ unsigned entry(int v, bool &general) const
+    {
+        general = false;
+        if (v >= hi) return 0u;
+        // arithmetic selection of the segment (selects of addresses would put
+        // the fields on the stack)
+        const int s1 = v >= b1, s2 = v >= b2, s3 = v >= b3;
+        const int sg = s1 + s2 + s3;
+        const int start = s1 * (b1 + s2 * (b2 - b1 + s3 * (b3 - b2)));
+        const int nfs = nf0 + s1 * (nf1 - nf0 + s2 * (nf2 - nf1 + s3 * (nf3 - nf2)));
+        const int o = v - start;
+        const unsigned *r = base + (size_t)sg * segcap;
+        general = o >= nfs;
+        return general ? r[segcap - 1 - (o - nfs)] : r[o];
+    }
+};
+
+// One list item: the four 16-slot clusters of target tile it.tile against
+// share m of M of each cluster's list (the scheme of k_pairs_clist, with the
+// replica's constants and its region of the list arena).
+template <int KIND, int PREC, bool MEAS>
+__device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCoeffs &ec,
+                                                const double4 *__restrict__ pos,
+                                                const unsigned *__restrict__ clist,
+                                                const int2 *__restrict__ ccounts,
+                                                const BatchItem &it, double4 *__restrict__ sw,
+                                                int lane, double *__restrict__ out)
+{
+    ForceConsts c{};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        c.L[k] = rc.L[k];
+        c.invL[k] = rc.invL[k];
+        c.halfL_m[k] = rc.halfL_m[k];
+    }
+    c.rc2 = rc.rc2;
+    c.mu = rc.mu;
+    c.evcut = rc.evcut;
+    c.muf = rc.muf;
+    c.mul2ef = rc.mul2ef;
+    c.ec = ec;
+    const int M = -it.s1, m = it.s0;
+    const int g = lane / CL, gl = lane % CL;
+    const int t = it.tile;
+    const int i = t * TILE + lane;
+    const bool ivalid = i - rc.off < rc.n;
+    const double nanv = __builtin_nan("");
+    const double4 pi = ivalid ? pos[i] : make_double4(nanv, nanv, nanv, 0.0);
+    const int clg = t * CPW + g;                    // global cluster (ccounts index)
+    const int cll = (t - rc.tile0) * CPW + g;       // cluster within the replica
+    const bool cvalid = cll * CL < rc.n;            // clusters past n have no list
+    // segment boundaries and fast counts as scalars (NSEG = 4), so nothing
+    // is indexed dynamically
+    static_assert(NSEG == 4, "segment selection below is written for four segments");
+    int2 c0 = make_int2(0, 0), c1 = c0, c2 = c0, c3 = c0;
+    if (cvalid) {
+        const int2 *cp = ccounts + (size_t)clg * NSEG;
+        c0 = cp[0]; c1 = cp[1]; c2 = cp[2]; c3 = cp[3];
+    }
+    const int b1 = c0.x + c0.y, b2 = b1 + c1.x + c1.y, b3 = b2 + c2.x + c2.y;
+    const int total = b3 + c3.x + c3.y;
+    const int per = (total + M - 1) / M;
+    const int lo = min(m * per, total), hi = min(lo + per, total);
+    const int share = hi - lo;
+    int iters = share;
+#pragma unroll
+    for (int off = CL; off < TILE; off <<= 1) {
+        const int o = __shfl_xor(iters, off, 64);
+        iters = o > iters ? o : iters;
+    }
+    iters = __builtin_amdgcn_readfirstlane(iters);
+    const int segcap = rc.segcap;
+    const unsigned *mine = clist + rc.lists_off + (size_t)cll * NSEG * segcap;
+    const SegList sl{mine, segcap, hi, b1, b2, b3, c0.x, c1.x, c2.x, c3.x};
+    Acc<MEAS> acc;
+    bool g_cur, g_nxt;
+    unsigned e_cur = sl.entry(lo + gl, g_cur);
+    double4 pj_cur = pos[rc.off + (e_cur & SLOT_MASK)];
+    unsigned e_nxt = sl.entry(lo + CL + gl, g_nxt);
+    for (int b0 = 0; b0 < iters; b0 += CL) {
+        const int v = lo + b0 + gl;
+        bool genl = false;
+        __builtin_amdgcn_wave_barrier();   // the previous batch's reads are done
+        if (v < hi) {
+            double4 pj = pj_cur;
+            if (!g_cur) {   // apply the entry's periodic image
+                pj.x += c.L[0] * (double)((int)((e_cur >> 26) & 3u) - 1);
+                pj.y += c.L[1] * (double)((int)((e_cur >> 28) & 3u) - 1);
+                pj.z += c.L[2] * (double)((int)((e_cur >> 30) & 3u) - 1);
+            } else {
+                genl = true;
+            }
+            pj.w = stage_w<KIND, PREC>(pj.w);
+            sw[lane] = pj;
+        }
+        const bool gen = __ballot(genl) != 0ull;
+        __builtin_amdgcn_wave_barrier();
+        e_cur = e_nxt;
+        g_cur = g_nxt;
+        pj_cur = pos[rc.off + (e_cur & SLOT_MASK)];
+        e_nxt = sl.entry(lo + b0 + 2 * CL + gl, g_nxt);
+        const int kb = min(CL, iters - b0);
+        const int mycount = share - b0;
+        const double4 *gs = sw + g * CL;
+        if (!gen) {
+            double4 pn = gs[0];
+#pragma unroll 4
+            for (int k = 0; k < kb; ++k) {
+                const double4 pj = pn;
+                pn = gs[min(k + 1, CL - 1)];
+                interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w, pi.x - pj.x, pi.y - pj.y,
+                                                 pi.z - pj.z, k >= mycount, acc);
+            }
+        } else {
+#pragma unroll 2
+            for (int k = 0; k < kb; ++k) {
+                const double4 pj = gs[k];
+                interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w,
+                                                 min_image(pi.x - pj.x, c.L[0], c.invL[0]),
+                                                 min_image(pi.y - pj.y, c.L[1], c.invL[1]),
+                                                 min_image(pi.z - pj.z, c.L[2], c.invL[2]),
+                                                 k >= mycount, acc);
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    double *o = out + lane;
+    o[0] = acc.fx; o[TILE] = acc.fy; o[2 * TILE] = acc.fz;
+    if constexpr (MEAS) {
+        o[3 * TILE] = acc.e; o[4 * TILE] = acc.w;
+        o[5 * TILE] = acc.pxx; o[6 * TILE] = acc.pxy; o[7 * TILE] = acc.pxz;
+        o[8 * TILE] = acc.pyy; o[9 * TILE] = acc.pyz; o[10 * TILE] = acc.pzz;
+    }
+}
+
 // Persistent: every wave loops pulling items until the queue is empty.
 // accel_pass != 0 evaluates only replicas admitted since the last pass
 // (fresh); otherwise replicas whose step budget is used up are skipped.
 template <bool MEAS>
-__global__ void k_batch_pairs(const double4 *__restrict__ pos, const TileBox *__restrict__ box,
+__global__ __launch_bounds__(512) void k_batch_pairs(const double4 *__restrict__ pos,
+                                                     const TileBox *__restrict__ box,
                               const RepConsts *__restrict__ reps,
                               const RepState *__restrict__ state,
                               const BatchItem *__restrict__ items, int nitems,
                               unsigned *__restrict__ queue, ExpCoeffs ec, int accel_pass,
-                              double *__restrict__ part)
+                              const unsigned *__restrict__ clist,
+                              const int2 *__restrict__ ccounts, double *__restrict__ part)
 {
     extern __shared__ double4 sj_dyn[];
     const int lane = threadIdx.x & (TILE - 1);
@@ -164,6 +314,20 @@ __global__ void k_batch_pairs(const double4 *__restrict__ pos, const TileBox *__
         }
         const int kind = __builtin_amdgcn_readfirstlane(rc.kind);
         const int prec = __builtin_amdgcn_readfirstlane(rc.prec);
+        if (it.s1 < 0) {
+            if (kind == PURE) {
+                if (prec == 0)
+                    batch_list_item<PURE, 0, MEAS>(rc, ec, pos, clist, ccounts, it, sw, lane, out);
+                else
+                    batch_list_item<PURE, 1, MEAS>(rc, ec, pos, clist, ccounts, it, sw, lane, out);
+            } else {
+                if (prec == 0)
+                    batch_list_item<MIX, 0, MEAS>(rc, ec, pos, clist, ccounts, it, sw, lane, out);
+                else
+                    batch_list_item<MIX, 1, MEAS>(rc, ec, pos, clist, ccounts, it, sw, lane, out);
+            }
+            continue;
+        }
         if (kind == PURE) {
             if (prec == 0) batch_item<PURE, 0, MEAS>(rc, ec, pos, box, it, sw, lane, out);
             else batch_item<PURE, 1, MEAS>(rc, ec, pos, box, it, sw, lane, out);
@@ -189,11 +353,17 @@ __global__ __launch_bounds__(BLOCK) void k_batch_drift(double4 *__restrict__ pos
                                                        const int *__restrict__ rep,
                                                        const RepConsts *__restrict__ reps,
                                                        const RepState *__restrict__ state,
-                                                       int cap, TileBox *__restrict__ box)
+                                                       int nslots, int cap,
+                                                       TileBox *__restrict__ box,
+                                                       const int *__restrict__ tile_rep,
+                                                       const double4 *__restrict__ xref,
+                                                       unsigned long long *__restrict__ maxdisp)
 {
+    // slots [0, nslots) are the used extent of the arena; cap is the stride of
+    // the component-major arrays
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     const size_t nn = (size_t)cap;
-    const int r = (i < cap) ? rep[i] : -1;
+    const int r = (i < nslots) ? rep[i] : -1;
     const bool valid = (r >= 0);
     double x[3] = {0, 0, 0};
     if (valid) {
@@ -212,7 +382,167 @@ __global__ __launch_bounds__(BLOCK) void k_batch_drift(double4 *__restrict__ pos
         }
     }
     const int tile = i / TILE;
-    if (tile < cap / TILE) tile_box_store(x, valid, tile, box);
+    if (tile < nslots / TILE) {
+        tile_box_store(x, valid, tile, box);
+        // largest displacement since the replica's list build (list replicas)
+        const int tr = tile_rep[tile];   // one replica per tile: wave-uniform
+        if (MOVE && tr >= 0 && reps[tr].list_items > 0) {
+            double d2 = 0.0;
+            if (valid) {
+                const double4 q = xref[i];
+                const double dx = x[0] - q.x, dy = x[1] - q.y, dz = x[2] - q.z;
+                d2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double o = __shfl_xor(d2, off, 64);
+                d2 = o > d2 ? o : d2;
+            }
+            if ((threadIdx.x & (TILE - 1)) == 0 && d2 > 0.0) atomicMax(&maxdisp[tr], as_bits(d2));
+        }
+    }
+}
+
+// Cluster lists of the list replicas (grid: arena tiles x NSEG segments),
+// rebuilt for a replica whose largest displacement since its last build
+// exceeds skin/2 (set to +inf at admission and after a re-sort).  Same
+// scheme as k_clist_build with the replica's slot range, constants and list
+// region; entries hold slots relative to the replica's first slot.
+__global__ __launch_bounds__(BLOCK) void k_batch_clist_build(
+    const double4 *__restrict__ pos, const TileBox *__restrict__ box,
+    const int *__restrict__ list_tiles, const int *__restrict__ tile_rep,
+    const RepConsts *__restrict__ reps, const unsigned long long *__restrict__ maxdisp,
+    unsigned *__restrict__ clist, int2 *__restrict__ ccounts, double4 *__restrict__ xref)
+{
+    const int t = list_tiles[blockIdx.x];   // tiles of list replicas
+    const int seg = blockIdx.y;
+    const int r = tile_rep[t];
+    if (r < 0) return;
+    const RepConsts &rc = reps[r];
+    if (rc.list_items <= 0 || maxdisp[r] <= as_bits(rc.thresh2)) return;   // uniform
+    constexpr int W = BLOCK / TILE;
+    __shared__ double4 stage[2][W][TILE];
+    __shared__ int ctiles[CLIST_MAX_N / TILE / NSEG + 1];
+    const int lane = threadIdx.x & (TILE - 1), wv = threadIdx.x / TILE;
+    const int cll = (t - rc.tile0) * CPW + wv;   // cluster within the replica
+    const bool cvalid = cll * CL < rc.n;
+    double c[3], h[3];
+    {
+        const int il = cll * CL + lane;
+        const bool v = lane < CL && il < rc.n;
+        const double4 p = v ? pos[rc.off + il] : make_double4(0, 0, 0, 0);
+        if (v && seg == 0) xref[rc.off + il] = p;
+        const double xi[3] = {p.x, p.y, p.z};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double lo = v ? xi[k] : __builtin_inf(), hi = v ? xi[k] : -__builtin_inf();
+            wave_minmax(lo, hi);
+            c[k] = 0.5 * (lo + hi);
+            h[k] = 0.5 * (hi - lo);
+        }
+    }
+    const bool live = cvalid && h[0] >= 0.0;
+    float hf[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) hf[k] = live ? (float)h[k] * 1.000001f : 0.0f;
+    int ncand = 0;
+    {
+        const double rl = __builtin_sqrt(rc.rl2);
+        double tc[3], th[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { tc[k] = box[t].c[k]; th[k] = box[t].h[k]; }
+        const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        for (int tb = 0; tb < rc.ntiles; tb += TILE * NSEG) {
+            const int stl = tb + lane * NSEG + seg;   // source tile within the replica
+            bool tcand = false;
+            if (stl < rc.ntiles) {
+                const int st = rc.tile0 + stl;
+                double gap2 = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    double d = tc[k] - box[st].c[k];
+                    d = __builtin_fma(-rc.L[k], __builtin_rint(d * rc.invL[k]), d);
+                    double gk = __builtin_fabs(d) - th[k] - box[st].h[k];
+                    gk = gk > 0.0 ? gk : 0.0;
+                    gap2 = __builtin_fma(gk, gk, gap2);
+                }
+                tcand = gap2 <= rl * rl;
+            }
+            const unsigned long long tm = __ballot(tcand);
+            if (wv == 0 && tcand) ctiles[ncand + __popcll(tm & lt)] = stl;
+            ncand += __popcll(tm);
+        }
+    }
+    __syncthreads();
+    unsigned *dst = clist + rc.lists_off + ((size_t)(cvalid ? cll : 0) * NSEG + seg) * rc.segcap;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    auto load_round = [&](int q) -> double4 {
+        const int idx = q * W + wv;
+        const int jl = (idx < ncand) ? ctiles[idx] * TILE + lane : rc.n;
+        return (jl < rc.n) ? pos[rc.off + jl] : make_double4(0, 0, 0, 0);
+    };
+    const int nrounds = (ncand + W - 1) / W;
+    double4 pa = load_round(0), pb = load_round(1);
+    int nf = 0, ng = 0;
+    for (int q = 0; q < nrounds; ++q) {
+        stage[q & 1][wv][lane] = pa;
+        pa = pb;
+        pb = load_round(q + 2);
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const int idx = q * W + w;
+            const int jl = (idx < ncand) ? ctiles[idx] * TILE + lane : rc.n;
+            bool cand = false, fast = true;
+            unsigned code = 0;
+            if (live && jl < rc.n) {
+                const double4 pj = stage[q & 1][w][lane];
+                const double xj[3] = {pj.x, pj.y, pj.z};
+                float d2 = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float cd = (float)(c[k] - xj[k]);
+                    const float m = __builtin_rintf(cd * rc.invLf[k]);
+                    const float ev = __builtin_fabsf(__builtin_fmaf(-rc.Lf[k], m, cd));
+                    float gk = ev - hf[k];
+                    gk = gk > 0.0f ? gk : 0.0f;
+                    d2 = __builtin_fmaf(gk, gk, d2);
+                    const bool m_ok = (m >= -1.0f) && (m <= 1.0f);
+                    fast = fast && m_ok && (ev + hf[k] + rc.skinf < rc.halfLf_m[k]);
+                    code |= (unsigned)(m_ok ? (int)m + 1 : 1) << (26 + 2 * k);
+                }
+                cand = d2 <= rc.rl2f;
+            }
+            const unsigned long long bf = __ballot(cand && fast);
+            const unsigned long long bg = __ballot(cand && !fast);
+            if (cand && fast) dst[nf + __popcll(bf & lt)] = (unsigned)jl | code;
+            if (cand && !fast) dst[rc.segcap - 1 - (ng + __popcll(bg & lt))] = (unsigned)jl | code;
+            nf += __popcll(bf);
+            ng += __popcll(bg);
+        }
+    }
+    if (cvalid && lane == 0) ccounts[(t * CPW + wv) * NSEG + seg] = make_int2(nf, ng);
+}
+
+// Gather of every per-slot array by the sorted permutation over the used
+// extent [0, nslots) of the arena (component-major arrays have stride cap).
+__global__ __launch_bounds__(BLOCK) void k_batch_permute(const int *__restrict__ perm, int nslots,
+                                                         int cap, PermArrays a)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nslots) return;
+    const size_t nn = (size_t)cap;
+    const int s = perm[i];
+    a.pos2[i] = a.pos[s];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        a.vel2[k * nn + i] = a.vel[k * nn + s];
+        a.vold2[k * nn + i] = a.vold[k * nn + s];
+        a.acc2[k * nn + i] = a.acc[k * nn + s];
+    }
+    a.aii2[i] = a.aii[s];
+    a.den2[i] = a.den[s];
+    a.orig2[i] = a.orig[s];
 }
 
 // Keys of an arena re-sort (a stable radix sort of 64-bit keys over every
@@ -228,7 +558,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch_sort_keys(double4 *__restrict__
                                                            const RepConsts *__restrict__ reps,
                                                            const int *__restrict__ due, int cap,
                                                            unsigned long long *__restrict__ key,
-                                                           int *__restrict__ val)
+                                                           int *__restrict__ val,
+                                                           unsigned long long *__restrict__ maxdisp)
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= cap) return;
@@ -236,6 +567,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_sort_keys(double4 *__restrict__
     unsigned long long k = (unsigned long long)i << 32;
     if (r >= 0 && due[r]) {
         const RepConsts &rc = reps[r];
+        if (i == rc.off) maxdisp[r] = as_bits(__builtin_inf());   // slots move: rebuild lists
         const int g = rc.grid_bits;
         const int cmax = (1 << g) - 1;
         const double4 p = pos[i];
@@ -263,7 +595,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_finalize(
     const double *__restrict__ part, const int2 *__restrict__ tile_items,
     const double4 *__restrict__ pos, const double *__restrict__ den,
     const double *__restrict__ aii, const int *__restrict__ orig, const int *__restrict__ rep,
-    const RepConsts *__restrict__ reps, const RepState *__restrict__ state, int cap,
+    const RepConsts *__restrict__ reps, const RepState *__restrict__ state, int nslots, int cap,
     double *__restrict__ acc, double *__restrict__ vel, const double *__restrict__ vold,
     double *__restrict__ blockpart)
 {
@@ -273,7 +605,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_finalize(
     double q[NQ];
 #pragma unroll
     for (int k = 0; k < NQ; ++k) q[k] = 0.0;
-    const int r = (i < cap) ? rep[i] : -1;
+    const int r = (i < nslots) ? rep[i] : -1;
     if (r >= 0 && (MODE == 0 ? state[r].fresh != 0 : state[r].steps_left > 0)) {
         const RepConsts &rc = reps[r];
         const int t = i / TILE, lane = i % TILE;
@@ -337,12 +669,17 @@ __global__ __launch_bounds__(BLOCK) void k_batch_replica(const double *__restric
                                                          int nrep, MeasOut *__restrict__ records,
                                                          int rec_cap,
                                                          MeasOut *__restrict__ init_records,
-                                                         unsigned *__restrict__ queue)
+                                                         unsigned *__restrict__ queue,
+                                                         unsigned long long *__restrict__ maxdisp)
 {
     const int r = blockIdx.x;
     if (r == 0 && threadIdx.x == 0) *queue = 0u;
     if (r >= nrep) return;
     const RepConsts &rc = reps[r];
+    // the list build of this step has consumed the displacement flag
+    if (threadIdx.x == 0 && rc.list_items > 0 && maxdisp[r] > as_bits(rc.thresh2) &&
+        (MODE == 0 ? state[r].fresh != 0 : state[r].steps_left > 0))
+        maxdisp[r] = 0ull;
     const bool live = rc.n > 0 && (MODE == 0 ? state[r].fresh != 0 : state[r].steps_left > 0);
     if constexpr (MEAS) {
         double q[NQ];

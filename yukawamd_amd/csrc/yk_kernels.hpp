@@ -814,14 +814,15 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
     auto entry_at = [&](int v, bool &general) -> unsigned {
         general = false;
         if (v >= hi) return 0u;
-        int sg = 0;
+        // segment of entry v (unrolled selects: no dynamic register indexing)
+        int o = v, nfs = sc[0].x, sg = 0;
 #pragma unroll
-        for (int q = 1; q < NSEG; ++q) sg += (v >= sbeg[q]) ? 1 : 0;
-        const int o = v - sbeg[sg];
+        for (int q = 1; q < NSEG; ++q)
+            if (v >= sbeg[q]) { o = v - sbeg[q]; nfs = sc[q].x; sg = q; }
         const unsigned *r = mine + (size_t)sg * segcap;
-        if (o < sc[sg].x) return r[o];
+        if (o < nfs) return r[o];
         general = true;
-        return r[segcap - 1 - (o - sc[sg].x)];
+        return r[segcap - 1 - (o - nfs)];
     };
     Acc<MEAS> acc;
     double4 *sw = sj[wv];

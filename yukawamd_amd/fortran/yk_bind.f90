@@ -199,12 +199,12 @@ module yk_bind
       integer(c_int32_t), value :: id
     end function
 
-    integer(c_int) function yk_batch_set_tuning(b, nbuckets, n_upper, chunk_tiles, waves) &
-        bind(C, name='yk_batch_set_tuning')
+    integer(c_int) function yk_batch_set_tuning(b, nbuckets, n_upper, chunk_tiles, waves, &
+                                                list_items) bind(C, name='yk_batch_set_tuning')
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: b
       integer(c_int32_t), value :: nbuckets
-      integer(c_int32_t), intent(in) :: n_upper(*), chunk_tiles(*), waves(*)
+      integer(c_int32_t), intent(in) :: n_upper(*), chunk_tiles(*), waves(*), list_items(*)
     end function
 
     integer(c_int) function yk_batch_set_resort(b, every) bind(C, name='yk_batch_set_resort')
