@@ -25,7 +25,7 @@ struct RepConsts {
     double L[3], invL[3], halfL_m[3];
     double rc2, mu, evcut, cnum, margin, cull2;
     double dt, afac, rho, kT, xmass, vol;
-    float muf, mul2ef;
+    float muf, mul2ef, mul2e_c;   // mul2e_c: as ForceConsts::mul2e_c
     float three_n, n_f;
     int kind, prec, off, n;
     int tile0, ntiles, blk0, nblk;
@@ -72,6 +72,7 @@ __device__ __forceinline__ void batch_item(const RepConsts &rc, const ExpCoeffs 
     c.evcut = rc.evcut;
     c.muf = rc.muf;
     c.mul2ef = rc.mul2ef;
+    c.mul2e_c = rc.mul2e_c;
     c.ec = ec;
     const int t = it.tile;
     const int i = t * TILE + lane;
@@ -185,6 +186,7 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
     c.evcut = rc.evcut;
     c.muf = rc.muf;
     c.mul2ef = rc.mul2ef;
+    c.mul2e_c = rc.mul2e_c;
     c.ec = ec;
     const int M = -it.s1, m = it.s0;
     const int g = lane / CL, gl = lane % CL;

@@ -50,6 +50,10 @@ struct ForceConsts {
     double margin;                       // absolute rounding margin for AABB tests
     double xp2i, xpi, aa, aa2, bpc, bmc, vc;
     float muf, mul2ef, xp2il2ef, xpif, aaf, aa2f, bpcf, bmcf, vcf;
+    // -(mu log2 e - mul2ef) ln 2: first-order correction of exp2(-mul2ef r)
+    // for the rounding of mu log2 e to float, which would otherwise bias every
+    // pair at distance r by the same relative amount (mu r x 6e-8 at most)
+    float mul2e_c;
     int n, ntiles, coul_on, nuc_on;
     ExpCoeffs ec;
 };
@@ -154,7 +158,7 @@ __device__ __forceinline__ void interact(const ForceConsts &c, double zi, double
                 const float r2f = (float)r2;
                 const float rinv = rsqrt32(r2f);
                 const float r = r2f * rinv;
-                const float ex = exp2_32(-c.mul2ef * r);
+                const float ex = exp2_32(-c.mul2ef * r) * __builtin_fmaf(c.mul2e_c, r, 1.0f);
                 const float exr = ex * rinv;
                 float fc = __builtin_fmaf(c.muf, r, 1.0f) * exr * (rinv * rinv);
                 if constexpr (KIND == MIX) fc = staged_zf(zj) * fc;
@@ -200,7 +204,7 @@ __device__ __forceinline__ void interact(const ForceConsts &c, double zi, double
                 if (hc) {
                     float rinv = rsqrt32(r2f);
                     float r = r2f * rinv;
-                    float ex = exp2_32(-c.mul2ef * r);
+                    float ex = exp2_32(-c.mul2ef * r) * __builtin_fmaf(c.mul2e_c, r, 1.0f);
                     float exr = c.vcf * ex * rinv;
                     sf += __builtin_fmaf(c.muf, r, 1.0f) * exr * (rinv * rinv);
                     if constexpr (MEAS) phi += (double)exr - c.evcut;
