@@ -55,12 +55,14 @@ struct BatchItem {
 // ---------------------------------------------------------------------------
 template <int KIND, int PREC, bool MEAS>
 __device__ __forceinline__ void batch_item(const RepConsts &rc, const ExpCoeffs &ec,
+                                           const double *__restrict__ etab,
                                            const double4 *__restrict__ pos,
                                            const TileBox *__restrict__ box, const BatchItem &it,
                                            double4 *__restrict__ sw, int lane,
                                            double *__restrict__ out)
 {
     ForceConsts c{};
+    c.etab = etab;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         c.L[k] = rc.L[k];
@@ -168,6 +170,7 @@ struct SegList {
 // replica's constants and its region of the list arena).
 template <int KIND, int PREC, bool MEAS>
 __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCoeffs &ec,
+                                                const double *__restrict__ etab,
                                                 const double4 *__restrict__ pos,
                                                 const unsigned *__restrict__ clist,
                                                 const int2 *__restrict__ ccounts,
@@ -175,6 +178,7 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
                                                 int lane, double *__restrict__ out)
 {
     ForceConsts c{};
+    c.etab = etab;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         c.L[k] = rc.L[k];
@@ -296,8 +300,11 @@ __global__ __launch_bounds__(512) void k_batch_pairs(const double4 *__restrict__
                               const int2 *__restrict__ ccounts, double *__restrict__ part)
 {
     extern __shared__ double4 sj_dyn[];
+    __shared__ double etab_s[8][64];   // up to 8 waves per workgroup
     const int lane = threadIdx.x & (TILE - 1);
     double4 *sw = sj_dyn + (threadIdx.x / TILE) * TILE;
+    double *etab = etab_s[threadIdx.x / TILE];
+    load_exp_table(etab, ec, lane);
     constexpr int NC = MEAS ? 11 : 3;
     for (;;) {
         unsigned k = 0;
@@ -319,23 +326,23 @@ __global__ __launch_bounds__(512) void k_batch_pairs(const double4 *__restrict__
         if (it.s1 < 0) {
             if (kind == PURE) {
                 if (prec == 0)
-                    batch_list_item<PURE, 0, MEAS>(rc, ec, pos, clist, ccounts, it, sw, lane, out);
+                    batch_list_item<PURE, 0, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, lane, out);
                 else
-                    batch_list_item<PURE, 1, MEAS>(rc, ec, pos, clist, ccounts, it, sw, lane, out);
+                    batch_list_item<PURE, 1, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, lane, out);
             } else {
                 if (prec == 0)
-                    batch_list_item<MIX, 0, MEAS>(rc, ec, pos, clist, ccounts, it, sw, lane, out);
+                    batch_list_item<MIX, 0, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, lane, out);
                 else
-                    batch_list_item<MIX, 1, MEAS>(rc, ec, pos, clist, ccounts, it, sw, lane, out);
+                    batch_list_item<MIX, 1, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, lane, out);
             }
             continue;
         }
         if (kind == PURE) {
-            if (prec == 0) batch_item<PURE, 0, MEAS>(rc, ec, pos, box, it, sw, lane, out);
-            else batch_item<PURE, 1, MEAS>(rc, ec, pos, box, it, sw, lane, out);
+            if (prec == 0) batch_item<PURE, 0, MEAS>(rc, ec, etab, pos, box, it, sw, lane, out);
+            else batch_item<PURE, 1, MEAS>(rc, ec, etab, pos, box, it, sw, lane, out);
         } else {
-            if (prec == 0) batch_item<MIX, 0, MEAS>(rc, ec, pos, box, it, sw, lane, out);
-            else batch_item<MIX, 1, MEAS>(rc, ec, pos, box, it, sw, lane, out);
+            if (prec == 0) batch_item<MIX, 0, MEAS>(rc, ec, etab, pos, box, it, sw, lane, out);
+            else batch_item<MIX, 1, MEAS>(rc, ec, etab, pos, box, it, sw, lane, out);
         }
     }
 }

@@ -56,6 +56,7 @@ struct ForceConsts {
     float mul2e_c;
     int n, ntiles, coul_on, nuc_on;
     ExpCoeffs ec;
+    const double *etab;   // the wave's LDS copy of ec.t64 (set in the kernel)
 };
 
 struct StepConsts {
@@ -146,7 +147,7 @@ __device__ __forceinline__ void interact(const ForceConsts &c, double zi, double
                 double rinv = rsqrt64(r2, c.ec);
                 double r = r2 * rinv;
                 double mr = c.mu * r;
-                double ex = exp_neg64(-mr, c.ec);
+                double ex = exp_neg64t(-mr, c.ec, c.etab);
                 double exr = ex * rinv;
                 double fc = (1.0 + mr) * exr * (rinv * rinv);
                 s = (KIND == MIX) ? zj * fc : fc;
@@ -178,7 +179,7 @@ __device__ __forceinline__ void interact(const ForceConsts &c, double zi, double
             const bool same = (zi == zj);
             if constexpr (PREC == 0) {
                 if (hn) {
-                    double g = exp_neg64(-r2 * c.xp2i, c.ec);
+                    double g = exp_neg64t(-r2 * c.xp2i, c.ec, c.etab);
                     double bc = same ? c.bpc : c.bmc;
                     s = c.xpi * __builtin_fma(c.aa2, g, bc) * g;
                     if constexpr (MEAS) phi = __builtin_fma(c.aa, g, bc) * g;
@@ -187,7 +188,7 @@ __device__ __forceinline__ void interact(const ForceConsts &c, double zi, double
                     double rinv = rsqrt64(r2, c.ec);
                     double r = r2 * rinv;
                     double mr = c.mu * r;
-                    double ex = exp_neg64(-mr, c.ec);
+                    double ex = exp_neg64t(-mr, c.ec, c.etab);
                     double exr = c.vc * ex * rinv;
                     s += (1.0 + mr) * exr * (rinv * rinv);
                     if constexpr (MEAS) phi += exr - c.evcut;
@@ -235,8 +236,13 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const double4 *__restrict__ pos
 {
     int nswept = 0;   // source tiles this wave evaluated
     __shared__ double4 sj[STAGE];
+    __shared__ double etab_s[BLOCK / TILE][64];
     const int n = c.n;
     const int lane = threadIdx.x & (TILE - 1);
+    if constexpr (PREC == 0) {
+        load_exp_table(etab_s[threadIdx.x / TILE], c.ec, lane);
+        c.etab = etab_s[threadIdx.x / TILE];
+    }
     const int itile = __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / TILE) +
                                                      (int)(threadIdx.x / TILE));
     const int i = itile * TILE + lane;
@@ -446,11 +452,16 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_list(const double4 *__restrict_
                                                       unsigned long long *__restrict__ swept)
 {
     __shared__ double4 sj[BLOCK / TILE][TILE];
+    __shared__ double etab_s[BLOCK / TILE][64];
     const int n = c.n;
     // the build kernel (previous launch) has consumed the displacement flag
     if (blockIdx.x == 0 && threadIdx.x == 0 && *maxdisp > thresh_bits) *maxdisp = 0ull;
     const int lane = threadIdx.x & (TILE - 1);
     const int wv = threadIdx.x / TILE;
+    if constexpr (PREC == 0) {
+        load_exp_table(etab_s[wv], c.ec, lane);
+        c.etab = etab_s[wv];
+    }
     const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / TILE) + wv);
     const int t = item / M, m = item % M;
     if (t >= c.ntiles) return;   // whole wave; this kernel has no block barrier
@@ -777,11 +788,16 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
                                                        unsigned long long *__restrict__ swept)
 {
     __shared__ double4 sj[BLOCK / TILE][TILE];
+    __shared__ double etab_s[BLOCK / TILE][64];
     const int n = c.n;
     // the build kernels (previous launches) have consumed the displacement flag
     if (blockIdx.x == 0 && threadIdx.x == 0 && *maxdisp > thresh_bits) *maxdisp = 0ull;
     const int lane = threadIdx.x & (TILE - 1);
     const int wv = threadIdx.x / TILE;
+    if constexpr (PREC == 0) {
+        load_exp_table(etab_s[wv], c.ec, lane);
+        c.etab = etab_s[wv];
+    }
     const int g = lane / CL, gl = lane % CL;
     const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / TILE) + wv);
     const int set = item / M, m = item % M;
