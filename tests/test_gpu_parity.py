@@ -267,9 +267,9 @@ def test_positions_outside_the_box():
 
 
 def test_pairlist_matches_allpairs_over_a_trajectory():
-    """60 steps from a bcc start: the pair lists (rebuilt on the device when a
-    particle has moved skin/2) and the all-pairs kernel give the same
-    trajectory to summation-order rounding."""
+    """60 steps from a bcc start: the pair lists (rebuilt on the device when the
+    two largest displacements since the build add up to the skin) and the
+    all-pairs kernel give the same trajectory to summation-order rounding."""
     w = W.ion_mixture(n=3456, seed=102, start="bcc")
     out = {}
     for ap in (False, True):
@@ -288,3 +288,37 @@ def test_pairlist_matches_allpairs_over_a_trajectory():
     for r1, r2 in zip(ra, rb):
         assert r1["ev"] == pytest.approx(r2["ev"], rel=1e-11)
         assert r1["ek"] == pytest.approx(r2["ek"], rel=1e-10)
+
+
+@pytest.mark.parametrize("path", ["clist", "tilelist"])
+def test_pairlist_valid_with_one_fast_ion(path):
+    """One fast ion (0.45 skin per step) among nearly cold ones: the lists are
+    kept while the two largest displacements since the build add up to at
+    most the skin, so the fast ion uses lists built up to 0.9 skin away.  After
+    every step the forces from the lists match the all-pairs kernel on the
+    same positions to summation-order rounding (a pair missed near the cutoff
+    would change forces far beyond that)."""
+    w = W.ion_mixture(n=3456, seed=103, start="bcc")
+    p = w.params
+    skin = 0.05 * p.rccut
+    v = w.v * 1e-3
+    u = np.array([0.6, 0.5, 0.62])
+    v[17] = u / np.linalg.norm(u) * 0.45 * skin / w.dt
+    lp = p.replace(tile_lists=path == "tilelist")
+    with YukawaSystem(lp, Z=w.Z, A=w.A, dt=w.dt) as s, \
+            YukawaSystem(p.replace(allpairs=True), Z=w.Z, A=w.A) as ref:
+        s.put_state(w.x, v)
+        s.accel(False)
+        x0 = s.get_state()[0]
+        for k in range(12):
+            s.run(1)
+            x, vv, a = s.get_state()
+            ref.put_state(x, vv)
+            ref.accel(False)
+            ar = ref.a
+            np.testing.assert_allclose(a, ar, rtol=1e-10, atol=1e-12 * np.abs(ar).max(),
+                                       err_msg=f"{path} step {k + 1}")
+    L = p.xl[0]
+    d = np.abs(x - x0)
+    d = np.minimum(d, L - d)
+    assert np.sqrt((d[17] ** 2).sum()) > 3 * skin     # the ion did travel

@@ -114,13 +114,13 @@ struct yk_batch {
     std::vector<char> list_used;   // per LIST_UNIT
     int2 *ccounts = nullptr;       // [cap/16][NSEG]
     double4 *xref = nullptr;       // positions at each replica's last list build
-    unsigned long long *rep_maxdisp = nullptr;   // [max_rep] largest displacement^2 (bits)
+    unsigned long long *rep_maxdisp = nullptr;   // [2 max_rep] two largest displacements^2 (bits)
     int nlist = 0;                 // active list replicas
     int span = 0;                  // used extent of the arena (slots)
     int *list_tiles = nullptr;     // tiles of list replicas (build launch)
     int n_list_tiles = 0;
     size_t list_tiles_cap = 0;
-    unsigned long long md_stage = 0;   // host source of an async maxdisp upload
+    unsigned long long md_stage[2] = {0, 0};   // host source of an async maxdisp upload
     int waves = 4;           // waves per workgroup of the persistent launch
     // captured step graph (valid while !dirty)
     hipGraphExec_t gexec = nullptr;
@@ -243,13 +243,14 @@ RepConsts rep_consts(const HostReplica &h)
         c.skin = 0.05 * p.rccut;
         const double rl = p.rccut + c.skin + c.margin;
         c.rl2 = rl * rl;
-        c.thresh2 = 0.25 * c.skin * c.skin;
         for (int k = 0; k < 3; ++k) {
             c.Lf[k] = (float)c.L[k];
             c.invLf[k] = (float)c.invL[k];
             c.halfLf_m[k] = (float)(c.halfL_m[k] - 1e-5 * p.xl[k]);
         }
-        c.rl2f = (float)(c.rl2 * (1.0 + 1e-5));
+        const double lm = std::max(p.xl[0], std::max(p.xl[1], p.xl[2]));
+        const double rlf = rl * (1.0 + 5e-6) + 1e-6 * lm;   // as the single-system bound
+        c.rl2f = (float)(rlf * rlf);
         c.skinf = (float)c.skin;
     }
     return c;
@@ -686,7 +687,7 @@ int yk_batch_create(int64_t capacity, int32_t max_replicas, int32_t max_records,
     YB_HIPB(hipMalloc(&b->queue, sizeof(unsigned)));
     YB_HIPB(hipMalloc(&b->xref, sizeof(double4) * nn));
     YB_HIPB(hipMalloc(&b->ccounts, sizeof(int2) * (nn / CL) * NSEG));
-    YB_HIPB(hipMalloc(&b->rep_maxdisp, sizeof(unsigned long long) * (size_t)max_replicas));
+    YB_HIPB(hipMalloc(&b->rep_maxdisp, 2 * sizeof(unsigned long long) * (size_t)max_replicas));
     YB_HIPB(hipMalloc(&b->keys, sizeof(unsigned long long) * nn));
     YB_HIPB(hipMalloc(&b->keys2, sizeof(unsigned long long) * nn));
     YB_HIPB(hipMalloc(&b->vals, sizeof(int) * nn));
@@ -854,8 +855,9 @@ int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *
     const double inf = __builtin_inf();   // lists to be built at the accel pass
     unsigned long long infbits;
     std::memcpy(&infbits, &inf, sizeof infbits);
-    b->md_stage = infbits;
-    YB_HIP(hipMemcpyAsync(b->rep_maxdisp + id, &b->md_stage, sizeof infbits,
+    b->md_stage[0] = infbits;
+    b->md_stage[1] = 0;
+    YB_HIP(hipMemcpyAsync(b->rep_maxdisp + 2 * id, b->md_stage, sizeof b->md_stage,
                           hipMemcpyHostToDevice, b->stream));
     YB_HIP(hipStreamSynchronize(b->stream));
     for (int s = start; s < start + nblk; ++s) b->used[s] = 1;

@@ -31,11 +31,11 @@ struct RepConsts {
     int tile0, ntiles, blk0, nblk;
     int ref_f1, grid_bits;   // grid_bits: Morton cells per axis = 2^grid_bits
     // cluster lists (list_items > 0): the replica's region of the list arena
-    // (entries), segment capacity, list radius^2, skin, rebuild threshold
-    // (skin/2)^2 and the FP32 bounds of the candidate test (as ListConsts)
+    // (entries), segment capacity, list radius^2, skin and the FP32 bounds of
+    // the candidate test (as ListConsts)
     long long lists_off;
     int list_items, segcap;
-    double rl2, skin, thresh2;
+    double rl2, skin;
     float Lf[3], invLf[3], halfLf_m[3], rl2f, skinf;
 };
 
@@ -402,19 +402,15 @@ __global__ __launch_bounds__(BLOCK) void k_batch_drift(double4 *__restrict__ pos
                 const double dx = x[0] - q.x, dy = x[1] - q.y, dz = x[2] - q.z;
                 d2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
             }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double o = __shfl_xor(d2, off, 64);
-                d2 = o > d2 ? o : d2;
-            }
-            if ((threadIdx.x & (TILE - 1)) == 0 && d2 > 0.0) atomicMax(&maxdisp[tr], as_bits(d2));
+            record_disp(maxdisp + 2 * tr, d2);
         }
     }
 }
 
 // Cluster lists of the list replicas (grid: arena tiles x NSEG segments),
-// rebuilt for a replica whose largest displacement since its last build
-// exceeds skin/2 (set to +inf at admission and after a re-sort).  Same
+// rebuilt for a replica whose lists are stale (lists_stale over its two
+// largest displacements since its last build; +inf at admission and after a
+// re-sort).  Same
 // scheme as k_clist_build with the replica's slot range, constants and list
 // region; entries hold slots relative to the replica's first slot.
 __global__ __launch_bounds__(BLOCK) void k_batch_clist_build(
@@ -428,13 +424,16 @@ __global__ __launch_bounds__(BLOCK) void k_batch_clist_build(
     const int r = tile_rep[t];
     if (r < 0) return;
     const RepConsts &rc = reps[r];
-    if (rc.list_items <= 0 || maxdisp[r] <= as_bits(rc.thresh2)) return;   // uniform
+    if (rc.list_items <= 0 || !lists_stale(maxdisp + 2 * r, rc.skin)) return;   // uniform
     constexpr int W = BLOCK / TILE;
-    __shared__ double4 stage[2][W][TILE];
+    __shared__ float4 stage[2][W][TILE];
     __shared__ int ctiles[CLIST_MAX_N / TILE / NSEG + 1];
     const int lane = threadIdx.x & (TILE - 1), wv = threadIdx.x / TILE;
     const int cll = (t - rc.tile0) * CPW + wv;   // cluster within the replica
     const bool cvalid = cll * CL < rc.n;
+    double tc[3], th[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { tc[k] = box[t].c[k]; th[k] = box[t].h[k]; }
     double c[3], h[3];
     {
         const int il = cll * CL + lane;
@@ -451,15 +450,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_clist_build(
         }
     }
     const bool live = cvalid && h[0] >= 0.0;
-    float hf[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) hf[k] = live ? (float)h[k] * 1.000001f : 0.0f;
+    const ClusterView cv = cluster_view(c, h, live, tc, rc.Lf, rc.halfLf_m, rc.skinf, rc.rl2f);
     int ncand = 0;
     {
         const double rl = __builtin_sqrt(rc.rl2);
-        double tc[3], th[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) { tc[k] = box[t].c[k]; th[k] = box[t].h[k]; }
         const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
         for (int tb = 0; tb < rc.ntiles; tb += TILE * NSEG) {
             const int stl = tb + lane * NSEG + seg;   // source tile within the replica
@@ -483,53 +477,11 @@ __global__ __launch_bounds__(BLOCK) void k_batch_clist_build(
         }
     }
     __syncthreads();
-    unsigned *dst = clist + rc.lists_off + ((size_t)(cvalid ? cll : 0) * NSEG + seg) * rc.segcap;
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    auto load_round = [&](int q) -> double4 {
-        const int idx = q * W + wv;
-        const int jl = (idx < ncand) ? ctiles[idx] * TILE + lane : rc.n;
-        return (jl < rc.n) ? pos[rc.off + jl] : make_double4(0, 0, 0, 0);
-    };
-    const int nrounds = (ncand + W - 1) / W;
-    double4 pa = load_round(0), pb = load_round(1);
-    int nf = 0, ng = 0;
-    for (int q = 0; q < nrounds; ++q) {
-        stage[q & 1][wv][lane] = pa;
-        pa = pb;
-        pb = load_round(q + 2);
-        __syncthreads();
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            const int idx = q * W + w;
-            const int jl = (idx < ncand) ? ctiles[idx] * TILE + lane : rc.n;
-            bool cand = false, fast = true;
-            unsigned code = 0;
-            if (live && jl < rc.n) {
-                const double4 pj = stage[q & 1][w][lane];
-                const double xj[3] = {pj.x, pj.y, pj.z};
-                float d2 = 0.0f;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const float cd = (float)(c[k] - xj[k]);
-                    const float m = __builtin_rintf(cd * rc.invLf[k]);
-                    const float ev = __builtin_fabsf(__builtin_fmaf(-rc.Lf[k], m, cd));
-                    float gk = ev - hf[k];
-                    gk = gk > 0.0f ? gk : 0.0f;
-                    d2 = __builtin_fmaf(gk, gk, d2);
-                    const bool m_ok = (m >= -1.0f) && (m <= 1.0f);
-                    fast = fast && m_ok && (ev + hf[k] + rc.skinf < rc.halfLf_m[k]);
-                    code |= (unsigned)(m_ok ? (int)m + 1 : 1) << (26 + 2 * k);
-                }
-                cand = d2 <= rc.rl2f;
-            }
-            const unsigned long long bf = __ballot(cand && fast);
-            const unsigned long long bg = __ballot(cand && !fast);
-            if (cand && fast) dst[nf + __popcll(bf & lt)] = (unsigned)jl | code;
-            if (cand && !fast) dst[rc.segcap - 1 - (ng + __popcll(bg & lt))] = (unsigned)jl | code;
-            nf += __popcll(bf);
-            ng += __popcll(bg);
-        }
-    }
+    int nf, ng;
+    clist_scan<W>(pos + rc.off, rc.n, ctiles, ncand, cv, live, tc, rc.L, rc.invL, rc.Lf, rc.invLf,
+                  rc.rl2f, stage, clist,
+                  rc.lists_off + ((size_t)(cvalid ? cll : 0) * NSEG + seg) * rc.segcap, rc.segcap,
+                  lane, wv, nf, ng);
     if (cvalid && lane == 0) ccounts[(t * CPW + wv) * NSEG + seg] = make_int2(nf, ng);
 }
 
@@ -576,7 +528,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_sort_keys(double4 *__restrict__
     unsigned long long k = (unsigned long long)i << 32;
     if (r >= 0 && due[r]) {
         const RepConsts &rc = reps[r];
-        if (i == rc.off) maxdisp[r] = as_bits(__builtin_inf());   // slots move: rebuild lists
+        if (i == rc.off) maxdisp[2 * r] = as_bits(__builtin_inf());   // slots move: rebuild lists
         const int g = rc.grid_bits;
         const int cmax = (1 << g) - 1;
         const double4 p = pos[i];
@@ -686,9 +638,11 @@ __global__ __launch_bounds__(BLOCK) void k_batch_replica(const double *__restric
     if (r >= nrep) return;
     const RepConsts &rc = reps[r];
     // the list build of this step has consumed the displacement flag
-    if (threadIdx.x == 0 && rc.list_items > 0 && maxdisp[r] > as_bits(rc.thresh2) &&
-        (MODE == 0 ? state[r].fresh != 0 : state[r].steps_left > 0))
-        maxdisp[r] = 0ull;
+    if (threadIdx.x == 0 && rc.list_items > 0 && lists_stale(maxdisp + 2 * r, rc.skin) &&
+        (MODE == 0 ? state[r].fresh != 0 : state[r].steps_left > 0)) {
+        maxdisp[2 * r] = 0ull;
+        maxdisp[2 * r + 1] = 0ull;
+    }
     const bool live = rc.n > 0 && (MODE == 0 ? state[r].fresh != 0 : state[r].steps_left > 0);
     if constexpr (MEAS) {
         double q[NQ];

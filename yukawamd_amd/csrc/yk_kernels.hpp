@@ -339,22 +339,23 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const double4 *__restrict__ pos
 // radius rl = cutoff + skin.  An entry is the slot index (bits 0-25) and, per
 // axis, the periodic image of the source relative to the tile centre
 // (bits 26+2k: 0 = -L, 1 = 0, 2 = +L).  "Fast" entries -- every target of the
-// tile sees the source through that one image as long as no particle has
-// moved more than skin/2 -- are stored from the front of the tile's region;
-// the rest ("general", minimum image per pair) from the back.  The order is
-// deterministic (by slot).  A list stays valid while the largest displacement
-// since its build is below skin/2; the drift kernel tracks that maximum and
-// the build kernel runs only when it is exceeded (or after a re-sort).
+// tile sees the source through that one image as long as no pair's
+// separation has changed by more than the skin -- are stored from the front
+// of the tile's region; the rest ("general", minimum image per pair) from the
+// back.  The order is deterministic (by slot).  A list stays valid while the
+// two largest displacements since its build add up to at most the skin
+// (lists_stale); the drift kernel tracks them and the build kernel runs only
+// when that bound is exceeded (or after a re-sort).
 struct ListConsts {
     double L[3], invL[3], halfL_m[3];
     double rl2;          // (cutoff + skin)^2
     double skin;
-    double thresh2;      // (skin/2)^2: rebuild when a displacement exceeds it
     int n, ntiles, cap;  // cap = entries per tile region (= n)
     int pad;
     // FP32 copies for the cluster-list candidate test, widened so that the
     // float test keeps every candidate and only calls an entry fast when the
-    // double test would: rl2f = rl^2 (1 + 1e-5); halfLf_m = halfL_m - 1e-5 L
+    // double test would: rl2f = (rl (1 + 5e-6) + 1e-6 max L)^2 (the float
+    // offsets are within a few ulp of L); halfLf_m = halfL_m - 1e-5 L
     float Lf[3], invLf[3], halfLf_m[3], rl2f, skinf;
 };
 
@@ -363,6 +364,54 @@ constexpr unsigned SLOT_MASK = (1u << 26) - 1u;
 __device__ __forceinline__ unsigned long long as_bits(double v)
 {
     return (unsigned long long)__double_as_longlong(v);
+}
+
+// Pair-list validity.  The lists hold every pair within cutoff + skin at
+// their build; since then a pair's separation has changed by at most
+// |d_i| + |d_j| <= d1 + d2, the two largest displacements of distinct
+// particles since the build, so the lists stay valid while d1 + d2 <= skin.
+// md[0] and md[1] hold d1^2 and d2^2 as the bit patterns of non-negative
+// doubles (whose integer order is their numeric order); md[0] = +inf forces
+// a rebuild.  The sum is formed in FP32 and enlarged by 1e-5, far more than
+// its rounding, so the test can only call lists stale early, never late
+// (every kernel uses this one test, so they agree).
+__device__ __forceinline__ bool lists_stale(const unsigned long long *md, double skin)
+{
+    // hardware square roots (within a few ulp)
+    const float s = __builtin_amdgcn_sqrtf((float)__longlong_as_double((long long)md[0])) +
+                    __builtin_amdgcn_sqrtf((float)__longlong_as_double((long long)md[1]));
+    return !(s * 1.00001f <= (float)skin);
+}
+
+// Adds one wave's squared displacements (d2 >= 0 per lane) to md.  The wave's
+// largest value m1 enters md[0], and whichever of m1 and md[0]'s previous
+// value is smaller enters md[1], as does the wave's second-largest value m2
+// (m2 <= m1 <= md[0] by then, so it never competes for md[0]).  Every value
+// but one copy of the final maximum thus reaches md[1], which ends as the
+// second-largest displacement.  Called by all 64 lanes of the wave.
+__device__ __forceinline__ void record_disp(unsigned long long *md, double d2)
+{
+    double m1 = d2;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m1, off, 64);
+        m1 = o > m1 ? o : m1;
+    }
+    const int lane = threadIdx.x & 63;
+    const unsigned long long at = __ballot(d2 == m1);
+    double m2 = (at != 0ull && lane == __ffsll((unsigned long long)at) - 1) ? 0.0 : d2;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m2, off, 64);
+        m2 = o > m2 ? o : m2;
+    }
+    if (lane == 0 && m1 > 0.0) {
+        const unsigned long long b1 = as_bits(m1);
+        const unsigned long long old = atomicMax(&md[0], b1);
+        const unsigned long long lose = old < b1 ? old : b1;
+        if (lose) atomicMax(&md[1], lose);
+        if (m2 > 0.0) atomicMax(&md[1], as_bits(m2));
+    }
 }
 
 // One workgroup per target tile; sources scanned in slot order, 256 at a time.
@@ -374,7 +423,7 @@ static __global__ __launch_bounds__(BLOCK) void k_list_build(const double4 *__re
                                                       int2 *__restrict__ counts,
                                                       double4 *__restrict__ xref)
 {
-    if (*maxdisp <= as_bits(lc.thresh2)) return;   // list still valid (uniform)
+    if (!lists_stale(maxdisp, lc.skin)) return;   // list still valid (uniform)
     const int t = blockIdx.x;
     const int lane = threadIdx.x & (TILE - 1), wv = threadIdx.x / TILE;
     __shared__ int wf[BLOCK / TILE], wg[BLOCK / TILE];
@@ -448,14 +497,17 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_list(const double4 *__restrict_
                                                       ForceConsts c, int cap, int M,
                                                       double *__restrict__ part,
                                                       unsigned long long *__restrict__ maxdisp,
-                                                      unsigned long long thresh_bits,
+                                                      double skin,
                                                       unsigned long long *__restrict__ swept)
 {
     __shared__ double4 sj[BLOCK / TILE][TILE];
     __shared__ double etab_s[BLOCK / TILE][64];
     const int n = c.n;
     // the build kernel (previous launch) has consumed the displacement flag
-    if (blockIdx.x == 0 && threadIdx.x == 0 && *maxdisp > thresh_bits) *maxdisp = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && lists_stale(maxdisp, skin)) {
+        maxdisp[0] = 0ull;
+        maxdisp[1] = 0ull;
+    }
     const int lane = threadIdx.x & (TILE - 1);
     const int wv = threadIdx.x / TILE;
     if constexpr (PREC == 0) {
@@ -590,8 +642,8 @@ static __global__ __launch_bounds__(BLOCK) void k_tile_boxes(const double4 *__re
 // the tile lists (slot | per-axis image code).  A 16-particle cluster's AABB
 // is much smaller than a 64-particle tile's: about 30 % fewer candidate
 // pairs.  Fast entries (one image for every target of the cluster while no
-// particle has moved more than skin/2) fill the cluster's region from the
-// front, general ones from the back, both in slot order.
+// pair's separation has changed by more than the skin) fill the cluster's
+// region from the front, general ones from the back, both in slot order.
 constexpr int CL = 16;              // slots per cluster
 constexpr int CPW = TILE / CL;      // clusters per wave in the pair kernel
 constexpr int CLIST_MAX_N = 65536;  // above this the tile lists are used
@@ -602,28 +654,235 @@ constexpr int CLIST_MAX_N = 65536;  // above this the tile lists are used
 // order.
 constexpr int NSEG = 4;
 
+// Candidate test of the cluster-list builds.  A workgroup stages every
+// candidate source once: its offset from the target tile's centre, reduced to
+// the minimum image in FP64 and rounded to float, and in .w the entry's
+// per-axis image code for the shift that reduction applied (bits 26-31; the
+// source is seen at x + (code - 1) L), bit 0 set when a shift lies beyond one
+// box (such a source gives general entries only).  An absent source (a slot
+// beyond the system) is staged FAR_AWAY on every axis with bit 0 set.
+constexpr unsigned CODE_MASK = ~SLOT_MASK;
+constexpr float FAR_AWAY = 3.0e38f;
+
+__device__ __forceinline__ float4 clist_stage(const double4 &p, bool present, const double tc[3],
+                                              const double L[3], const double invL[3])
+{
+    const double x[3] = {p.x, p.y, p.z};
+    float o[3];
+    unsigned w = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double d = x[k] - tc[k];
+        const double m = __builtin_rint(d * invL[k]);
+        o[k] = present ? (float)__builtin_fma(-L[k], m, d) : FAR_AWAY;
+        const bool ok = __builtin_fabs(m) <= 1.0;
+        w |= (unsigned)(ok ? 1 - (int)m : 1) << (26 + 2 * k);
+        w |= ok ? 0u : 1u;
+    }
+    return make_float4(o[0], o[1], o[2], __uint_as_float(present ? w : 1u));
+}
+
+// A cluster as the candidate test sees it: centre offset from the tile centre
+// (cr), widened half extents (hf), the largest |offset| per axis at which the
+// staged image still serves every target for the list's lifetime
+// (lim = L/2 - margin - hf - skin), and a wave-uniform tier:
+//   2  |cr| + hf + list radius < L/2 on every axis (with margin): a source
+//      whose offset d from the cluster centre exceeds L/2 (its nearer image
+//      is at L - |d|) is beyond the list radius either way, so |d| serves
+//      as the distance;
+//   1  |cr| < L/4: every |d| < L, the minimum image distance is
+//      min(|d|, L - |d|);
+//   0  otherwise (a cluster far from its tile's centre): full minimum image,
+//      general entries only.
+struct ClusterView {
+    float cr[3], hf[3], lim[3];
+    int tier;
+};
+
+__device__ __forceinline__ ClusterView cluster_view(const double c[3], const double h[3],
+                                                    bool live, const double tc[3],
+                                                    const float Lf[3], const float halfLf_m[3],
+                                                    float skinf, float rl2f)
+{
+    ClusterView v;
+    const float rlf = __builtin_sqrtf(rl2f);
+    bool t2 = live, t1 = live;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        // half extents in float, widened by 1e-6 (more than one rounding), so
+        // the float box contains the double one
+        v.hf[k] = live ? (float)h[k] * 1.000001f : 0.0f;
+        v.cr[k] = live ? (float)(c[k] - tc[k]) : 0.0f;
+        v.lim[k] = halfLf_m[k] - v.hf[k] - skinf;
+        const float acr = __builtin_fabsf(v.cr[k]);
+        t2 = t2 && acr + v.hf[k] + rlf + 1e-5f * Lf[k] < 0.5f * Lf[k];
+        t1 = t1 && acr < 0.25f * Lf[k];
+    }
+    v.tier = __builtin_amdgcn_readfirstlane(t2 ? 2 : t1 ? 1 : 0);   // wave-uniform
+    return v;
+}
+
+// The candidate test of the staged source s against the cluster box (FP32
+// with the widened bounds of ListConsts): cand, the source is within the list
+// radius; fast, its staged image serves every target of the cluster for the
+// list's lifetime; and the same as this wave's lane masks mcand, mfast.  Each
+// mask is the ballot of a single compare (which compiles to the compare's own
+// result), combined with scalar ANDs.  Absent sources fail at tiers 1 and 2
+// (FAR_AWAY); tier 0 callers test presence.
+template <int TIER>
+__device__ __forceinline__ void clist_test(const ClusterView &v, const float4 &s, const float Lf[3],
+                                           const float invLf[3], float rl2f, bool &cand,
+                                           bool &fast, unsigned long long &mcand,
+                                           unsigned long long &mfast)
+{
+    const float sx[3] = {s.x, s.y, s.z};
+    float d2 = 0.0f;
+    const bool wok = (__float_as_uint(s.w) & 1u) == 0u;
+    fast = TIER > 0 && wok;
+    mfast = TIER > 0 ? __builtin_amdgcn_ballot_w64(wok) : 0ull;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float cd = v.cr[k] - sx[k];
+        const float a = __builtin_fabsf(cd);
+        float ev;
+        if constexpr (TIER == 2) {
+            ev = a;
+        } else if constexpr (TIER == 1) {
+            ev = __builtin_fminf(a, __builtin_fabsf(Lf[k] - a));
+        } else {
+            ev = __builtin_fabsf(__builtin_fmaf(-Lf[k], __builtin_rintf(cd * invLf[k]), cd));
+        }
+        if constexpr (TIER > 0) {
+            const bool in = a < v.lim[k];
+            fast = fast & in;
+            mfast &= __builtin_amdgcn_ballot_w64(in);
+        }
+        float gk = ev - v.hf[k];
+        gk = gk > 0.0f ? gk : 0.0f;
+        d2 = __builtin_fmaf(gk, gk, d2);
+    }
+    cand = d2 <= rl2f;
+    mcand = __builtin_amdgcn_ballot_w64(cand);
+}
+
+// Lanes below this one whose bit is set in m.
+__device__ __forceinline__ unsigned lanes_below(unsigned long long m)
+{
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// One round of a cluster-list build for one wave: its cluster against the W
+// staged source tiles of the round (tile slots base[w] + lane; slots at or
+// beyond n are absent), entries appended to the segment at clist + seg_off
+// (fast from the front, general from the back of its segcap entries; seg_off
+// is wave-uniform).  The entry is the slot with the staged image code, which
+// the pair kernels apply to fast entries only (general ones take the per-pair
+// minimum image).
+template <int TIER, int W>
+__device__ __forceinline__ void clist_round(const ClusterView &v, const float4 (*st)[TILE],
+                                            const int *base, int n, bool live, int lane,
+                                            const float Lf[3], const float invLf[3], float rl2f,
+                                            unsigned *__restrict__ clist, size_t seg_off,
+                                            unsigned segcap, unsigned &nf, unsigned &ng)
+{
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const int j = base[w] + lane;
+        const float4 sv = st[w][lane];
+        bool cand, fast;
+        unsigned long long mcand, mfast;
+        clist_test<TIER>(v, sv, Lf, invLf, rl2f, cand, fast, mcand, mfast);
+        if constexpr (TIER == 0) {
+            const bool present = j < n;
+            cand = cand & live & present;
+            mcand &= live ? __builtin_amdgcn_ballot_w64(present) : 0ull;
+        }
+        const unsigned long long bf = mcand & mfast, bg = mcand & ~mfast;
+        if (cand) {
+            const unsigned at = fast ? nf + lanes_below(bf) : segcap - 1u - (ng + lanes_below(bg));
+            clist[seg_off + at] = (unsigned)j | (__float_as_uint(sv.w) & CODE_MASK);
+        }
+        nf += (unsigned)__popcll(bf);
+        ng += (unsigned)__popcll(bg);
+    }
+}
+
+// The scan of a cluster-list build for one wave: its cluster against every
+// candidate source tile (ctiles[0..ncand), slots tile * 64 + lane of pos,
+// those at or beyond n absent), the workgroup staging W tiles per round as
+// clist_stage offsets (double buffered, the loads of the next two rounds in
+// flight).  nf, ng: the fast and general entries written to the segment at dst.
+template <int W>
+__device__ __forceinline__ void clist_scan(const double4 *__restrict__ pos, int n,
+                                           const int *ctiles, int ncand, const ClusterView &cv,
+                                           bool live, const double tc[3], const double L[3],
+                                           const double invL[3], const float Lf[3],
+                                           const float invLf[3], float rl2f,
+                                           float4 (*stage)[W][TILE], unsigned *__restrict__ clist,
+                                           size_t seg_off, int segcap, int lane, int wv, int &nf,
+                                           int &ng)
+{
+    auto slot_of = [&](int r) -> int {
+        const int idx = r * W + wv;
+        return (idx < ncand) ? ctiles[idx] * TILE + lane : n;
+    };
+    auto load = [&](int j) -> double4 { return (j < n) ? pos[j] : make_double4(0, 0, 0, 0); };
+    const int nrounds = (ncand + W - 1) / W;
+    // wave-uniform segment offset (scalar registers)
+    seg_off = ((size_t)__builtin_amdgcn_readfirstlane((unsigned)(seg_off >> 32)) << 32) |
+              __builtin_amdgcn_readfirstlane((unsigned)seg_off);
+    int ja = slot_of(0), jb = slot_of(1);
+    double4 pa = load(ja), pb = load(jb);
+    unsigned uf = 0, ug = 0;
+    for (int r = 0; r < nrounds; ++r) {
+        stage[r & 1][wv][lane] = clist_stage(pa, ja < n, tc, L, invL);
+        pa = pb;
+        ja = jb;
+        jb = slot_of(r + 2);
+        pb = load(jb);
+        __syncthreads();
+        int base[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) base[w] = (r * W + w < ncand) ? ctiles[r * W + w] * TILE : n;
+        if (cv.tier == 2)
+            clist_round<2, W>(cv, stage[r & 1], base, n, live, lane, Lf, invLf, rl2f, clist,
+                              seg_off, (unsigned)segcap, uf, ug);
+        else if (cv.tier == 1)
+            clist_round<1, W>(cv, stage[r & 1], base, n, live, lane, Lf, invLf, rl2f, clist,
+                              seg_off, (unsigned)segcap, uf, ug);
+        else
+            clist_round<0, W>(cv, stage[r & 1], base, n, live, lane, Lf, invLf, rl2f, clist,
+                              seg_off, (unsigned)segcap, uf, ug);
+    }
+    nf = (int)uf;
+    ng = (int)ug;
+}
+
 // One workgroup per tile, wave w builds the list of the tile's cluster w.
 // Source tiles are culled by the minimum-image gap between their AABB
 // (current positions, from k_drift / k_tile_boxes) and the tile's; the four
 // waves then load the surviving source tiles four at a time into LDS (double
 // buffered, the loads of the next two rounds in flight) and each tests its
 // own cluster against every staged slot, so one read of a source tile serves
-// the tile's four clusters.  Also records the positions the list was built
-// from (xref).
+// the tile's four clusters (staged once per workgroup as clist_stage
+// offsets).  Also records the positions the list was built from (xref).
 static __global__ __launch_bounds__(BLOCK) void k_clist_build(
     const double4 *__restrict__ pos, const TileBox *__restrict__ box, ListConsts lc,
     int segcap, const unsigned long long *__restrict__ maxdisp, unsigned *__restrict__ clist,
     int2 *__restrict__ ccounts, double4 *__restrict__ xref)
 {
-    if (*maxdisp <= as_bits(lc.thresh2)) return;   // lists still valid (uniform)
+    if (!lists_stale(maxdisp, lc.skin)) return;   // lists still valid (uniform)
     constexpr int W = BLOCK / TILE;
-    __shared__ double4 stage[2][W][TILE];
+    __shared__ float4 stage[2][W][TILE];
     __shared__ int ctiles[CLIST_MAX_N / TILE / NSEG + 1];
     const int t = blockIdx.x;
     const int seg = blockIdx.y;   // source tiles st with st % NSEG == seg
     const int lane = threadIdx.x & (TILE - 1), wv = threadIdx.x / TILE;
     const int cl = t * CPW + wv;
     const bool cvalid = cl * CL < lc.n;
+    double tc[3], th[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { tc[k] = box[t].c[k]; th[k] = box[t].h[k]; }
     // cluster AABB (lanes 0..15 hold its slots)
     double c[3], h[3];
     {
@@ -641,18 +900,11 @@ static __global__ __launch_bounds__(BLOCK) void k_clist_build(
         }
     }
     const bool live = cvalid && h[0] >= 0.0;   // the cluster holds a particle
-    // half extents in float, widened by 1e-6 (more than one rounding), so the
-    // float box contains the double one
-    float hf[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) hf[k] = live ? (float)h[k] * 1.000001f : 0.0f;
+    const ClusterView cv = cluster_view(c, h, live, tc, lc.Lf, lc.halfLf_m, lc.skinf, lc.rl2f);
     // candidate source tiles of the whole tile, in ascending order
     int ncand = 0;
     {
         const double rl = __builtin_sqrt(lc.rl2);
-        double tc[3], th[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) { tc[k] = box[t].c[k]; th[k] = box[t].h[k]; }
         const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
         for (int tb = 0; tb < lc.ntiles; tb += TILE * NSEG) {
             const int st = tb + lane * NSEG + seg;
@@ -675,57 +927,10 @@ static __global__ __launch_bounds__(BLOCK) void k_clist_build(
         }
     }
     __syncthreads();
-    unsigned *dst = clist + ((size_t)(cvalid ? cl : 0) * NSEG + seg) * segcap;
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    auto load_round = [&](int r) -> double4 {
-        const int idx = r * W + wv;
-        const int j = (idx < ncand) ? ctiles[idx] * TILE + lane : lc.n;
-        return (j < lc.n) ? pos[j] : make_double4(0, 0, 0, 0);
-    };
-    const int nrounds = (ncand + W - 1) / W;
-    double4 pa = load_round(0), pb = load_round(1);
-    int nf = 0, ng = 0;
-    for (int r = 0; r < nrounds; ++r) {
-        stage[r & 1][wv][lane] = pa;
-        pa = pb;
-        pb = load_round(r + 2);
-        __syncthreads();
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            const int idx = r * W + w;
-            const int j = (idx < ncand) ? ctiles[idx] * TILE + lane : lc.n;
-            bool cand = false, fast = true;
-            unsigned code = 0;
-            if (live && j < lc.n) {
-                // offset from the cluster centre formed in FP64, tested in
-                // FP32 with the widened bounds of ListConsts
-                const double4 pj = stage[r & 1][w][lane];
-                const double xj[3] = {pj.x, pj.y, pj.z};
-                float d2 = 0.0f;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const float cd = (float)(c[k] - xj[k]);
-                    const float m = __builtin_rintf(cd * lc.invLf[k]);
-                    const float ev = __builtin_fabsf(__builtin_fmaf(-lc.Lf[k], m, cd));
-                    float gk = ev - hf[k];
-                    gk = gk > 0.0f ? gk : 0.0f;
-                    d2 = __builtin_fmaf(gk, gk, d2);
-                    // positions outside [0, L) can give |m| > 1: those take the
-                    // per-pair minimum image and keep a zero image code
-                    const bool m_ok = (m >= -1.0f) && (m <= 1.0f);
-                    fast = fast && m_ok && (ev + hf[k] + lc.skinf < lc.halfLf_m[k]);
-                    code |= (unsigned)(m_ok ? (int)m + 1 : 1) << (26 + 2 * k);
-                }
-                cand = d2 <= lc.rl2f;
-            }
-            const unsigned long long bf = __ballot(cand && fast);
-            const unsigned long long bg = __ballot(cand && !fast);
-            if (cand && fast) dst[nf + __popcll(bf & lt)] = (unsigned)j | code;
-            if (cand && !fast) dst[segcap - 1 - (ng + __popcll(bg & lt))] = (unsigned)j | code;
-            nf += __popcll(bf);
-            ng += __popcll(bg);
-        }
-    }
+    int nf, ng;
+    clist_scan<W>(pos, lc.n, ctiles, ncand, cv, live, tc, lc.L, lc.invL, lc.Lf, lc.invLf, lc.rl2f,
+                  stage, clist, ((size_t)(cvalid ? cl : 0) * NSEG + seg) * segcap, segcap, lane,
+                  wv, nf, ng);
     if (cvalid && lane == 0) ccounts[cl * NSEG + seg] = make_int2(nf, ng);
 }
 
@@ -738,7 +943,7 @@ static __global__ __launch_bounds__(CORDER_THREADS) void k_corder(
     const int2 *__restrict__ ccounts, int ncl, ListConsts lc,
     const unsigned long long *__restrict__ maxdisp, int *__restrict__ order)
 {
-    if (*maxdisp <= as_bits(lc.thresh2)) return;
+    if (!lists_stale(maxdisp, lc.skin)) return;
     __shared__ unsigned key[CORDER_MAX];
     int np = 1;
     while (np < ncl) np <<= 1;
@@ -784,14 +989,17 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
                                                        ForceConsts c, int segcap, int ncl, int M,
                                                        double *__restrict__ part,
                                                        unsigned long long *__restrict__ maxdisp,
-                                                       unsigned long long thresh_bits,
+                                                       double skin,
                                                        unsigned long long *__restrict__ swept)
 {
     __shared__ double4 sj[BLOCK / TILE][TILE];
     __shared__ double etab_s[BLOCK / TILE][64];
     const int n = c.n;
     // the build kernels (previous launches) have consumed the displacement flag
-    if (blockIdx.x == 0 && threadIdx.x == 0 && *maxdisp > thresh_bits) *maxdisp = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && lists_stale(maxdisp, skin)) {
+        maxdisp[0] = 0ull;
+        maxdisp[1] = 0ull;
+    }
     const int lane = threadIdx.x & (TILE - 1);
     const int wv = threadIdx.x / TILE;
     if constexpr (PREC == 0) {
@@ -1165,12 +1373,7 @@ __global__ __launch_bounds__(BLOCK) void k_drift(double4 *__restrict__ pos,
                 d2 = __builtin_fma(d, d, d2);
             }
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            double o = __shfl_xor(d2, off, 64);
-            d2 = o > d2 ? o : d2;
-        }
-        if ((threadIdx.x & (TILE - 1)) == 0 && d2 > 0.0) atomicMax(maxdisp, as_bits(d2));
+        record_disp(maxdisp, d2);
     }
 }
 

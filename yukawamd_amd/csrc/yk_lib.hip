@@ -7,6 +7,7 @@
 // structure-of-arrays happen only in yk_put_state / yk_get_state.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -78,7 +79,7 @@ struct yk_system {
     int segcap = 0;                          // entries per cluster-list segment
     bool corder_stale = true;                // slots moved: order to be recomputed
     double4 *xref = nullptr;                 // positions at the last build
-    unsigned long long *maxdisp = nullptr;   // max displacement^2 since build (bits)
+    unsigned long long *maxdisp = nullptr;   // [2] two largest displacements^2 since build (bits)
     unsigned long long *swept = nullptr;     // directed pairs evaluated (counter)
     unsigned long long swept_base = 0;
     double *part = nullptr;        // [nchunks][11][n] or [ntiles][M][11][64]
@@ -237,7 +238,6 @@ ListConsts list_consts(const yk_system *s)
     double rl = list_cutoff(p) + s->skin + margin;
     l.rl2 = rl * rl;
     l.skin = s->skin;
-    l.thresh2 = 0.25 * s->skin * s->skin;
     l.n = s->n;
     l.ntiles = s->ntiles;
     l.cap = s->n;
@@ -246,7 +246,11 @@ ListConsts list_consts(const yk_system *s)
         l.invLf[k] = (float)l.invL[k];
         l.halfLf_m[k] = (float)(l.halfL_m[k] - 1e-5 * p.xl[k]);
     }
-    l.rl2f = (float)(l.rl2 * (1.0 + 1e-5));
+    {   // float offsets carry errors of a few ulp of L: widen absolutely too
+        const double lm = std::max(p.xl[0], std::max(p.xl[1], p.xl[2]));
+        const double rlf = rl * (1.0 + 5e-6) + 1e-6 * lm;
+        l.rl2f = (float)(rlf * rlf);
+    }
     l.skinf = (float)s->skin;
     return l;
 }
@@ -390,32 +394,30 @@ void launch_pairs_k(yk_system *s, const ForceConsts &c, bool meas)
 {
     if (s->use_clist) {
         const ListConsts lc = list_consts(s);
-        const unsigned long long tb = bits_of(lc.thresh2);
         const int items = (s->ncl + CPW - 1) / CPW * s->list_M;
         dim3 grid((items + BLOCK / TILE - 1) / (BLOCK / TILE));
         if (meas)
             hipLaunchKernelGGL((k_pairs_clist<KIND, PREC, true>), grid, dim3(BLOCK), 0,
                                s->stream, s->pos, s->clist, s->ccounts, s->corder, c,
-                               s->segcap, s->ncl, s->list_M, s->part, s->maxdisp, tb, s->swept);
+                               s->segcap, s->ncl, s->list_M, s->part, s->maxdisp, lc.skin, s->swept);
         else
             hipLaunchKernelGGL((k_pairs_clist<KIND, PREC, false>), grid, dim3(BLOCK), 0,
                                s->stream, s->pos, s->clist, s->ccounts, s->corder, c,
-                               s->segcap, s->ncl, s->list_M, s->part, s->maxdisp, tb, s->swept);
+                               s->segcap, s->ncl, s->list_M, s->part, s->maxdisp, lc.skin, s->swept);
         return;
     }
     if (s->use_list) {
         const ListConsts lc = list_consts(s);
-        const unsigned long long tb = bits_of(lc.thresh2);
         const int items = s->ntiles * s->list_M;
         dim3 grid((items + BLOCK / TILE - 1) / (BLOCK / TILE));
         if (meas)
             hipLaunchKernelGGL((k_pairs_list<KIND, PREC, true>), grid, dim3(BLOCK), 0, s->stream,
                                s->pos, s->list, s->counts, c, s->n, s->list_M, s->part,
-                               s->maxdisp, tb, s->swept);
+                               s->maxdisp, lc.skin, s->swept);
         else
             hipLaunchKernelGGL((k_pairs_list<KIND, PREC, false>), grid, dim3(BLOCK), 0,
                                s->stream, s->pos, s->list, s->counts, c, s->n, s->list_M,
-                               s->part, s->maxdisp, tb, s->swept);
+                               s->part, s->maxdisp, lc.skin, s->swept);
         return;
     }
     dim3 grid(s->nblocks, s->nchunks);
@@ -427,8 +429,9 @@ void launch_pairs_k(yk_system *s, const ForceConsts &c, bool meas)
                            s->pos, s->box, c, s->stages_per_chunk, s->part, s->swept);
 }
 
-// Rebuild the pair lists if any particle moved more than skin/2 since the
-// last build (decided on the device; the kernel exits at once otherwise).
+// Rebuild the pair lists if the two largest displacements since the last
+// build add up to more than the skin (decided on the device; the kernels exit
+// at once otherwise).
 int launch_list_build(yk_system *s)
 {
     if (!s->use_list) return YK_OK;
@@ -732,7 +735,8 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
             YK_HIPB(hipMalloc(&s->corder, sizeof(int) * (size_t)s->ncl));
         }
     }
-    YK_HIPB(hipMalloc(&s->maxdisp, sizeof(unsigned long long)));
+    YK_HIPB(hipMalloc(&s->maxdisp, 2 * sizeof(unsigned long long)));
+    YK_HIPB(hipMemsetAsync(s->maxdisp, 0, 2 * sizeof(unsigned long long), s->stream));
     YK_HIPB(hipMalloc(&s->swept, sizeof(unsigned long long)));
     YK_HIPB(hipMemsetAsync(s->swept, 0, sizeof(unsigned long long), s->stream));
     YK_HIPB(hipMalloc(&s->part, sizeof(double) * s->part_doubles));
