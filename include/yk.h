@@ -191,6 +191,53 @@ int yk_run(yk_system *sys, double dt, int32_t nsteps, int32_t nind,
  * (0.5*xmass*sum A v^2)/n, src/gen_utils.f:40-55. */
 int yk_ttot(yk_system *sys, double *eka);
 
+/* Velocity utilities of the step loop, on the device with fixed-order
+ * reductions and no host synchronisation (the caller's next download or
+ * yk_run sees their effect):
+ *   yk_tnorm        tnorm, src/gen_utils.f:61-75: v /= sqrt(eka/(1.5*kT))
+ *                   with eka the kinetic energy per particle and kT from the
+ *                   handle's parameters;
+ *   yk_cancel_vcom  cancel_vcom, src/gen_utils.f:82-107: subtract the
+ *                   mass-weighted centre-of-mass velocity, then tnorm.
+ * Both need kT > 0. */
+int yk_tnorm(yk_system *sys);
+int yk_cancel_vcom(yk_system *sys);
+
+/* strain, src/strain.f:19-82: one step of normal strain of the box.
+ * deps[k] = YK_XNOSTRAIN leaves axis k to the constant-volume rule (all three
+ * unstrained: nothing to do); xl0: the gage dimensions.  The new box follows
+ * the reference's formulas (host arithmetic; xl_out may be NULL); positions
+ * beyond it wrap on the device; the pair lists are rebuilt at the next
+ * evaluation; the accelerations stay valid (positions move by whole box
+ * edges only). */
+#define YK_XNOSTRAIN 1000.0
+int yk_strain(yk_system *sys, double dt, const double deps[3], const double xl0[3],
+              double xl_out[3]);
+
+/* The measurement-phase step loop of src/md.F:124-133 on the device. */
+typedef struct yk_loop {
+    double dt;
+    int32_t nsteps;    /* steps in this call                                   */
+    int32_t istep0;    /* the step counter istep at entry (src/md.F:121-124)   */
+    int32_t nind;      /* energy/pressure measurement period (0: never)        */
+    int32_t nptensor;  /* pressure-tensor measurement period (0: never)        */
+    int32_t ncom;      /* cancel_vcom period (0: never)                        */
+    int32_t ntnorm;    /* tnorm period (0: never)                              */
+    double deps[3];    /* strain rates, YK_XNOSTRAIN for none                  */
+    double xl0[3];     /* gage dimensions (read when a rate is set)            */
+} yk_loop;
+
+/* Per step: cancel_vcom when mdmod(istep,ncom) == 0, tnorm when
+ * mdmod(istep,ntnorm) == 0, strain, then newton measuring when
+ * mdmod(istep+1,nind) == 0 or mdmod(istep+1,nptensor) == 0 (mdmod of a zero
+ * period never is 0, src/md.F:248-256).  The r-th measuring step writes
+ * records[r] and rec_steps[r] (its istep after the step); records must hold
+ * every measuring step (max_records) when not NULL; *nrec gets their count and
+ * xl_out the final box (any output may be NULL).  One host synchronisation
+ * at the end. */
+int yk_run_md(yk_system *sys, const yk_loop *loop, yk_meas *records, int32_t *rec_steps,
+              int32_t max_records, int32_t *nrec, double xl_out[3]);
+
 int yk_get_timing(yk_system *sys, yk_timing *out);
 int yk_reset_timing(yk_system *sys);
 

@@ -60,6 +60,9 @@ def load():
         lib.yko_newton_sw.argtypes = [P, C.c_int, C.c_double, _D, _D, _D, _D, _D, C.c_int, _D]
         lib.yko_ttot.argtypes = [P, C.c_int, _D, _D]
         lib.yko_ttot.restype = C.c_double
+        lib.yko_tnorm.argtypes = [P, C.c_int, _D, _D]
+        lib.yko_cancel_vcom.argtypes = [P, C.c_int, _D, _D]
+        lib.yko_strain.argtypes = [P, C.c_int, _D, _D, _D, C.c_double]
         lib.yko_accel_truth.argtypes = [P, C.c_int, _D, _D, _D, _D]
         lib.yko_accel_omp.argtypes = [P, C.c_int, _D, _D, _D, _D, C.c_int]
         _lib = lib
@@ -84,6 +87,9 @@ def load_ref():
         lib.ykref_set_accel.argtypes = [_D]
         lib.ykref_get.argtypes = [_D, _D, _D]
         lib.ykref_ttot.restype = C.c_double
+        lib.ykref_strain.argtypes = [_D, _D, C.c_double, _D]
+        lib.ykref_md_loop.argtypes = [C.c_int] * 6 + [C.c_double, _D, _D, C.c_int, _D,
+                                                        C.POINTER(C.c_int), _D]
         lib.ykref_timers.argtypes = [_D]
         _ref = lib
     return _ref
@@ -159,6 +165,66 @@ def ttot(p, v, A):
     n = v.shape[0]
     ps = params_struct(p)
     return load().yko_ttot(C.byref(ps), n, _ptr(_arr(v, (n, 3))), _ptr(_arr(A, (n,))))
+
+
+XNOSTRAIN = 1000.0   # src/md_globals.f:22
+
+
+def tnorm(p, v, A):
+    """tnorm (src/gen_utils.f:61-75) on a copy of v."""
+    n = v.shape[0]
+    v = _arr(v, (n, 3)).copy()
+    load().yko_tnorm(C.byref(params_struct(p)), n, _ptr(v), _ptr(_arr(A, (n,))))
+    return v
+
+
+def cancel_vcom(p, v, A):
+    """cancel_vcom (src/gen_utils.f:82-107, tnorm included) on a copy of v."""
+    n = v.shape[0]
+    v = _arr(v, (n, 3)).copy()
+    load().yko_cancel_vcom(C.byref(params_struct(p)), n, _ptr(v), _ptr(_arr(A, (n,))))
+    return v
+
+
+def strain(p, x, xl0, deps, dt):
+    """strain (src/strain.f:19-82): (positions, new box)."""
+    n = x.shape[0]
+    x = _arr(x, (n, 3)).copy()
+    ps = params_struct(p)
+    load().yko_strain(C.byref(ps), n, _ptr(x), _ptr(_arr(xl0, (3,))), _ptr(_arr(deps, (3,))),
+                      float(dt))
+    return x, tuple(ps.xl[k] for k in range(3))
+
+
+def _mdmod(p, q):
+    return -1 if q == 0 else p % q   # src/md.F:248-256
+
+
+def md_loop(p, dt, x, v, a, Z, A, nsteps, istep0=0, nind=0, nptensor=0, ncom=0, ntnorm=0,
+            deps=(XNOSTRAIN,) * 3, xl0=None, ref_f1=None):
+    """The measurement-phase step loop of src/md.F:124-133 over the restated
+    routines: per step cancel_vcom (every ncom), tnorm (every ntnorm),
+    strain, newton measuring when mdmod(istep+1,nind) or
+    mdmod(istep+1,nptensor) is 0.  Returns (x, v, a, records, box) with one
+    record per measuring step (its 'istep' included)."""
+    xl0 = p.xl if xl0 is None else xl0
+    x, v, a = _arr(x, x.shape).copy(), _arr(v, v.shape).copy(), _arr(a, a.shape).copy()
+    vold = np.zeros_like(v)
+    istep = istep0
+    recs = []
+    for _ in range(nsteps):
+        if _mdmod(istep, ncom) == 0:
+            v = cancel_vcom(p, v, A)
+        if _mdmod(istep, ntnorm) == 0:
+            v = tnorm(p, v, A)
+        x, box = strain(p, x, xl0, deps, dt)
+        p = p.replace(xl=box)
+        dm = _mdmod(istep + 1, nind) == 0 or _mdmod(istep + 1, nptensor) == 0
+        x, v, vold, a, m = newton(p, dt, x, v, vold, a, Z, A, dm, ref_f1=ref_f1)
+        istep += 1
+        if dm:
+            recs.append(dict(m, istep=istep))
+    return x, v, a, recs, p.xl
 
 
 def truth(p, x, Z):
@@ -239,6 +305,56 @@ def ref_newton_alt(p, dt, x, v, a, Z, A, which, nsteps=1):
     xo, vo, ao = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
     lib.ykref_get(_ptr(xo), _ptr(vo), _ptr(ao))
     return xo, vo, ao, _meas(m)
+
+
+def ref_tnorm(p, v, A):
+    n = v.shape[0]
+    ref_setup(p, np.zeros((n, 3)), v, np.ones(n), A)
+    load_ref().ykref_tnorm()
+    xo, vo, ao = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+    load_ref().ykref_get(_ptr(xo), _ptr(vo), _ptr(ao))
+    return vo
+
+
+def ref_cancel_vcom(p, v, A):
+    n = v.shape[0]
+    ref_setup(p, np.zeros((n, 3)), v, np.ones(n), A)
+    load_ref().ykref_cancel_vcom()
+    xo, vo, ao = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+    load_ref().ykref_get(_ptr(xo), _ptr(vo), _ptr(ao))
+    return vo
+
+
+def ref_strain(p, x, xl0, deps, dt):
+    n = x.shape[0]
+    ref_setup(p, x, None, np.ones(n), np.ones(n))
+    xl = np.zeros(3)
+    load_ref().ykref_strain(_ptr(_arr(deps, (3,))), _ptr(_arr(xl0, (3,))), float(dt), _ptr(xl))
+    xo, vo, ao = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+    load_ref().ykref_get(_ptr(xo), _ptr(vo), _ptr(ao))
+    return xo, tuple(xl)
+
+
+def ref_md_loop(p, dt, x, v, a, Z, A, nsteps, istep0=0, nind=0, nptensor=0, ncom=0, ntnorm=0,
+                deps=(XNOSTRAIN,) * 3, xl0=None):
+    """The same loop on the reference's own routines (ref_harness.f90)."""
+    n = x.shape[0]
+    xl0 = p.xl if xl0 is None else xl0
+    ref_setup(p, x, v, Z, A)
+    lib = load_ref()
+    lib.ykref_set_accel(_ptr(_arr(a, (n, 3))))
+    maxrec = max(nsteps, 1)
+    rec = np.zeros((maxrec, 10))
+    nrec = C.c_int(0)
+    xl = np.zeros(3)
+    lib.ykref_md_loop(int(nsteps), int(istep0), int(nind), int(nptensor), int(ncom), int(ntnorm),
+                      float(dt), _ptr(_arr(deps, (3,))), _ptr(_arr(xl0, (3,))), maxrec,
+                      _ptr(rec), C.byref(nrec), _ptr(xl))
+    xo, vo, ao = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+    lib.ykref_get(_ptr(xo), _ptr(vo), _ptr(ao))
+    keys = ("ev", "ek", "px", "pp11", "pp12", "pp13", "pp22", "pp23", "pp33")
+    recs = [dict(zip(keys, r[1:]), istep=int(r[0])) for r in rec[:nrec.value]]
+    return xo, vo, ao, recs, tuple(xl)
 
 
 def ref_timers():

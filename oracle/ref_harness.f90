@@ -179,6 +179,85 @@ contains
     ykref_ttot = eka
   end function ykref_ttot
 
+  ! The reference's velocity utilities on the globals (src/gen_utils.f:61-107);
+  ! kT is the value ykref_setup set.
+  subroutine ykref_tnorm() bind(C, name='ykref_tnorm')
+    call tnorm()
+  end subroutine ykref_tnorm
+
+  subroutine ykref_cancel_vcom() bind(C, name='ykref_cancel_vcom')
+    call cancel_vcom()
+  end subroutine ykref_cancel_vcom
+
+  ! The reference's strain (src/strain.f:19-82) with these rates, gage
+  ! dimensions and timestep; returns the new box.
+  subroutine ykref_strain(deps_in, xl0_in, dtin, xlout) bind(C, name='ykref_strain')
+    real(c_double), intent(in) :: deps_in(3), xl0_in(3)
+    real(c_double), value :: dtin
+    real(c_double), intent(out) :: xlout(3)
+    deps = deps_in
+    xl0 = xl0_in
+    dt = dtin
+    call strain()
+    xlout = xl
+  end subroutine ykref_strain
+
+  ! The measurement-phase step loop of src/md.F:124-133 on the globals: per
+  ! step cancel_vcom (every ncom), tnorm (every ntnorm), strain, then newton
+  ! measuring when mdmod(istep+1,nind) or mdmod(istep+1,nptensor) is 0.  One
+  ! record per measuring step, as the .meas record of src/md.F:150:
+  ! (istep, ev, ek, px, pp11, pp12, pp13, pp22, pp23, pp33).  xlout: the box
+  ! after the loop.
+  subroutine ykref_md_loop(nsteps, istep0, nind_, nptensor_, ncom_, ntnorm_, dtin, deps_in, &
+                           xl0_in, maxrec, rec, nrec, xlout) bind(C, name='ykref_md_loop')
+    integer(c_int), value :: nsteps, istep0, nind_, nptensor_, ncom_, ntnorm_, maxrec
+    real(c_double), value :: dtin
+    real(c_double), intent(in) :: deps_in(3), xl0_in(3)
+    real(c_double), intent(out) :: rec(10, maxrec)
+    integer(c_int), intent(out) :: nrec
+    real(c_double), intent(out) :: xlout(3)
+    integer :: i, istep
+    logical :: dm
+    dt = dtin
+    deps = deps_in
+    xl0 = xl0_in
+    istep = istep0
+    nrec = 0
+    do i = 1, nsteps
+      if (ymod(istep, ncom_) == 0) call cancel_vcom()
+      if (ymod(istep, ntnorm_) == 0) call tnorm()
+      call strain()
+      dm = ymod(istep + 1, nind_) == 0 .or. ymod(istep + 1, nptensor_) == 0
+      call newton(dm)
+      istep = istep + 1
+      if (dm .and. nrec < maxrec) then
+        nrec = nrec + 1
+        rec(1, nrec) = real(istep, c_double)
+        rec(2, nrec) = ev
+        rec(3, nrec) = ek
+        rec(4, nrec) = px
+        rec(5, nrec) = pp(1, 1)
+        rec(6, nrec) = pp(1, 2)
+        rec(7, nrec) = pp(1, 3)
+        rec(8, nrec) = pp(2, 2)
+        rec(9, nrec) = pp(2, 3)
+        rec(10, nrec) = pp(3, 3)
+      end if
+    end do
+    xlout = xl
+  end subroutine ykref_md_loop
+
+  ! mdmod of src/md.F:248-256 (an internal function of the main program
+  ! there): -1 for a zero period, so that step never comes.
+  integer function ymod(p, q)
+    integer, intent(in) :: p, q
+    if (q == 0) then
+      ymod = -1
+    else
+      ymod = mod(p, q)
+    end if
+  end function ymod
+
   ! Reference timers (src/perf.h): pair-loop time and count, accel, newton.
   subroutine ykref_timers(t) bind(C, name='ykref_timers')
     real(c_double), intent(out) :: t(6)

@@ -503,6 +503,81 @@ double yko_ttot(const yko_params *p, int n, const double *v, const double *aii)
     return (0.5 * p->xmass * ekx) / (double)(float)n;
 }
 
+/* tnorm, src/gen_utils.f:61-75: fac = sqrt(eka/(1.5*kT)), v = v/fac. */
+void yko_tnorm(const yko_params *p, int n, double *v, const double *aii)
+{
+    const double eka = yko_ttot(p, n, v, aii);
+    const double fac = sqrt(eka / (1.5 * p->kT));
+    for (int i = 0; i < 3 * n; ++i) v[i] = v[i] / fac;
+}
+
+/* cancel_vcom, src/gen_utils.f:82-107: mass-weighted sums in particle order,
+ * vcom = ptot/mtot subtracted, then tnorm. */
+void yko_cancel_vcom(const yko_params *p, int n, double *v, const double *aii)
+{
+    double mtot = 0.0, ptot[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < n; ++i) {
+        mtot = mtot + aii[i];
+        for (int k = 0; k < 3; ++k) ptot[k] = ptot[k] + aii[i] * v[3 * i + k];
+    }
+    double vcom[3];
+    for (int k = 0; k < 3; ++k) vcom[k] = ptot[k] / mtot;
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) v[3 * i + k] = v[3 * i + k] - vcom[k];
+    yko_tnorm(p, n, v, aii);
+}
+
+/* strain, src/strain.f:19-82.  The strained axes grow by deps*dt*xl0; with one
+ * or two strained axes the others follow from constant volume, with three
+ * all follow their rates.  Then positions beyond the new box wrap once. */
+void yko_strain(yko_params *p, int n, double *x, const double *xl0, const double *deps,
+                double dt)
+{
+    double *xl = p->xl;
+    const int c = (deps[0] != YKO_XNOSTRAIN ? 1 : 0) | (deps[1] != YKO_XNOSTRAIN ? 2 : 0) |
+                  (deps[2] != YKO_XNOSTRAIN ? 4 : 0);
+    switch (c) {
+    case 1:
+        xl[0] = xl[0] + deps[0] * dt * xl0[0];
+        xl[1] = xl0[1] * sqrt(xl0[0] / xl[0]);
+        xl[2] = xl0[2] * sqrt(xl0[0] / xl[0]);
+        break;
+    case 2:
+        xl[1] = xl[1] + deps[1] * dt * xl0[1];
+        xl[0] = xl0[0] * sqrt(xl0[1] / xl[1]);
+        xl[2] = xl0[2] * sqrt(xl0[1] / xl[1]);
+        break;
+    case 3:
+        xl[0] = xl[0] + deps[0] * dt * xl0[0];
+        xl[1] = xl[1] + deps[1] * dt * xl0[1];
+        xl[2] = xl0[2] * (xl0[0] / xl[0]) * (xl0[1] / xl[1]);
+        break;
+    case 4:
+        xl[2] = xl[2] + deps[2] * dt * xl0[2];
+        xl[0] = xl0[0] * sqrt(xl0[2] / xl[2]);
+        xl[1] = xl0[1] * sqrt(xl0[2] / xl[2]);
+        break;
+    case 5:
+        xl[0] = xl[0] + deps[0] * dt * xl0[0];
+        xl[2] = xl[2] + deps[2] * dt * xl0[2];
+        xl[1] = xl0[1] * (xl0[0] / xl[0]) * (xl0[2] / xl[2]);
+        break;
+    case 6:
+        xl[1] = xl[1] + deps[1] * dt * xl0[1];
+        xl[2] = xl[2] + deps[2] * dt * xl0[2];
+        xl[0] = xl0[0] * (xl0[1] / xl[1]) * (xl0[2] / xl[2]);
+        break;
+    case 7:
+        for (int k = 0; k < 3; ++k) xl[k] = xl[k] + deps[k] * dt * xl0[k];
+        break;
+    default:
+        break;
+    }
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k)
+            if (x[3 * i + k] > xl[k]) x[3 * i + k] = x[3 * i + k] - xl[k];
+}
+
 /* ------------------------------------------------------------------------ */
 /* Long-double full-matrix evaluation of the same physics (no F1 defect).    */
 /* Pair energies are summed per i over all j != i and halved.                */

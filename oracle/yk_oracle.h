@@ -16,6 +16,9 @@
  *   yko_newton_kdk      <- newton           src/vel_verlet_1.f:16-89 (algorithm spec)
  *   yko_newton_sw       <- newton           src/vel_verlet_3.f:20-100 (algorithm spec)
  *   yko_ttot            <- ttot             src/gen_utils.f:40-55
+ *   yko_tnorm           <- tnorm            src/gen_utils.f:61-75
+ *   yko_cancel_vcom     <- cancel_vcom      src/gen_utils.f:82-107
+ *   yko_strain          <- strain           src/strain.f:19-82
  *   yko_accel_truth     <- same physics, full N x N sum in long double
  *
  * Arrays are the reference's Fortran column-major (3,n) layout: element (k,i)
@@ -75,6 +78,16 @@ int yko_newton_sw(const yko_params *p, int n, double dt, double *x, double *v,
                   double *meas);
 
 double yko_ttot(const yko_params *p, int n, const double *v, const double *aii);
+/* Scale v to the kinetic energy 1.5 kT per particle. */
+void yko_tnorm(const yko_params *p, int n, double *v, const double *aii);
+/* Remove the centre-of-mass velocity (mass weights aii), then tnorm. */
+void yko_cancel_vcom(const yko_params *p, int n, double *v, const double *aii);
+/* XNOSTRAIN in deps[k] means no strain rate on axis k (src/md_globals.f:22). */
+#define YKO_XNOSTRAIN 1000.0
+/* Box update for one step of strain rates deps from the gage dimensions
+ * xl0 (p->xl is updated), then the wrap of positions beyond the new box. */
+void yko_strain(yko_params *p, int n, double *x, const double *xl0, const double *deps,
+                double dt);
 
 /* Full-matrix force/energy/virial in long double: the "truth" that both the
  * reference and the GPU are compared with.  force[3*i+k] is the physical force

@@ -268,3 +268,62 @@ def test_spreiter_walter_free_particle_gyrates():
     s0 = np.hypot(v[:, 0], v[:, 1])
     s1 = np.hypot(vo[:, 0], vo[:, 1])
     assert np.all(np.abs(s1 - s0) <= 2 * th ** 2 * s0)
+
+
+# --- velocity utilities and strain: what the mathematics fixes ---------------
+def test_tnorm_sets_the_temperature():
+    """After tnorm the kinetic energy per particle is 1.5 kT (equipartition,
+    src/gen_utils.f:58-60), whatever the starting scale."""
+    w = W.ion_mixture(n=500, seed=31)
+    for scale in (0.3, 1.0, 4.0):
+        v = oracle.tnorm(w.params, w.v * scale, w.A)
+        assert oracle.ttot(w.params, v, w.A) == pytest.approx(1.5 * w.params.kT, rel=1e-14)
+        # a uniform rescale: directions unchanged
+        np.testing.assert_allclose(v / np.linalg.norm(v), w.v / np.linalg.norm(w.v), rtol=1e-13)
+
+
+def test_cancel_vcom_removes_momentum():
+    """Mass-weighted momentum vanishes after cancel_vcom, and the temperature is
+    restored to 1.5 kT by its closing tnorm."""
+    w = W.ion_mixture(n=500, seed=32)
+    v0 = w.v + np.array([0.003, -0.002, 0.001])
+    v = oracle.cancel_vcom(w.params, v0, w.A)
+    ptot = (w.A[:, None] * v).sum(axis=0)
+    assert np.abs(ptot).max() <= 1e-13 * (w.A[:, None] * np.abs(v)).sum()
+    assert oracle.ttot(w.params, v, w.A) == pytest.approx(1.5 * w.params.kT, rel=1e-14)
+
+
+@pytest.mark.parametrize("deps", [(2e-6, None, None), (None, -3e-6, None), (None, None, 1e-6),
+                                  (2e-6, -1e-6, None), (-2e-6, None, 1e-6),
+                                  (None, 1e-6, -2e-6)])
+def test_strain_keeps_the_volume(deps):
+    """One or two strained axes: the others follow so that the volume stays the
+    gage volume (src/strain.f:15-17); a strained edge grows by deps*dt*xl0."""
+    X = oracle.XNOSTRAIN
+    d = tuple(X if e is None else e for e in deps)
+    w = W.ion_mixture(n=50, seed=33)
+    xl0 = w.params.xl
+    p = w.params
+    dt = 1000.0
+    box = p.xl
+    for _ in range(5):
+        _, box = oracle.strain(p.replace(xl=box), w.x, xl0, d, dt)
+    assert math.prod(box) == pytest.approx(math.prod(xl0), rel=1e-14)
+    for k in range(3):
+        if deps[k] is not None:
+            assert box[k] == pytest.approx(xl0[k] * (1 + 5 * deps[k] * dt), rel=1e-14)
+
+
+def test_strain_wraps_only_beyond_the_new_box():
+    """Positions above a shrunk edge move down by one edge length; all others
+    stay (src/strain.f:76-82)."""
+    X = oracle.XNOSTRAIN
+    w = W.ion_mixture(n=400, seed=34)
+    p = w.params
+    x1, box = oracle.strain(p, w.x, p.xl, (-1e-3, X, X), 10.0)
+    assert box[0] < p.xl[0] and box[1] > p.xl[1]
+    moved = w.x[:, 0] > box[0]
+    assert moved.any()
+    np.testing.assert_array_equal(x1[moved, 0], w.x[moved, 0] - box[0])
+    np.testing.assert_array_equal(x1[~moved], w.x[~moved])
+    np.testing.assert_array_equal(x1[:, 1:], w.x[:, 1:])

@@ -93,3 +93,60 @@ def test_alternate_integrators_bitexact(which, name, make):
     assert np.array_equal(x, xr) and np.array_equal(v, vr) and np.array_equal(a, ar)
     for k in ("ev", "ek", "px", "pp11", "pp12", "pp13", "pp22", "pp23", "pp33"):
         assert m[k] == mr[k], k
+
+
+# --- velocity utilities, strain and the md.F step loop -----------------------
+X = oracle.XNOSTRAIN
+
+
+@pytest.mark.parametrize("name,make", CASES[:2])
+def test_tnorm_cancel_vcom_bitexact(name, make):
+    """tnorm and cancel_vcom (src/gen_utils.f:61-107) on velocities with a
+    drift and the wrong temperature."""
+    w = make()
+    v = w.v * 1.7 + np.array([0.002, -0.001, 0.0005])
+    assert np.array_equal(oracle.tnorm(w.params, v, w.A), oracle.ref_tnorm(w.params, v, w.A))
+    assert np.array_equal(oracle.cancel_vcom(w.params, v, w.A),
+                          oracle.ref_cancel_vcom(w.params, v, w.A))
+
+
+STRAINS = [(X, X, X), (2e-6, X, X), (X, -3e-6, X), (2e-6, -1e-6, X), (X, X, 3e-6),
+           (-2e-6, X, 1e-6), (X, 1e-6, -2e-6), (1e-6, 2e-6, -3e-6)]
+
+
+@pytest.mark.parametrize("deps", STRAINS)
+def test_strain_bitexact(deps):
+    """Every case of src/strain.f:31-72 (one, two or three strained axes, or
+    none), from a box that differs from the gage, with particles at the upper
+    faces so that the shrinking edges wrap them."""
+    w = W.ion_mixture(n=300, seed=16)
+    L = np.array(w.params.xl)
+    xl0 = tuple(0.97 * L)
+    x = w.x.copy()
+    x[:6] = L * 0.99999
+    dt = w.dt * 200
+    x1, b1 = oracle.strain(w.params, x, xl0, deps, dt)
+    x2, b2 = oracle.ref_strain(w.params, x, xl0, deps, dt)
+    assert b1 == b2
+    assert np.array_equal(x1, x2)
+    if deps != (X, X, X):
+        assert b1 != tuple(L) and not np.array_equal(x1, x)
+
+
+def test_md_loop_bitexact():
+    """The measurement-phase loop of src/md.F:124-133 -- cancel_vcom every 5,
+    tnorm every 3, an x strain each step, newton measuring at the nind=4 and
+    nptensor=6 steps -- restated against the reference's own routines."""
+    w = W.ion_mixture(n=200, seed=17)
+    p = w.params
+    a, _ = oracle.accel(p, w.x, w.Z, w.A, True, ref_f1=1)
+    kw = dict(nsteps=13, istep0=0, nind=4, nptensor=6, ncom=5, ntnorm=3, deps=(2e-6, X, X),
+              xl0=p.xl)
+    x1, v1, a1, r1, b1 = oracle.md_loop(p, w.dt, w.x, w.v, a, w.Z, w.A, ref_f1=1, **kw)
+    x2, v2, a2, r2, b2 = oracle.ref_md_loop(p, w.dt, w.x, w.v, a, w.Z, w.A, **kw)
+    assert b1 == b2 and b1 != p.xl
+    assert np.array_equal(x1, x2) and np.array_equal(v1, v2) and np.array_equal(a1, a2)
+    assert [r["istep"] for r in r1] == [r["istep"] for r in r2] == [4, 6, 8, 12]
+    for q1, q2 in zip(r1, r2):
+        for k in q2:
+            assert q1[k] == q2[k], (q2["istep"], k)

@@ -39,6 +39,15 @@ class YkMeas(C.Structure):
                 "pp": list(self.pp)}
 
 
+XNOSTRAIN = 1000.0   # YK_XNOSTRAIN: no strain rate on an axis (src/md_globals.f:22)
+
+
+class YkLoop(C.Structure):
+    _fields_ = [("dt", C.c_double), ("nsteps", C.c_int32), ("istep0", C.c_int32),
+                ("nind", C.c_int32), ("nptensor", C.c_int32), ("ncom", C.c_int32),
+                ("ntnorm", C.c_int32), ("deps", C.c_double * 3), ("xl0", C.c_double * 3)]
+
+
 class YkTiming(C.Structure):
     _fields_ = [("force_ms", C.c_double), ("force_launches", C.c_int64),
                 ("step_ms", C.c_double), ("steps", C.c_int64), ("pairs_swept", C.c_int64)]
@@ -73,6 +82,11 @@ SIGNATURES = {
     "yk_newton": (C.c_int, [P, C.c_double, C.c_int32, C.POINTER(YkMeas)]),
     "yk_run": (C.c_int, [P, C.c_double, C.c_int32, C.c_int32, C.POINTER(YkMeas)]),
     "yk_ttot": (C.c_int, [P, D]),
+    "yk_tnorm": (C.c_int, [P]),
+    "yk_cancel_vcom": (C.c_int, [P]),
+    "yk_strain": (C.c_int, [P, C.c_double, D, D, D]),
+    "yk_run_md": (C.c_int, [P, C.POINTER(YkLoop), C.POINTER(YkMeas), C.POINTER(C.c_int32),
+                            C.c_int32, C.POINTER(C.c_int32), D]),
     "yk_get_timing": (C.c_int, [P, C.POINTER(YkTiming)]),
     "yk_reset_timing": (C.c_int, [P]),
     # batched ensembles / continuous batching

@@ -1395,6 +1395,126 @@ static __global__ __launch_bounds__(BLOCK) void k_ttot(const double *__restrict_
 }
 
 // Final fixed-order reduction of block partials into the scalars.
+// ---------------------------------------------------------------------------
+// Velocity utilities of the step loop on the device (src/gen_utils.f:40-107):
+// the centre-of-mass velocity and the rescale to the temperature, with
+// fixed-order reductions (per-block sums, then one workgroup over the blocks:
+// repeatable, no host round trip).  Per block: q[0] = sum aii |v|^2,
+// q[1] = sum aii, q[2..4] = sum aii v_k.
+constexpr int NVQ = 5;
+
+static __global__ __launch_bounds__(BLOCK) void k_vsums(const double *__restrict__ vel,
+                                                       const double *__restrict__ aii, int n,
+                                                       double *__restrict__ blockpart)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const size_t nn = (size_t)n;
+    double q[NVQ] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i < n) {
+        const double a = aii[i], vx = vel[i], vy = vel[nn + i], vz = vel[2 * nn + i];
+        q[0] = a * (vx * vx + vy * vy + vz * vz);   // src/gen_utils.f:50
+        q[1] = a;                                   // src/gen_utils.f:96-97
+        q[2] = a * vx;
+        q[3] = a * vy;
+        q[4] = a * vz;
+    }
+    __shared__ double red[NVQ][BLOCK];
+#pragma unroll
+    for (int k = 0; k < NVQ; ++k) red[k][threadIdx.x] = q[k];
+    __syncthreads();
+    for (int st = BLOCK / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+#pragma unroll
+            for (int k = 0; k < NVQ; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < NVQ) blockpart[(size_t)blockIdx.x * NVQ + threadIdx.x] = red[threadIdx.x][0];
+}
+
+constexpr int VEL_COM = 0;     // vstate[0..2] = vcom = ptot / mtot
+constexpr int VEL_TNORM = 1;   // vstate[3] = fac = sqrt(eka / (1.5 kT))
+
+struct VelConsts {
+    double xmass, kT;
+    float n_f;   // float(n), src/gen_utils.f:53
+    int mode;
+};
+
+static __global__ __launch_bounds__(BLOCK) void k_vfinal(const double *__restrict__ blockpart,
+                                                        int nblocks, VelConsts vc,
+                                                        double *__restrict__ vstate)
+{
+    double q[NVQ] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += BLOCK) {
+#pragma unroll
+        for (int k = 0; k < NVQ; ++k) q[k] += blockpart[(size_t)b * NVQ + k];
+    }
+    __shared__ double red[NVQ][BLOCK];
+#pragma unroll
+    for (int k = 0; k < NVQ; ++k) red[k][threadIdx.x] = q[k];
+    __syncthreads();
+    for (int st = BLOCK / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+#pragma unroll
+            for (int k = 0; k < NVQ; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (vc.mode == VEL_COM) {   // src/gen_utils.f:100
+        for (int k = 0; k < 3; ++k) vstate[k] = red[2 + k][0] / red[1][0];
+    } else {                    // src/gen_utils.f:53, :68
+        const double eka = (0.5 * vc.xmass * red[0][0]) / (double)vc.n_f;
+        vstate[3] = __builtin_sqrt(eka / (1.5 * vc.kT));
+    }
+}
+
+static __global__ __launch_bounds__(BLOCK) void k_vapply(double *__restrict__ vel, int n, int mode,
+                                                        const double *__restrict__ vstate)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const size_t nn = (size_t)n;
+    if (mode == VEL_COM) {   // src/gen_utils.f:101-105
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vel[k * nn + i] = vel[k * nn + i] - vstate[k];
+    } else {                 // src/gen_utils.f:70-73
+        const double fac = vstate[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vel[k * nn + i] = vel[k * nn + i] / fac;
+    }
+}
+
+// A new box (strain, or a caller's new edge lengths): positions, continuous
+// on the device since the last re-sort, first take the periodic wrap into
+// the old box that the reference applies every step (src/vel_verlet_2.f:
+// 39-40; a floor-based wrap for anything farther out), then strain's wrap of
+// those beyond the new box (src/strain.f:76-82).
+static __global__ __launch_bounds__(BLOCK) void k_box_change(double4 *__restrict__ pos, int n,
+                                                            double lo0, double lo1, double lo2,
+                                                            double ln0, double ln1, double ln2)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double4 p = pos[i];
+    double x[3] = {p.x, p.y, p.z};
+    const double lo[3] = {lo0, lo1, lo2}, ln[3] = {ln0, ln1, ln2};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double v = x[k];
+        if (v < 0.0) v = v + lo[k];
+        if (v > lo[k]) v = v - lo[k];
+        if (v < 0.0 || v > lo[k]) v = v - lo[k] * __builtin_floor(v / lo[k]);
+        if (v > ln[k]) v = v - ln[k];
+        x[k] = v;
+    }
+    p.x = x[0];
+    p.y = x[1];
+    p.z = x[2];
+    pos[i] = p;
+}
+
 struct MeasOut {
     double ev, ek, px, vir, pp[6];
 };

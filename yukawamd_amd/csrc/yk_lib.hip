@@ -86,6 +86,7 @@ struct yk_system {
     size_t part_doubles = 0;
     double *blockpart = nullptr;   // [nblocks][NQ]
     MeasOut *d_meas = nullptr;     // device scalar records
+    double *d_vstate = nullptr;    // [4] vcom, fac of the velocity utilities
     int meas_cap = 0;
     // sort scratch
     unsigned *keys = nullptr, *keys2 = nullptr;
@@ -603,6 +604,100 @@ int resort(yk_system *s)
     return invalidate_list(s);   // slots moved: the lists refer to old slots
 }
 
+// Velocity utilities (src/gen_utils.f:61-107) on the stream, no host sync.
+int vel_util(yk_system *s, int mode)
+{
+    hipLaunchKernelGGL(k_vsums, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->vel, s->aii,
+                       s->n, s->blockpart);
+    const VelConsts vc{s->p.xmass, s->p.kT, (float)s->n, mode};
+    hipLaunchKernelGGL(k_vfinal, dim3(1), dim3(BLOCK), 0, s->stream, s->blockpart, s->nblocks,
+                       vc, s->d_vstate);
+    hipLaunchKernelGGL(k_vapply, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->vel, s->n, mode,
+                       s->d_vstate);
+    YK_HIP(hipGetLastError());
+    return YK_OK;
+}
+
+int do_tnorm(yk_system *s) { return vel_util(s, VEL_TNORM); }
+
+int do_cancel_vcom(yk_system *s)   // src/gen_utils.f:82-107: then tnorm
+{
+    int rc = vel_util(s, VEL_COM);
+    return rc ? rc : do_tnorm(s);
+}
+
+// New box edges: device positions wrap into the old box and beyond the new
+// one (k_box_change), the pair lists are rebuilt at the next evaluation.
+int change_box(yk_system *s, const double *xl)
+{
+    if (xl[0] == s->p.xl[0] && xl[1] == s->p.xl[1] && xl[2] == s->p.xl[2]) return YK_OK;
+    hipLaunchKernelGGL(k_box_change, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos, s->n,
+                       s->p.xl[0], s->p.xl[1], s->p.xl[2], xl[0], xl[1], xl[2]);
+    YK_HIP(hipGetLastError());
+    for (int k = 0; k < 3; ++k) s->p.xl[k] = xl[k];
+    return invalidate_list(s);
+}
+
+// The box after one step of strain rates deps from the gage dimensions xl0,
+// with the formulas of src/strain.f:31-72 (host arithmetic, so that the
+// box is the reference's to the last bit).  Returns false for no strain.
+bool strain_box(double *xl, const double *xl0, const double *deps, double dt)
+{
+    const int c = (deps[0] != YK_XNOSTRAIN ? 1 : 0) | (deps[1] != YK_XNOSTRAIN ? 2 : 0) |
+                  (deps[2] != YK_XNOSTRAIN ? 4 : 0);
+    // the edge that follows when one or two are strained, as a function of the others
+    switch (c) {
+    case 0:
+        return false;
+    case 1:
+        xl[0] = xl[0] + deps[0] * dt * xl0[0];
+        xl[1] = xl0[1] * std::sqrt(xl0[0] / xl[0]);
+        xl[2] = xl0[2] * std::sqrt(xl0[0] / xl[0]);
+        break;
+    case 2:
+        xl[1] = xl[1] + deps[1] * dt * xl0[1];
+        xl[0] = xl0[0] * std::sqrt(xl0[1] / xl[1]);
+        xl[2] = xl0[2] * std::sqrt(xl0[1] / xl[1]);
+        break;
+    case 3:
+        xl[0] = xl[0] + deps[0] * dt * xl0[0];
+        xl[1] = xl[1] + deps[1] * dt * xl0[1];
+        xl[2] = xl0[2] * (xl0[0] / xl[0]) * (xl0[1] / xl[1]);
+        break;
+    case 4:
+        xl[2] = xl[2] + deps[2] * dt * xl0[2];
+        xl[0] = xl0[0] * std::sqrt(xl0[2] / xl[2]);
+        xl[1] = xl0[1] * std::sqrt(xl0[2] / xl[2]);
+        break;
+    case 5:
+        xl[0] = xl[0] + deps[0] * dt * xl0[0];
+        xl[2] = xl[2] + deps[2] * dt * xl0[2];
+        xl[1] = xl0[1] * (xl0[0] / xl[0]) * (xl0[2] / xl[2]);
+        break;
+    case 6:
+        xl[1] = xl[1] + deps[1] * dt * xl0[1];
+        xl[2] = xl[2] + deps[2] * dt * xl0[2];
+        xl[0] = xl0[0] * (xl0[1] / xl[1]) * (xl0[2] / xl[2]);
+        break;
+    default:
+        for (int k = 0; k < 3; ++k) xl[k] = xl[k] + deps[k] * dt * xl0[k];
+        break;
+    }
+    return true;
+}
+
+int do_strain(yk_system *s, double dt, const double *deps, const double *xl0)
+{
+    double xl[3] = {s->p.xl[0], s->p.xl[1], s->p.xl[2]};
+    if (!strain_box(xl, xl0, deps, dt)) return YK_OK;
+    for (int k = 0; k < 3; ++k)
+        if (!(std::isfinite(xl[k]) && xl[k] > 0.0))
+            return fail(YK_ERR_ARG, "strain gives box edge %d = %g", k, xl[k]);
+    return change_box(s, xl);
+}
+
+int mdmod(int p, int q) { return q == 0 ? -1 : p % q; }   // src/md.F:248-256
+
 // One step on the device: (periodic re-sort), drift, pairs, finalize
 // (+ measurement into slot).
 int device_step(yk_system *s, double dt, bool meas, int slot)
@@ -629,7 +724,8 @@ void release(yk_system *s)
                       (void *)s->vold, (void *)s->vold2, (void *)s->acc, (void *)s->acc2,
                       (void *)s->aii, (void *)s->aii2, (void *)s->den, (void *)s->den2,
                       (void *)s->orig, (void *)s->orig2, (void *)s->box, (void *)s->part,
-                      (void *)s->blockpart, (void *)s->d_meas, (void *)s->keys,
+                      (void *)s->blockpart, (void *)s->d_meas, (void *)s->d_vstate,
+                      (void *)s->keys,
                       (void *)s->keys2, (void *)s->vals, (void *)s->vals2, s->sort_tmp,
                       (void *)s->list, (void *)s->counts, (void *)s->xref,
                       (void *)s->clist, (void *)s->ccounts, (void *)s->corder,
@@ -741,6 +837,7 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
     YK_HIPB(hipMemsetAsync(s->swept, 0, sizeof(unsigned long long), s->stream));
     YK_HIPB(hipMalloc(&s->part, sizeof(double) * s->part_doubles));
     YK_HIPB(hipMalloc(&s->blockpart, sizeof(double) * NQ * (size_t)s->nblocks));
+    YK_HIPB(hipMalloc(&s->d_vstate, 4 * sizeof(double)));
     YK_HIPB(hipMalloc(&s->keys, sizeof(unsigned) * nn));
     YK_HIPB(hipMalloc(&s->keys2, sizeof(unsigned) * nn));
     YK_HIPB(hipMalloc(&s->vals, sizeof(int) * nn));
@@ -779,6 +876,7 @@ int yk_set_params(yk_system *s, const yk_params *params)
         return fail(YK_ERR_UNSUPPORTED, "YK_FLAG_ALLPAIRS / YK_FLAG_TILE_LISTS are fixed at yk_create");
     if (list_cutoff(*params) + 0.0 != list_cutoff(s->p) && s->use_list)
         s->skin = 0.05 * list_cutoff(*params);
+    if ((rc = change_box(s, params->xl))) return rc;
     s->p = *params;
     if (params->resort_every > 0) s->resort_every = params->resort_every;
     s->have_accel = false;
@@ -972,6 +1070,93 @@ int yk_ttot(yk_system *s, double *eka)
                           s->stream));
     YK_HIP(hipStreamSynchronize(s->stream));
     *eka = s->h_meas[0].ek;
+    return YK_OK;
+}
+
+int yk_tnorm(yk_system *s)
+{
+    if (!s) return fail(YK_ERR_ARG, "handle is NULL");
+    if (!(s->p.kT > 0.0)) return fail(YK_ERR_ARG, "tnorm needs kT > 0 (kT=%g)", s->p.kT);
+    return do_tnorm(s);
+}
+
+int yk_cancel_vcom(yk_system *s)
+{
+    if (!s) return fail(YK_ERR_ARG, "handle is NULL");
+    if (!(s->p.kT > 0.0)) return fail(YK_ERR_ARG, "cancel_vcom needs kT > 0 (kT=%g)", s->p.kT);
+    return do_cancel_vcom(s);
+}
+
+int yk_strain(yk_system *s, double dt, const double *deps, const double *xl0, double *xl_out)
+{
+    if (!s || !deps || !xl0) return fail(YK_ERR_ARG, "handle, deps or xl0 is NULL");
+    int rc = do_strain(s, dt, deps, xl0);
+    if (rc) return rc;
+    if (xl_out)
+        for (int k = 0; k < 3; ++k) xl_out[k] = s->p.xl[k];
+    return YK_OK;
+}
+
+int yk_run_md(yk_system *s, const yk_loop *lp, yk_meas *records, int32_t *rec_steps,
+              int32_t max_records, int32_t *nrec, double *xl_out)
+{
+    if (!s || !lp) return fail(YK_ERR_ARG, "handle or loop is NULL");
+    if (lp->nsteps < 0 || lp->istep0 < 0 || lp->nind < 0 || lp->nptensor < 0 || lp->ncom < 0 ||
+        lp->ntnorm < 0)
+        return fail(YK_ERR_ARG, "negative step count or period");
+    if (!std::isfinite(lp->dt)) return fail(YK_ERR_ARG, "dt=%g is not finite", lp->dt);
+    if (!s->have_accel)
+        return fail(YK_ERR_STATE, "run needs accelerations of the current positions: "
+                                  "call yk_accel first (src/md.F:69)");
+    if ((lp->ncom > 0 || lp->ntnorm > 0) && !(s->p.kT > 0.0))
+        return fail(YK_ERR_ARG, "cancel_vcom/tnorm need kT > 0 (kT=%g)", s->p.kT);
+    // measuring steps, known in advance
+    int count = 0;
+    for (int st = 0; st < lp->nsteps; ++st) {
+        const int is = lp->istep0 + st;
+        if (mdmod(is + 1, lp->nind) == 0 || mdmod(is + 1, lp->nptensor) == 0) ++count;
+    }
+    if (records && count > max_records)
+        return fail(YK_ERR_ARG, "%d measuring steps, records holds %d", count, max_records);
+    int rc = ensure_meas_cap(s, count > 0 ? count : 1);
+    if (rc) return rc;
+    const bool timing = (s->p.flags & YK_FLAG_TIMING) != 0;
+    if (timing) {
+        if (!s->step0) YK_HIP(hipEventCreate(&s->step0));
+        if (!s->step1) YK_HIP(hipEventCreate(&s->step1));
+        YK_HIP(hipEventRecord(s->step0, s->stream));
+    }
+    int slot = 0;
+    for (int st = 0; st < lp->nsteps; ++st) {   // src/md.F:124-133
+        const int is = lp->istep0 + st;
+        if (mdmod(is, lp->ncom) == 0 && (rc = do_cancel_vcom(s))) return rc;
+        if (mdmod(is, lp->ntnorm) == 0 && (rc = do_tnorm(s))) return rc;
+        if ((rc = do_strain(s, lp->dt, lp->deps, lp->xl0))) return rc;
+        const bool meas = mdmod(is + 1, lp->nind) == 0 || mdmod(is + 1, lp->nptensor) == 0;
+        if ((rc = device_step(s, lp->dt, meas, meas ? slot : 0))) return rc;
+        if (meas) {
+            if (rec_steps && slot < max_records) rec_steps[slot] = is + 1;
+            ++slot;
+        }
+    }
+    if (timing) {
+        YK_HIP(hipEventRecord(s->step1, s->stream));
+        YK_HIP(hipEventSynchronize(s->step1));
+        float ms = 0.f;
+        YK_HIP(hipEventElapsedTime(&ms, s->step0, s->step1));
+        s->timing.step_ms += ms;
+        s->timing.steps += lp->nsteps;
+    }
+    if ((rc = timing_collect(s))) return rc;
+    if (count > 0 && records)
+        YK_HIP(hipMemcpyAsync(s->h_meas, s->d_meas, sizeof(MeasOut) * (size_t)count,
+                              hipMemcpyDeviceToHost, s->stream));
+    YK_HIP(hipStreamSynchronize(s->stream));
+    if (count > 0 && records)
+        for (int r = 0; r < count; ++r) to_meas(s->h_meas[r], &records[r]);
+    if (nrec) *nrec = count;
+    if (xl_out)
+        for (int k = 0; k < 3; ++k) xl_out[k] = s->p.xl[k];
     return YK_OK;
 }
 

@@ -10,6 +10,9 @@
 !   yk_accel   accel(do_measurements)   src/int_control.f:52-81
 !   yk_newton  newton(do_measurements)  src/vel_verlet_2.f:18-120
 !   yk_run     nsteps of newton with the md.F measurement cadence (src/md.F:129)
+!   yk_tnorm, yk_cancel_vcom  tnorm, cancel_vcom  src/gen_utils.f:61-107
+!   yk_strain  strain                   src/strain.f:19-82
+!   yk_run_md  the measurement-phase step loop of src/md.F:124-133
 !   yk_batch_* many independent ion systems advanced together (no reference
 !              counterpart: the reference runs one system per process)
 !-------------------------------------------------------------------------------
@@ -25,6 +28,7 @@ module yk_bind
                                            YK_FLAG_TIMING = 4, YK_FLAG_ALLPAIRS = 8, &
                                            YK_FLAG_SW = 16, YK_FLAG_TILE_LISTS = 32
   integer(c_int), parameter, public :: YK_OK = 0
+  real(c_double), parameter, public :: YK_XNOSTRAIN = 1000.0d0
 
   type, bind(C), public :: yk_params_t
     integer(c_int32_t) :: sim_type = YK_PURE_ION
@@ -64,8 +68,16 @@ module yk_bind
     integer(c_int64_t) :: pairs_swept = 0
   end type yk_timing_t
 
+  type, bind(C), public :: yk_loop_t
+    real(c_double) :: dt = 0
+    integer(c_int32_t) :: nsteps = 0, istep0 = 0, nind = 0, nptensor = 0, ncom = 0, ntnorm = 0
+    real(c_double) :: deps(3) = YK_XNOSTRAIN
+    real(c_double) :: xl0(3) = 0
+  end type yk_loop_t
+
   public :: yk_abi_version, yk_create, yk_destroy, yk_set_params, yk_put_state, &
             yk_put_accel, yk_get_state, yk_accel, yk_newton, yk_run, yk_ttot, &
+            yk_tnorm, yk_cancel_vcom, yk_strain, yk_run_md, &
             yk_get_timing, yk_reset_timing, yk_error_message
   public :: yk_batch_create, yk_batch_destroy, yk_batch_footprint, yk_batch_add, &
             yk_batch_remove, yk_batch_set_tuning, yk_batch_set_resort, yk_batch_run, &
@@ -144,6 +156,36 @@ module yk_bind
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: sys
       real(c_double), intent(out) :: eka
+    end function
+
+    integer(c_int) function yk_tnorm(sys) bind(C, name='yk_tnorm')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sys
+    end function
+
+    integer(c_int) function yk_cancel_vcom(sys) bind(C, name='yk_cancel_vcom')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sys
+    end function
+
+    integer(c_int) function yk_strain(sys, dt, deps, xl0, xl_out) bind(C, name='yk_strain')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: sys
+      real(c_double), value :: dt
+      real(c_double), intent(in) :: deps(3), xl0(3)
+      real(c_double), intent(out) :: xl_out(3)
+    end function
+
+    integer(c_int) function yk_run_md(sys, loop, records, rec_steps, max_records, nrec, &
+                                      xl_out) bind(C, name='yk_run_md')
+      import :: c_int, c_int32_t, c_ptr, c_double, yk_loop_t, yk_meas_t
+      type(c_ptr), value :: sys
+      type(yk_loop_t), intent(in) :: loop
+      type(yk_meas_t), intent(inout) :: records(*)
+      integer(c_int32_t), intent(inout) :: rec_steps(*)
+      integer(c_int32_t), value :: max_records
+      integer(c_int32_t), intent(out) :: nrec
+      real(c_double), intent(out) :: xl_out(3)
     end function
 
     integer(c_int) function yk_get_timing(sys, t) bind(C, name='yk_get_timing')

@@ -144,6 +144,52 @@ class YukawaSystem:
         _lib.check(self.lib.yk_ttot(self._h, C.byref(e)))
         return e.value
 
+    # -- step-loop utilities (src/gen_utils.f:61-107, src/strain.f) ---------
+    def tnorm(self):
+        _lib.check(self.lib.yk_tnorm(self._h))
+
+    def cancel_vcom(self):
+        _lib.check(self.lib.yk_cancel_vcom(self._h))
+
+    def strain(self, deps, xl0, dt: float | None = None) -> tuple:
+        """One step of strain rates deps (None or XNOSTRAIN: unstrained axis)
+        from the gage dimensions xl0; returns the new box."""
+        dt = self.dt if dt is None else float(dt)
+        d = np.array([_lib.XNOSTRAIN if e is None else e for e in deps], dtype=np.float64)
+        x0 = np.ascontiguousarray(xl0, dtype=np.float64)
+        box = np.zeros(3)
+        _lib.check(self.lib.yk_strain(self._h, dt, _dptr(d), _dptr(x0), _dptr(box)))
+        self.params = self.params.replace(xl=tuple(float(b) for b in box))
+        return self.params.xl
+
+    def run_md(self, nsteps: int, istep0: int = 0, nind: int = 0, nptensor: int = 0,
+               ncom: int = 0, ntnorm: int = 0, deps=None, xl0=None,
+               dt: float | None = None) -> list:
+        """The measurement-phase step loop of src/md.F:124-133 on the device
+        (cancel_vcom / tnorm / strain / newton cadence); one record per
+        measuring step, with its 'istep'."""
+        dt = self.dt if dt is None else float(dt)
+        lp = _lib.YkLoop()
+        lp.dt, lp.nsteps, lp.istep0 = dt, int(nsteps), int(istep0)
+        lp.nind, lp.nptensor, lp.ncom, lp.ntnorm = int(nind), int(nptensor), int(ncom), int(ntnorm)
+        for k in range(3):
+            e = None if deps is None else deps[k]
+            lp.deps[k] = _lib.XNOSTRAIN if e is None else float(e)
+            lp.xl0[k] = float((self.params.xl if xl0 is None else xl0)[k])
+        cap = max(int(nsteps), 1)
+        recs = (_lib.YkMeas * cap)()
+        steps = (C.c_int32 * cap)()
+        nrec = C.c_int32()
+        box = np.zeros(3)
+        _lib.check(self.lib.yk_run_md(self._h, C.byref(lp), recs, steps, cap, C.byref(nrec),
+                                      _dptr(box)))
+        self.time += dt * nsteps
+        self.params = self.params.replace(xl=tuple(float(b) for b in box))
+        out = [dict(recs[r].as_dict(), istep=int(steps[r])) for r in range(nrec.value)]
+        if out:
+            self._take(recs[nrec.value - 1])
+        return out
+
     # -- timing --------------------------------------------------------------
     def timing(self) -> dict:
         t = _lib.YkTiming()
