@@ -7,18 +7,15 @@
 //
 //   rsqrt64   hardware v_rsq_f64 seed + one third-order Newton-Raphson step
 //             (y' = y + y*e*(1/2 + 3e/8), e = 1 - x*y^2).
-//   exp_neg64 Cody-Waite reduction x = k*ln2 + r, |r| <= ln2/2, with the
-//             two-part ln2 of fdlibm, then the degree-12 Taylor polynomial of
-//             e^r in Horner form and a 2^k scale by ldexp.  Truncation error
-//             |r|^13/13! < 1.8e-16 relative.  Coefficients are computed on
-//             the host as 1/m! and passed in.
-//   exp_neg64t the same function by Tang's table-driven method (P. T. P. Tang,
+//   exp_neg64t e^x (x <= 0) by Tang's table-driven method (P. T. P. Tang,
 //             ACM TOMS 15 (1989) 144): x = (64 q + j) ln2/64 + r with
-//             |r| <= ln2/128 (two-part ln2/64), e^x = 2^q * 2^(j/64) * e^r,
-//             2^(j/64) from a 64-entry table (rounded to double on the host)
-//             that the kernels keep in LDS, e^r by its degree-5 Taylor
-//             polynomial (truncation |r|^6/6! < 3.5e-17).  Seven fewer FP64
-//             operations than exp_neg64 per call.
+//             |r| <= ln2/128 (ln2/64 split in two parts, from fdlibm's split
+//             of ln 2), e^x = 2^q * 2^(j/64) * e^r, 2^(j/64) from a 64-entry
+//             table (rounded to double on the host) that the kernels keep in
+//             LDS, e^r by its degree-5 Taylor polynomial (truncation
+//             |r|^6/6! < 3.5e-17).  Seven fewer FP64 operations per call than
+//             a Cody-Waite reduction to |r| <= ln2/2 with a degree-12
+//             polynomial.
 //   FP32      v_rsq_f32 and v_exp_f32 (2^x) hardware transcendentals.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -31,29 +28,21 @@ namespace yk {
 // arguments so they live in SGPRs (a VOP3 FMA may read one scalar operand)
 // instead of being rematerialised in VGPRs for every pair.
 struct ExpCoeffs {
-    double c[11];                 // Taylor coefficients 1/m!, m = 12 .. 2
-    double log2e, ln2_hi, ln2_lo; // Cody-Waite reduction
-    double three_eighths;         // rsqrt correction
+    double three_eighths;               // rsqrt correction
     double l2e64, ln2_64hi, ln2_64lo;   // table-driven reduction (64/ln2, ln2/64 split)
-    double t5[4];                 // 1/5!, 1/4!, 1/3!, 1/2!
-    double t64[64];               // 2^(j/64), j = 0 .. 63
+    double t5[4];                       // 1/5!, 1/4!, 1/3!, 1/2!
+    double t64[64];                     // 2^(j/64), j = 0 .. 63
 };
 
 inline ExpCoeffs exp_coeffs()
 {
     ExpCoeffs e;
-    double f = 1.0;
-    for (int m = 2; m <= 12; ++m) {
-        f *= m;
-        e.c[12 - m] = 1.0 / f;
-    }
-    e.log2e = 1.4426950408889634074;
-    e.ln2_hi = 6.93147180369123816490e-01;   // fdlibm split of ln 2
-    e.ln2_lo = 1.90821492927058770002e-10;
+    const double ln2_hi = 6.93147180369123816490e-01;   // fdlibm split of ln 2
+    const double ln2_lo = 1.90821492927058770002e-10;
     e.three_eighths = 0.375;
     e.l2e64 = 64.0 * 1.4426950408889634074;
-    e.ln2_64hi = e.ln2_hi / 64.0;   // exact: division by a power of two
-    e.ln2_64lo = e.ln2_lo / 64.0;
+    e.ln2_64hi = ln2_hi / 64.0;   // exact: division by a power of two
+    e.ln2_64lo = ln2_lo / 64.0;
     e.t5[0] = 1.0 / 120.0;
     e.t5[1] = 1.0 / 24.0;
     e.t5[2] = 1.0 / 6.0;
@@ -73,21 +62,7 @@ __device__ __forceinline__ double rsqrt64(double x, const ExpCoeffs &ec)
 }
 
 // e^x for x <= 0 (x = -0 included; large negative x underflows to 0 through
-// ldexp, so no clamp is needed).
-__device__ __forceinline__ double exp_neg64(double x, const ExpCoeffs &ec)
-{
-    double k = __builtin_rint(x * ec.log2e);
-    double r = __builtin_fma(-k, ec.ln2_hi, x);
-    r = __builtin_fma(-k, ec.ln2_lo, r);
-    double p = ec.c[0];
-#pragma unroll
-    for (int m = 1; m < 11; ++m) p = __builtin_fma(p, r, ec.c[m]);
-    p = __builtin_fma(p, r, 1.0);
-    p = __builtin_fma(p, r, 1.0);
-    return __builtin_amdgcn_ldexp(p, (int)k);
-}
-
-// e^x for x <= 0 by the table method; tab = the 64 entries of ec.t64 (LDS).
+// ldexp, so no clamp is needed); tab = the 64 entries of ec.t64 (LDS).
 __device__ __forceinline__ double exp_neg64t(double x, const ExpCoeffs &ec,
                                              const double *__restrict__ tab)
 {
