@@ -1,0 +1,104 @@
+"""The run driver against the reference program over a whole run (needs a GPU).
+
+oracle/_ref/md_omp (the reference built from its sources, one OpenMP
+thread) and yukawamd_amd/fortran/ykmd (our Fortran driver, state on the
+device) read the same runmd.in: a three-species random start, two warm-up
+groups (cancel_vcom and tnorm after each), two measurement groups with
+cancel_vcom every 7 steps, tnorm every 5, an x strain each step, energies
+every 3rd and the pressure tensor every 4th step.  Both start from the same
+bits (tests/test_ykmd_init.py); the device sums forces in another order, so
+the measurement records agree to rounding grown over the run, the final box
+(host arithmetic in both) exactly.
+"""
+import os
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from yukawamd_amd.xvb import read_xvb
+
+ROOT = Path(__file__).resolve().parent.parent
+MD_OMP = ROOT / "oracle" / "_ref" / "md_omp"
+YKMD = ROOT / "yukawamd_amd" / "fortran" / "ykmd"
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not (MD_OMP.exists() and YKMD.exists()),
+                                 reason="reference program or driver not built")]
+
+RUNMD = """&run_parameters
+ sim_type='ion', start='random', tstart=0.0, dt=13.1422,
+ nwgroup=2, nwsteps=5, ngroup=2, ntot=4, nind=3, nptensor=4, ncom=7, ntnorm=5,
+ spec_list(1)%n=900, spec_list(1)%z=8.0, spec_list(1)%a=16.0,
+ spec_list(2)%n=450, spec_list(2)%z=26.0, spec_list(2)%a=56.0,
+ spec_list(3)%n=150, spec_list(3)%z=34.0, spec_list(3)%a=80.0,
+ rho=7.18e-5, kT=0.0617451, deps(1)=2.0e-6,
+ coulomb='screened-coulomb', xlambda=21.3221, rccut=120.0,
+ iseed=2024,
+/
+"""
+
+
+def read_meas(path):
+    rec = np.dtype([("m0", ">i4"), ("istep", ">i4"), ("v", ">f8", (10,)), ("m1", ">i4")])
+    data = np.fromfile(path, dtype=rec)
+    assert (data["m0"] == 84).all() and (data["m1"] == 84).all()
+    return data["istep"], data["v"]
+
+
+def run(cmd, workdir, env_extra=None):
+    workdir.mkdir(parents=True, exist_ok=True)
+    (workdir / "runmd.in").write_text(RUNMD)
+    env = dict(os.environ, OMP_NUM_THREADS="1", **(env_extra or {}))
+    out = subprocess.run(cmd, cwd=workdir, env=env, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    log = next(workdir.glob("md.*.log")).read_text()
+    steps, vals = read_meas(next(workdir.glob("md.*.meas")))
+    final = max(workdir.glob("md.*.xv8b"), key=lambda p: p.name)
+    return log, steps, vals, read_xvb(final)
+
+
+def lines(log, tag):
+    return [ln.split() for ln in log.splitlines() if re.match(rf"^{tag}\s+\d", ln)]
+
+
+def test_driver_run_matches_reference(tmp_path):
+    rlog, rsteps, rvals, rfin = run([str(MD_OMP)], tmp_path / "ref")
+    glog, gsteps, gvals, gfin = run([str(YKMD)], tmp_path / "gpu")
+    # the measuring steps of both cadences, and one record each
+    assert list(gsteps) == list(rsteps) == [3, 4, 6, 8, 9, 12, 15, 16, 18, 20, 21, 24]
+    np.testing.assert_allclose(gvals[:, 0], rvals[:, 0], rtol=0, atol=1e-6)   # time
+    np.testing.assert_allclose(gvals[:, 1:], rvals[:, 1:], rtol=1e-9,
+                               atol=1e-12 * np.abs(rvals[:, 1:]).max())
+    # log lines: the same 'e', 'p' and 's' lines to the printed digits
+    for tag in ("e", "p", "s"):
+        le, lr = lines(glog, tag), lines(rlog, tag)
+        assert len(le) == len(lr) > 0, tag
+        for a, b in zip(le, lr):
+            assert a[:3] == b[:3], (tag, a, b)
+            np.testing.assert_allclose(np.array(a[3:], float), np.array(b[3:], float),
+                                       rtol=2e-8, atol=1e-300)
+    for key in ("Initial Vtot", "Final Vtot", "Final dimensions", "average potential"):
+        a = [ln for ln in glog.splitlines() if key in ln]
+        b = [ln for ln in rlog.splitlines() if key in ln]
+        assert len(a) == len(b) == 1, key
+        va = np.array(re.findall(r"-?\d\.\d+E[+-]\d+|-?\d+\.\d+", a[0], re.I), float)
+        vb = np.array(re.findall(r"-?\d\.\d+E[+-]\d+|-?\d+\.\d+", b[0], re.I), float)
+        np.testing.assert_allclose(va, vb, rtol=2e-8)
+    # end-of-run checkpoint: the strained box exactly, the state to rounding
+    assert gfin["xl"] == rfin["xl"] and gfin["xl"] != (rfin["xl"][0],) * 3
+    L = np.array(rfin["xl"])
+    dx = np.abs(gfin["x"] - rfin["x"])
+    dx = np.minimum(dx, L - dx)
+    assert dx.max() <= 1e-9 * L.max()
+    np.testing.assert_allclose(gfin["v"], rfin["v"], rtol=1e-7, atol=1e-10 * np.abs(rfin["v"]).max())
+
+
+def test_driver_mixed_precision_close_to_reference(tmp_path):
+    rlog, rsteps, rvals, _ = run([str(MD_OMP)], tmp_path / "ref")
+    glog, gsteps, gvals, _ = run([str(YKMD)], tmp_path / "gpu", {"YK_PRECISION": "mixed"})
+    assert list(gsteps) == list(rsteps)
+    # ev, ek, px of the mixed kernel stay within 1e-5 of the FP64 reference
+    np.testing.assert_allclose(gvals[:, 1:4], rvals[:, 1:4], rtol=1e-5)

@@ -1,0 +1,349 @@
+!-------------------------------------------------------------------------------
+! ykmd.f90 -- MD run driver over the GPU library, with the state resident on
+! the device.
+!
+! Reads the reference's run-parameter file (runmd.in, namelist
+! run_parameters with the names of src/input1.F:28-53, so the same file
+! serves both programs), derives the run parameters as src/input1.F and
+! src/initconfig.f do, builds the initial state of a start='random' run with
+! the reference's generator (irnd = 0 or 4: MT19937; yk_host.f90) or reads an
+! 'xv8b' start file, and runs the schedule of src/md.F:66-205 on the GPU:
+! initial accel and ttot, nwgroup warm-up groups (nwsteps steps, then
+! cancel_vcom and tnorm), ngroup measurement groups of ntot*nind steps with
+! the ncom / ntnorm / strain / nind / nptensor cadences (one yk_run_md call
+! per group), block statistics, the final accel and ttot.
+!
+! Output, in the reference's formats: the 'e', 'p' and 's' lines and the
+! initial / final lines of the log (md.<tend>.log), the binary measurement
+! records (md.<tend>.meas, src/md.F:151), the initial configuration
+! (md.<tstart>.xv8b, src/initconfig.f:385-390) and the end-of-run checkpoint
+! (md.<time>.xv8b, src/md_utils.f:19-38).  Configuration outputs every
+! nout steps and intermediate checkpoints are not written.
+!
+! YK_PRECISION=mixed selects the mixed-precision kernels; the mixture
+! kernel's last-particle behaviour of the reference (src/int_ion_mix.f:158)
+! is reproduced unless YK_FIX_F1=1.  `ykmd --init-only` stops after writing
+! the initial configuration (no device needed).
+!-------------------------------------------------------------------------------
+program ykmd
+  use iso_c_binding
+  use yk_bind
+  use yk_host
+  implicit none
+
+  ! run parameters (names and defaults of src/md_globals.f:36-135)
+  character(20) :: sim_type = ''
+  character(256) :: start = 'md.in.xv82'
+  real(dp) :: tstart = -1.0d0, dt = 0.0d0
+  integer :: nwgroup = 0, nwsteps = 0, ngroup = 0, ntot = 0, nind = 0, nptensor = 0
+  integer :: ncom = 0, ntnorm = 0, ngofr = 0, nbin = 0, nsbin = 0, nckpt = 0, nout = 0
+  real(dp) :: qmin = 0.0d0, dq = 0.0d0
+  character(6) :: ftype = 'x4b'
+  logical :: append = .true.
+  character(256) :: spec_file = ''
+  type(species_t) :: spec_list(MAXSPEC)
+  real(dp) :: rho = 0.0d0, aspect(3) = 0.0d0, deps(3) = XNOSTRAIN, stretch(3) = 1.0d0
+  real(dp) :: rccut = 1.0d50, kT = 0.0d0, xmass = 931.0d0
+  character(20) :: coulomb = '', nuclear = ''
+  real(dp) :: xlambda = -1.0d0, frp = 1.0d0
+  real(dp) :: bfield = 0.0d0, efield = 0.0d0, q0 = 0.0d0, w0 = 0.0d0, tref = 0.0d0
+  real(dp) :: xpacket = 1.25d0, aa = 110.0d0, bb = -26.0d0, cc = 24.0d0, rncut = 0.0d0
+  integer :: irnd = 0, iseed = 0
+  real(dp) :: rmax = 0.0d0
+  namelist /run_parameters/ sim_type, start, tstart, dt, nwgroup, nwsteps, ngroup, ntot, &
+                            nind, nptensor, ncom, ntnorm, ngofr, nbin, nsbin, qmin, dq, &
+                            nckpt, nout, ftype, append, spec_file, spec_list, rho, aspect, &
+                            deps, stretch, rccut, kT, xmass, coulomb, xlambda, frp, bfield, &
+                            efield, q0, w0, tref, nuclear, xpacket, aa, bb, cc, rncut, &
+                            irnd, iseed, rmax
+
+  integer :: n, nspec, i, j, k, ig, ios, irtn, istep, nrec, r, u8, u9
+  real(dp), allocatable :: x(:, :), v(:, :), a(:, :), zii(:), aii(:)
+  real(dp) :: xlf, xl0(3), xl(3), ztot, xmuc, time, tend, ev, ek, px, e, pp(3, 3)
+  real(dp) :: evave, ev2ave, pave, p2ave, eva, ev2a, pa, p2a, dev, dp_, devave, dpave
+  real(dp) :: x1, x2, rhof
+  character(40) :: suffix
+  character(256) :: env, arg
+  type(yk_params_t) :: p
+  type(yk_meas_t) :: m
+  type(yk_meas_t), allocatable :: recs(:)
+  integer(c_int32_t), allocatable :: rsteps(:)
+  type(yk_loop_t) :: lp
+  type(c_ptr) :: h
+  integer(c_int32_t) :: nrec32
+  real(c_double) :: box(3), eka
+
+  open(10, file='runmd.in', status='old', form='formatted', iostat=ios)
+  if (ios /= 0) call fail('cannot open runmd.in')
+  read(10, nml=run_parameters, iostat=ios)
+  if (ios /= 0) call fail('error reading namelist run_parameters from runmd.in')
+  close(10)
+
+  ! species from the internal list, in list order (src/initconfig.f:117-141)
+  if (trim(spec_file) /= '' .and. trim(spec_file) /= 'internal') &
+    call fail('only an internal spec_list is supported (spec_file)')
+  n = 0
+  nspec = 0
+  do k = 1, MAXSPEC
+    if (spec_list(k)%n == 0 .and. spec_list(k)%z == 0.0d0 .and. spec_list(k)%a == 0.0d0) exit
+    n = n + spec_list(k)%n
+    nspec = k
+  end do
+  if (n <= 0) call fail('no particles in spec_list')
+  allocate(zii(0:n - 1), aii(0:n - 1), a(3, 0:n - 1))
+  a = 0.0d0
+  ztot = 0.0d0
+  j = 0
+  do k = 1, nspec
+    ztot = ztot + spec_list(k)%n * spec_list(k)%z
+    do i = 1, spec_list(k)%n
+      zii(j) = spec_list(k)%z
+      aii(j) = spec_list(k)%a
+      j = j + 1
+    end do
+  end do
+
+  ! random-number seed (src/input1.F:174-177): MT19937 only
+  if (irnd /= 0 .and. irnd /= 4) &
+    call fail('only irnd = 0 or 4 (Mersenne Twister) is supported')
+  call mt_seed(iseed)
+
+  ! initial configuration (src/initconfig.f:202-226, :331-373)
+  select case (trim(start))
+  case ('random')
+    if (tstart < 0.0d0) call fail('tstart not specified')
+    call random_box(n, rho, aspect, xlf, xl0)
+    xl = xl0
+    allocate(x(3, 0:n - 1), v(3, 0:n - 1))
+    call random_positions(n, xl, x)
+    call maxwell_velocities(n, aii, kT, xmass, v)
+    call host_cancel_vcom(n, v, aii, xmass, kT)
+    time = tstart
+  case default
+    call read_xvb(trim(start), time, xl, k, x, v, irtn)
+    if (irtn /= 0) call fail('cannot read start file (xv8b expected): '//trim(start))
+    if (k /= n) call fail('start file holds another particle count than spec_list')
+    if (tstart >= 0.0d0) time = tstart
+    if (aspect(1) * aspect(2) * aspect(3) == 0.0d0) aspect = 1.0d0
+    xlf = (real(real(n), dp) / (aspect(1) * aspect(2) * aspect(3) * rho))**(1.0d0 / 3.0d0)
+    xl0 = aspect * xlf
+    stretch = xl / xl0
+  end select
+  if (trim(start) == 'random') then
+    pp = 0.0d0
+    write(suffix, '(i5.5,i6.6)') int(time / 1000000.d0), int(mod(time, 1000000.d0))
+    call write_xvb('md.'//trim(suffix)//'.xv8b', 'xv8b', sim_type, time, xl, 0.0d0, &
+                   0.0d0, 0.0d0, pp, n, x, v, a)
+  end if
+  call get_command_argument(1, arg)
+  if (trim(arg) == '--init-only') stop
+
+  ! derived parameters (src/input1.F:184-196, :222, :255-311)
+  xl = stretch * xl0
+  if (trim(sim_type) == 'ion') then
+    if (nspec == 1) then
+      sim_type = 'pure-ion'
+    else
+      sim_type = 'ion-mixture'
+    end if
+  end if
+  xmuc = 0.0d0
+  if (trim(coulomb) == 'screened-coulomb') then
+    if (xlambda > 0.0d0) then
+      xmuc = 1.0d0 / xlambda
+    else
+      xmuc = thomas_fermi_mu((ztot * rho) / n)
+    end if
+  end if
+  if (trim(nuclear) == 'HPP' .and. rncut == 0.0d0) rncut = auto_rncut(aa, bb, cc, xpacket)
+
+  ! the device system
+  select case (trim(sim_type))
+  case ('pure-ion')
+    p%sim_type = YK_PURE_ION
+    p%zi = zii(0)
+    p%ai = aii(0)
+  case ('ion-mixture')
+    p%sim_type = YK_ION_MIXTURE
+  case ('nucleon')
+    p%sim_type = YK_NUCLEON
+  case default
+    call fail('sim_type not supported: '//trim(sim_type))
+  end select
+  p%coulomb_on = merge(1, 0, trim(coulomb) == 'screened-coulomb')
+  p%nuclear_on = merge(1, 0, trim(nuclear) == 'HPP')
+  p%xl = xl
+  p%xmuc = xmuc
+  p%rccut = rccut
+  p%frp = frp
+  p%vc = ALPHA * HBARC
+  p%xmass = xmass
+  p%rho = rho
+  p%kT = kT
+  p%aa = aa
+  p%bb = bb
+  p%cc = cc
+  p%xpacket = xpacket
+  p%rncut = rncut
+  p%bfield = bfield
+  call get_environment_variable('YK_PRECISION', env)
+  if (trim(env) == 'mixed') p%precision = YK_MIXED
+  call get_environment_variable('YK_FIX_F1', env)
+  if (trim(env) /= '1') p%flags = ior(p%flags, YK_FLAG_REF_F1)
+  call check(yk_create(p, int(n, c_int32_t), zii, aii, h), 'yk_create')
+  call check(yk_put_state(h, x, v), 'yk_put_state')
+
+  ! log file (src/md.F:44-55)
+  tend = tstart + dt * (nwgroup * nwsteps + ngroup * ntot * nind)
+  write(suffix, '(i5.5,i6.6)') int(tend / 1000000.d0), nint(mod(tend, 1000000.d0))
+  open(newunit=u8, file='md.'//trim(suffix)//'.log', form='formatted', status='unknown')
+  write(u8, '(a)') '  ykmd: MD run on the GPU library (libyukawa_hip.so)'
+  write(u8, '(a,a)') '  sim_type          = ', trim(sim_type)
+  write(u8, '(a,i10)') '  n                 = ', n
+  write(u8, '(a,3(2x,f17.9),a)') '  config dimensions =', xl, ' fm'
+  write(u8, '(a,f13.8,a)') '  screening length  = ', merge(1.0d0 / xmuc, 0.0d0, xmuc > 0.0d0), ' fm'
+
+  ! initial state (src/md.F:66-77)
+  call check(yk_accel(h, 1_c_int32_t, m), 'yk_accel')
+  call check(yk_ttot(h, eka), 'yk_ttot')
+  ek = eka
+  e = m%ev + ek
+  write(u8, 10050) '  Initial Vtot, Ttot, E, P = ', m%ev, ek, e, m%px
+10050 format(/, a, 4(2x, es16.9))
+
+  ! warm-up groups (src/md.F:80-101)
+  istep = 0
+  if (nwgroup >= 1) write(u8, '(/a)') '  Warmup phase'
+  allocate(recs(max(1, ntot * nind)), rsteps(max(1, ntot * nind)))
+  do ig = 1, nwgroup
+    call check(yk_run(h, dt, int(nwsteps, c_int32_t), 0_c_int32_t, recs), 'yk_run')
+    do k = 1, nwsteps
+      time = time + dt
+    end do
+    istep = istep + nwsteps
+    call check(yk_cancel_vcom(h), 'yk_cancel_vcom')
+    call check(yk_tnorm(h), 'yk_tnorm')
+    write(u8, 10082) istep, time
+  end do
+10082 format(2x, i7, 1x, f14.2, '   warm up')
+
+  ! measurement groups (src/md.F:105-168)
+  if (ngroup >= 1) then
+    write(u8, 10090)
+    write(suffix, '(i5.5,i6.6)') int(tend / 1000000.d0), nint(mod(tend, 1000000.d0))
+    open(newunit=u9, file='md.'//trim(suffix)//'.meas', form='unformatted', status='unknown')
+  end if
+10090 format('  Measurement phase', /, &
+             'e   step     time            Vtot              Ttot               E              pressure', /, &
+             'p   step     time            P_xx              P_xy              P_xz              P_yy              P_yz              P_zz')
+  evave = 0.0d0
+  ev2ave = 0.0d0
+  pave = 0.0d0
+  p2ave = 0.0d0
+  istep = 0
+  nrec = 0
+  ev = m%ev
+  ek = eka
+  px = m%px
+  do ig = 1, ngroup
+    eva = 0.0d0
+    ev2a = 0.0d0
+    pa = 0.0d0
+    p2a = 0.0d0
+    lp%dt = dt
+    lp%nsteps = int(ntot * nind, c_int32_t)
+    lp%istep0 = int(istep, c_int32_t)
+    lp%nind = int(nind, c_int32_t)
+    lp%nptensor = int(nptensor, c_int32_t)
+    lp%ncom = int(ncom, c_int32_t)
+    lp%ntnorm = int(ntnorm, c_int32_t)
+    lp%deps = deps
+    lp%xl0 = xl0
+    call check(yk_run_md(h, lp, recs, rsteps, int(size(recs), c_int32_t), nrec32, box), &
+               'yk_run_md')
+    nrec = nrec32
+    xl = box
+    r = 0
+    do k = 1, ntot * nind
+      time = time + dt
+      istep = istep + 1
+      if (mdmod(istep, nind) /= 0 .and. mdmod(istep, nptensor) /= 0) cycle
+      r = r + 1
+      if (r > nrec .or. rsteps(r) /= istep) call fail('measurement records out of step')
+      ev = recs(r)%ev
+      ek = recs(r)%ek
+      px = recs(r)%px
+      if (mdmod(istep, nind) == 0) then
+        eva = eva + ev
+        ev2a = ev2a + ev**2
+        pa = pa + px
+        p2a = p2a + px**2
+        e = ek + ev
+        write(u8, 10101) 'e', istep, time, ev, ek, e, px
+      end if
+      if (mdmod(istep, nptensor) == 0) write(u8, 10101) 'p', istep, time, recs(r)%pp
+      write(u9) istep, time, ev, ek, px, recs(r)%pp
+    end do
+    call block_stats(eva, ev2a, ntot, dev, evave, ev2ave)
+    call block_stats(pa, p2a, ntot, dp_, pave, p2ave)
+    write(u8, 10110) ig, time, eva, dev, pa, dp_
+  end do
+10101 format(a1, 1x, i7, 1x, f14.2, 6(2x, es16.9))
+10110 format('s', i4, 1x, f14.2, 2x, 1pe16.9, 2x, 1pe16.9, 2x, 16x, 2x, 1pe16.9, 2x, 1pe16.9)
+
+  ! run averages and the end of the run (src/md.F:170-205)
+  x1 = 0.0d0
+  x2 = 0.0d0
+  if (ngroup >= 1) then
+    call block_stats(evave, ev2ave, ngroup, devave, x1, x2)
+    call block_stats(pave, p2ave, ngroup, dpave, x1, x2)
+    write(u8, 10200) '  average potential = ', evave, ' +/-', devave
+    write(u8, 10200) '  average pressure  = ', pave, ' +/-', dpave
+    close(u9)
+  end if
+10200 format(2(a, 1pe16.9))
+  ! end-of-run checkpoint, with the last measured energies (src/md.F:181)
+  if (istep > 0) then
+    call check(yk_get_state(h, x, v, a), 'yk_get_state')
+    pp = 0.0d0
+    if (nrec > 0) then
+      pp(1, 1) = recs(nrec)%pp(1); pp(1, 2) = recs(nrec)%pp(2); pp(1, 3) = recs(nrec)%pp(3)
+      pp(2, 1) = recs(nrec)%pp(2); pp(2, 2) = recs(nrec)%pp(4); pp(2, 3) = recs(nrec)%pp(5)
+      pp(3, 1) = recs(nrec)%pp(3); pp(3, 2) = recs(nrec)%pp(5); pp(3, 3) = recs(nrec)%pp(6)
+    end if
+    write(suffix, '(i5.5,i6.6)') int(time / 1000000.d0), nint(mod(time, 1000000.d0))
+    call write_xvb('md.'//trim(suffix)//'.xv8b', 'xv8b', sim_type, time, xl, ev, ek, px, pp, &
+                   n, x, v, a)
+  end if
+  call check(yk_accel(h, 1_c_int32_t, m), 'yk_accel')
+  call check(yk_ttot(h, eka), 'yk_ttot')
+  ek = eka
+  e = m%ev + ek
+  stretch = xl / xl0
+  aspect = stretch * aspect
+  rhof = real(real(n), dp) / (xl(1) * xl(2) * xl(3))
+  write(u8, 10206) xl, aspect, stretch, rhof
+10206 format('  Final dimensions  =', 3(2x, f17.9), ' fm', / &
+             '  Final aspect      =', 3(2x, f17.9), / &
+             '  Final stretch     =', 3(2x, f17.9), / &
+             '  Final density     =', 1x, es15.8, ' fm-3')
+  write(u8, 10210) '  Final Vtot, Ttot, E, P   = ', m%ev, ek, e, m%px
+10210 format(a, 4(2x, es16.9))
+  close(u8)
+  call yk_destroy(h)
+
+contains
+
+  subroutine check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: what
+    if (rc /= YK_OK) call fail(what//': '//yk_error_message())
+  end subroutine check
+
+  subroutine fail(msg)
+    character(*), intent(in) :: msg
+    write(0, '(a,a)') 'ykmd: ', msg
+    error stop 1
+  end subroutine fail
+
+end program ykmd
