@@ -113,6 +113,7 @@ struct yk_batch {
     int64_t clist_cap = 0;
     std::vector<char> list_used;   // per LIST_UNIT
     int2 *ccounts = nullptr;       // [cap/16][NSEG]
+    double *etab_g = nullptr;      // [EXP_TN] 2^(j/N) table of the FP64 exp
     double4 *xref = nullptr;       // positions at each replica's last list build
     unsigned long long *rep_maxdisp = nullptr;   // [2 max_rep] two largest displacements^2 (bits)
     int nlist = 0;                 // active list replicas
@@ -441,7 +442,8 @@ int enqueue_step(yk_batch *b)
     const size_t lds = sizeof(double4) * TILE * b->waves;
     hipLaunchKernelGGL((k_batch_pairs<MEAS>), dim3(persistent_blocks(b)), dim3(TILE * b->waves),
                        lds, b->stream, b->pos, b->box, b->reps, b->state, b->items, b->nitems,
-                       b->queue, ec, MODE == 0 ? 1 : 0, b->clist, b->ccounts, b->part);
+                       b->queue, ec, b->etab_g, MODE == 0 ? 1 : 0, b->clist, b->ccounts,
+                       b->part);
     hipLaunchKernelGGL((k_batch_finalize<MEAS, MODE>), slots, dim3(BLOCK), 0, b->stream, b->part,
                        b->tile_items, b->pos, b->den, b->aii, b->orig, b->rep, b->reps,
                        b->state, b->span, b->cap, b->acc, b->vel, b->vold, b->blockpart);
@@ -687,6 +689,12 @@ int yk_batch_create(int64_t capacity, int32_t max_replicas, int32_t max_records,
     YB_HIPB(hipMalloc(&b->queue, sizeof(unsigned)));
     YB_HIPB(hipMalloc(&b->xref, sizeof(double4) * nn));
     YB_HIPB(hipMalloc(&b->ccounts, sizeof(int2) * (nn / CL) * NSEG));
+    {
+        std::vector<double> t(EXP_TN);
+        exp_table(t.data());
+        YB_HIPB(hipMalloc(&b->etab_g, sizeof(double) * EXP_TN));
+        YB_HIPB(hipMemcpy(b->etab_g, t.data(), sizeof(double) * EXP_TN, hipMemcpyHostToDevice));
+    }
     YB_HIPB(hipMalloc(&b->rep_maxdisp, 2 * sizeof(unsigned long long) * (size_t)max_replicas));
     YB_HIPB(hipMalloc(&b->keys, sizeof(unsigned long long) * nn));
     YB_HIPB(hipMalloc(&b->keys2, sizeof(unsigned long long) * nn));
@@ -743,7 +751,7 @@ void yk_batch_destroy(yk_batch *b)
                     (void *)b->pos2, (void *)b->vel2, (void *)b->vold2, (void *)b->acc2,
                     (void *)b->aii2, (void *)b->den2, (void *)b->orig2, (void *)b->xref,
                     (void *)b->ccounts, (void *)b->rep_maxdisp, (void *)b->clist,
-                    (void *)b->list_tiles})
+                    (void *)b->list_tiles, (void *)b->etab_g})
         if (p) (void)hipFree(p);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);

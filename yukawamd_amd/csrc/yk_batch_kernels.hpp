@@ -245,6 +245,9 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
             }
             pj.w = stage_w<KIND, PREC>(pj.w);
             sw[lane] = pj;
+        } else {
+            const double q = __builtin_nan("");   // past the share: rejected by every pair test
+            sw[lane] = make_double4(q, q, q, 0.0);
         }
         const bool gen = __ballot(genl) != 0ull;
         __builtin_amdgcn_wave_barrier();
@@ -253,7 +256,6 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
         pj_cur = pos[rc.off + (e_cur & SLOT_MASK)];
         e_nxt = sl.entry(lo + b0 + 2 * CL + gl, g_nxt);
         const int kb = min(CL, iters - b0);
-        const int mycount = share - b0;
         const double4 *gs = sw + g * CL;
         if (!gen) {
             double4 pn = gs[0];
@@ -262,7 +264,7 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
                 const double4 pj = pn;
                 pn = gs[min(k + 1, CL - 1)];
                 interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w, pi.x - pj.x, pi.y - pj.y,
-                                                 pi.z - pj.z, k >= mycount, acc);
+                                                 pi.z - pj.z, false, acc);
             }
         } else {
 #pragma unroll 2
@@ -272,7 +274,7 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
                                                  min_image(pi.x - pj.x, c.L[0], c.invL[0]),
                                                  min_image(pi.y - pj.y, c.L[1], c.invL[1]),
                                                  min_image(pi.z - pj.z, c.L[2], c.invL[2]),
-                                                 k >= mycount, acc);
+                                                 false, acc);
             }
         }
     }
@@ -295,16 +297,17 @@ __global__ __launch_bounds__(512) void k_batch_pairs(const double4 *__restrict__
                               const RepConsts *__restrict__ reps,
                               const RepState *__restrict__ state,
                               const BatchItem *__restrict__ items, int nitems,
-                              unsigned *__restrict__ queue, ExpCoeffs ec, int accel_pass,
+                              unsigned *__restrict__ queue, ExpCoeffs ec,
+                              const double *__restrict__ etab_g, int accel_pass,
                               const unsigned *__restrict__ clist,
                               const int2 *__restrict__ ccounts, double *__restrict__ part)
 {
     extern __shared__ double4 sj_dyn[];
-    __shared__ double etab_s[8][64];   // up to 8 waves per workgroup
+    __shared__ double etab_s[EXP_TN];   // FP64 replicas' exp table, one per workgroup
     const int lane = threadIdx.x & (TILE - 1);
     double4 *sw = sj_dyn + (threadIdx.x / TILE) * TILE;
-    double *etab = etab_s[threadIdx.x / TILE];
-    load_exp_table(etab, ec, lane);
+    load_exp_table(etab_s, etab_g);
+    const double *etab = etab_s;
     constexpr int NC = MEAS ? 11 : 3;
     for (;;) {
         unsigned k = 0;

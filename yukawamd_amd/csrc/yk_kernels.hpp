@@ -56,7 +56,8 @@ struct ForceConsts {
     float mul2e_c;
     int n, ntiles, coul_on, nuc_on;
     ExpCoeffs ec;
-    const double *etab;   // the wave's LDS copy of ec.t64 (set in the kernel)
+    const double *etab_g; // the 2^(j/N) table of exp_neg64t (device memory)
+    const double *etab;   // the workgroup's LDS copy of it (set in the kernel)
 };
 
 struct StepConsts {
@@ -236,12 +237,12 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const double4 *__restrict__ pos
 {
     int nswept = 0;   // source tiles this wave evaluated
     __shared__ double4 sj[STAGE];
-    __shared__ double etab_s[BLOCK / TILE][64];
     const int n = c.n;
     const int lane = threadIdx.x & (TILE - 1);
     if constexpr (PREC == 0) {
-        load_exp_table(etab_s[threadIdx.x / TILE], c.ec, lane);
-        c.etab = etab_s[threadIdx.x / TILE];
+        __shared__ double etab_s[EXP_TN];
+        load_exp_table(etab_s, c.etab_g);
+        c.etab = etab_s;
     }
     const int itile = __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / TILE) +
                                                      (int)(threadIdx.x / TILE));
@@ -501,7 +502,6 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_list(const double4 *__restrict_
                                                       unsigned long long *__restrict__ swept)
 {
     __shared__ double4 sj[BLOCK / TILE][TILE];
-    __shared__ double etab_s[BLOCK / TILE][64];
     const int n = c.n;
     // the build kernel (previous launch) has consumed the displacement flag
     if (blockIdx.x == 0 && threadIdx.x == 0 && lists_stale(maxdisp, skin)) {
@@ -511,8 +511,9 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_list(const double4 *__restrict_
     const int lane = threadIdx.x & (TILE - 1);
     const int wv = threadIdx.x / TILE;
     if constexpr (PREC == 0) {
-        load_exp_table(etab_s[wv], c.ec, lane);
-        c.etab = etab_s[wv];
+        __shared__ double etab_s[EXP_TN];
+        load_exp_table(etab_s, c.etab_g);
+        c.etab = etab_s;
     }
     const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / TILE) + wv);
     const int t = item / M, m = item % M;
@@ -993,7 +994,6 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
                                                        unsigned long long *__restrict__ swept)
 {
     __shared__ double4 sj[BLOCK / TILE][TILE];
-    __shared__ double etab_s[BLOCK / TILE][64];
     const int n = c.n;
     // the build kernels (previous launches) have consumed the displacement flag
     if (blockIdx.x == 0 && threadIdx.x == 0 && lists_stale(maxdisp, skin)) {
@@ -1003,8 +1003,9 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
     const int lane = threadIdx.x & (TILE - 1);
     const int wv = threadIdx.x / TILE;
     if constexpr (PREC == 0) {
-        load_exp_table(etab_s[wv], c.ec, lane);
-        c.etab = etab_s[wv];
+        __shared__ double etab_s[EXP_TN];
+        load_exp_table(etab_s, c.etab_g);
+        c.etab = etab_s;
     }
     const int g = lane / CL, gl = lane % CL;
     const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / TILE) + wv);
@@ -1073,6 +1074,11 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
             }
             pj.w = stage_w<KIND, PREC>(pj.w);
             sw[lane] = pj;
+        } else {
+            // past this group's share: a NaN source, which every pair test
+            // rejects (NaN r^2 fails the cutoff), so no lane count is compared
+            const double q = __builtin_nan("");
+            sw[lane] = make_double4(q, q, q, 0.0);
         }
         // a batch holding a general entry takes the per-pair minimum image for
         // all of its pairs (a no-op for the already shifted ones)
@@ -1083,7 +1089,6 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
         pj_cur = pos[e_cur & SLOT_MASK];
         e_nxt = entry_at(lo + b0 + 2 * CL + gl, g_nxt);
         const int kb = min(CL, iters - b0);
-        const int mycount = share - b0;   // entries of this group in the batch (may be <= 0)
         const double4 *gs = sw + g * CL;
         if (!gen) {
             double4 pn = gs[0];
@@ -1092,7 +1097,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
                 const double4 pj = pn;
                 pn = gs[min(k + 1, CL - 1)];
                 interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w, pi.x - pj.x, pi.y - pj.y,
-                                                 pi.z - pj.z, k >= mycount, acc);
+                                                 pi.z - pj.z, false, acc);
             }
         } else {
 #pragma unroll 2
@@ -1102,7 +1107,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
                                                  min_image(pi.x - pj.x, c.L[0], c.invL[0]),
                                                  min_image(pi.y - pj.y, c.L[1], c.invL[1]),
                                                  min_image(pi.z - pj.z, c.L[2], c.invL[2]),
-                                                 k >= mycount, acc);
+                                                 false, acc);
             }
         }
         __builtin_amdgcn_wave_barrier();

@@ -87,6 +87,7 @@ struct yk_system {
     double *blockpart = nullptr;   // [nblocks][NQ]
     MeasOut *d_meas = nullptr;     // device scalar records
     double *d_vstate = nullptr;    // [4] vcom, fac of the velocity utilities
+    double *etab_g = nullptr;      // [EXP_TN] 2^(j/N) table of the FP64 exp
     int meas_cap = 0;
     // sort scratch
     unsigned *keys = nullptr, *keys2 = nullptr;
@@ -200,6 +201,7 @@ ForceConsts force_consts(const yk_system *s)
     c.coul_on = p.coulomb_on ? 1 : 0;
     c.nuc_on = p.nuclear_on ? 1 : 0;
     c.ec = exp_coeffs();
+    c.etab_g = s->etab_g;
     double lmax = std::max(p.xl[0], std::max(p.xl[1], p.xl[2]));
     c.margin = 1e-12 * lmax;
     for (int k = 0; k < 3; ++k) c.halfL_m[k] = 0.5 * p.xl[k] - 4.0 * c.margin;
@@ -724,7 +726,7 @@ void release(yk_system *s)
                       (void *)s->vold, (void *)s->vold2, (void *)s->acc, (void *)s->acc2,
                       (void *)s->aii, (void *)s->aii2, (void *)s->den, (void *)s->den2,
                       (void *)s->orig, (void *)s->orig2, (void *)s->box, (void *)s->part,
-                      (void *)s->blockpart, (void *)s->d_meas, (void *)s->d_vstate,
+                      (void *)s->blockpart, (void *)s->d_meas, (void *)s->d_vstate, (void *)s->etab_g,
                       (void *)s->keys,
                       (void *)s->keys2, (void *)s->vals, (void *)s->vals2, s->sort_tmp,
                       (void *)s->list, (void *)s->counts, (void *)s->xref,
@@ -838,6 +840,12 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
     YK_HIPB(hipMalloc(&s->part, sizeof(double) * s->part_doubles));
     YK_HIPB(hipMalloc(&s->blockpart, sizeof(double) * NQ * (size_t)s->nblocks));
     YK_HIPB(hipMalloc(&s->d_vstate, 4 * sizeof(double)));
+    {
+        std::vector<double> t(EXP_TN);
+        exp_table(t.data());
+        YK_HIPB(hipMalloc(&s->etab_g, sizeof(double) * EXP_TN));
+        YK_HIPB(hipMemcpy(s->etab_g, t.data(), sizeof(double) * EXP_TN, hipMemcpyHostToDevice));
+    }
     YK_HIPB(hipMalloc(&s->keys, sizeof(unsigned) * nn));
     YK_HIPB(hipMalloc(&s->keys2, sizeof(unsigned) * nn));
     YK_HIPB(hipMalloc(&s->vals, sizeof(int) * nn));

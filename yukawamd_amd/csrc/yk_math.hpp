@@ -8,14 +8,15 @@
 //   rsqrt64   hardware v_rsq_f64 seed + one third-order Newton-Raphson step
 //             (y' = y + y*e*(1/2 + 3e/8), e = 1 - x*y^2).
 //   exp_neg64t e^x (x <= 0) by Tang's table-driven method (P. T. P. Tang,
-//             ACM TOMS 15 (1989) 144): x = (64 q + j) ln2/64 + r with
-//             |r| <= ln2/128 (ln2/64 split in two parts, from fdlibm's split
-//             of ln 2), e^x = 2^q * 2^(j/64) * e^r, 2^(j/64) from a 64-entry
-//             table (rounded to double on the host) that the kernels keep in
-//             LDS, e^r by its degree-5 Taylor polynomial (truncation
-//             |r|^6/6! < 3.5e-17).  Seven fewer FP64 operations per call than
-//             a Cody-Waite reduction to |r| <= ln2/2 with a degree-12
-//             polynomial.
+//             ACM TOMS 15 (1989) 144): x = (1024 q + j) ln2/1024 + r with
+//             |r| <= ln2/2048 (ln2/1024 split in two parts, from fdlibm's
+//             split of ln 2), e^x = 2^q * 2^(j/1024) * e^r, 2^(j/1024) from a
+//             1024-entry table (rounded to double on the host; the kernels
+//             keep it in LDS), e^r by its degree-3 Taylor polynomial
+//             (truncation |r|^4/4! < 5.5e-16 relative, a few ulp; the force
+//             and energy parity needs 1e-12 / 1e-13).  Nine fewer FP64
+//             operations per call than a Cody-Waite reduction to
+//             |r| <= ln2/2 with a degree-12 polynomial.
 //   FP32      v_rsq_f32 and v_exp_f32 (2^x) hardware transcendentals.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -27,11 +28,13 @@ namespace yk {
 // Constants of the FP64 special functions.  Kernels receive them as kernel
 // arguments so they live in SGPRs (a VOP3 FMA may read one scalar operand)
 // instead of being rematerialised in VGPRs for every pair.
+constexpr int EXP_TBITS = 10;
+constexpr int EXP_TN = 1 << EXP_TBITS;   // entries of the 2^(j/N) table
+
 struct ExpCoeffs {
-    double three_eighths;               // rsqrt correction
-    double l2e64, ln2_64hi, ln2_64lo;   // table-driven reduction (64/ln2, ln2/64 split)
-    double t5[4];                       // 1/5!, 1/4!, 1/3!, 1/2!
-    double t64[64];                     // 2^(j/64), j = 0 .. 63
+    double three_eighths;         // rsqrt correction
+    double l2eN, ln2N_hi, ln2N_lo;   // table-driven reduction (N/ln2, ln2/N split)
+    double t3[2];                 // 1/3!, 1/2!
 };
 
 inline ExpCoeffs exp_coeffs()
@@ -40,15 +43,20 @@ inline ExpCoeffs exp_coeffs()
     const double ln2_hi = 6.93147180369123816490e-01;   // fdlibm split of ln 2
     const double ln2_lo = 1.90821492927058770002e-10;
     e.three_eighths = 0.375;
-    e.l2e64 = 64.0 * 1.4426950408889634074;
-    e.ln2_64hi = ln2_hi / 64.0;   // exact: division by a power of two
-    e.ln2_64lo = ln2_lo / 64.0;
-    e.t5[0] = 1.0 / 120.0;
-    e.t5[1] = 1.0 / 24.0;
-    e.t5[2] = 1.0 / 6.0;
-    e.t5[3] = 0.5;
-    for (int j = 0; j < 64; ++j) e.t64[j] = (double)std::exp2((long double)j / 64.0L);
+    e.l2eN = EXP_TN * 1.4426950408889634074;
+    // exact: division by a power of two; ln2_hi has 21 trailing zero bits, so
+    // k * ln2N_hi is exact for |k| < 2^21 (|x| < 1400)
+    e.ln2N_hi = ln2_hi / EXP_TN;
+    e.ln2N_lo = ln2_lo / EXP_TN;
+    e.t3[0] = 1.0 / 6.0;
+    e.t3[1] = 0.5;
     return e;
+}
+
+// The 2^(j/N) table, j = 0 .. N-1, rounded to double from long double.
+inline void exp_table(double *t)
+{
+    for (int j = 0; j < EXP_TN; ++j) t[j] = (double)std::exp2((long double)j / EXP_TN);
 }
 
 __device__ __forceinline__ double rsqrt64(double x, const ExpCoeffs &ec)
@@ -62,28 +70,27 @@ __device__ __forceinline__ double rsqrt64(double x, const ExpCoeffs &ec)
 }
 
 // e^x for x <= 0 (x = -0 included; large negative x underflows to 0 through
-// ldexp, so no clamp is needed); tab = the 64 entries of ec.t64 (LDS).
+// ldexp, so no clamp is needed); tab = the 2^(j/N) table (LDS).
 __device__ __forceinline__ double exp_neg64t(double x, const ExpCoeffs &ec,
                                              const double *__restrict__ tab)
 {
-    const double k = __builtin_rint(x * ec.l2e64);
-    double r = __builtin_fma(-k, ec.ln2_64hi, x);
-    r = __builtin_fma(-k, ec.ln2_64lo, r);
-    double p = __builtin_fma(ec.t5[0], r, ec.t5[1]);
-    p = __builtin_fma(p, r, ec.t5[2]);
-    p = __builtin_fma(p, r, ec.t5[3]);
+    const double k = __builtin_rint(x * ec.l2eN);
+    double r = __builtin_fma(-k, ec.ln2N_hi, x);
+    r = __builtin_fma(-k, ec.ln2N_lo, r);
+    double p = __builtin_fma(ec.t3[0], r, ec.t3[1]);
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
-    const int ki = (int)k;                 // x <= 0: ki <= 0, ki = 64 (ki >> 6) + (ki & 63)
-    return __builtin_amdgcn_ldexp(tab[ki & 63] * p, ki >> 6);
+    const int ki = (int)k;   // ki = N (ki >> TBITS) + (ki & (N-1)), also for ki < 0
+    return __builtin_amdgcn_ldexp(tab[ki & (EXP_TN - 1)] * p, ki >> EXP_TBITS);
 }
 
-// Copy of the 2^(j/64) table for one wave (lane l writes entry l).
-__device__ __forceinline__ void load_exp_table(double *__restrict__ dst, const ExpCoeffs &ec,
-                                               int lane)
+// The workgroup's LDS copy of the table (every thread of the block calls it,
+// before any of them may return).
+__device__ __forceinline__ void load_exp_table(double *__restrict__ dst,
+                                               const double *__restrict__ src)
 {
-    dst[lane] = ec.t64[lane];
-    __builtin_amdgcn_wave_barrier();
+    for (int j = threadIdx.x; j < EXP_TN; j += blockDim.x) dst[j] = src[j];
+    __syncthreads();
 }
 
 __device__ __forceinline__ float rsqrt32(float x) { return __builtin_amdgcn_rsqf(x); }
