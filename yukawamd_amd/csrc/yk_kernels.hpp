@@ -149,11 +149,12 @@ __device__ __forceinline__ void interact(const ForceConsts &c, double zi, double
                 double r = r2 * rinv;
                 double mr = c.mu * r;
                 double ex = exp_neg64t(-mr, c.ec, c.etab);
-                double exr = ex * rinv;
-                double fc = (1.0 + mr) * exr * (rinv * rinv);
+                // (1 + mu r) e^{-mu r} / r^3, the same operations whether or
+                // not the step measures
+                double fc = (__builtin_fma(mr, ex, ex) * rinv) * (rinv * rinv);
                 s = (KIND == MIX) ? zj * fc : fc;
                 if constexpr (MEAS) {
-                    phi = exr - c.evcut;
+                    phi = ex * rinv - c.evcut;
                     if (KIND == MIX) phi *= zj;
                 }
             } else {

@@ -34,7 +34,7 @@ constexpr int EXP_TN = 1 << EXP_TBITS;   // entries of the 2^(j/N) table
 struct ExpCoeffs {
     double three_eighths;         // rsqrt correction
     double l2eN, ln2N_hi, ln2N_lo;   // table-driven reduction (N/ln2, ln2/N split)
-    double t3[2];                 // 1/3!, 1/2!
+    double t3[1];                 // 1/3! (1/2! is written inline)
 };
 
 inline ExpCoeffs exp_coeffs()
@@ -49,7 +49,6 @@ inline ExpCoeffs exp_coeffs()
     e.ln2N_hi = ln2_hi / EXP_TN;
     e.ln2N_lo = ln2_lo / EXP_TN;
     e.t3[0] = 1.0 / 6.0;
-    e.t3[1] = 0.5;
     return e;
 }
 
@@ -77,7 +76,7 @@ __device__ __forceinline__ double exp_neg64t(double x, const ExpCoeffs &ec,
     const double k = __builtin_rint(x * ec.l2eN);
     double r = __builtin_fma(-k, ec.ln2N_hi, x);
     r = __builtin_fma(-k, ec.ln2N_lo, r);
-    double p = __builtin_fma(ec.t3[0], r, ec.t3[1]);
+    double p = __builtin_fma(r, ec.t3[0], 0.5);   // 1/2 as an inline constant
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
     const int ki = (int)k;   // ki = N (ki >> TBITS) + (ki & (N-1)), also for ki < 0
