@@ -25,7 +25,7 @@
 
 namespace yk {
 
-// Constants of the FP64 special functions.  Kernels receive them as kernel
+// Constants of the FP64 pair loop.  Kernels receive them as kernel
 // arguments so they live in SGPRs (a VOP3 FMA may read one scalar operand)
 // instead of being rematerialised in VGPRs for every pair.
 constexpr int EXP_TBITS = 10;
@@ -33,6 +33,7 @@ constexpr int EXP_TN = 1 << EXP_TBITS;   // entries of the 2^(j/N) table
 
 struct ExpCoeffs {
     double three_eighths;         // rsqrt correction
+    double r2_floor;              // 1e-200, added to r^2 by the FP64 list kernels
     double l2eN, ln2N_hi, ln2N_lo;   // table-driven reduction (N/ln2, ln2/N split)
     double t3[1];                 // 1/3! (1/2! is written inline)
 };
@@ -43,6 +44,7 @@ inline ExpCoeffs exp_coeffs()
     const double ln2_hi = 6.93147180369123816490e-01;   // fdlibm split of ln 2
     const double ln2_lo = 1.90821492927058770002e-10;
     e.three_eighths = 0.375;
+    e.r2_floor = 1e-200;
     e.l2eN = EXP_TN * 1.4426950408889634074;
     // exact: division by a power of two; ln2_hi has 21 trailing zero bits, so
     // k * ln2N_hi is exact for |k| < 2^21 (|x| < 1400)
