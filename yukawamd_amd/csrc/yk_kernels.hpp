@@ -136,16 +136,19 @@ __device__ __forceinline__ void interact(const ForceConsts &c, double zi, double
                                          double dx, double dy, double dz, bool self,
                                          Acc<MEAS> &acc)
 {
-    // FP64 ion pairs from lists: r^2 carries 1e-200, below the last bit of
-    // any r^2 above 1e-184 (so every other pair's r^2 is unchanged).  The
-    // self pair (d = 0 exactly) then has r^2 = 1e-200 and a finite factor
-    // ~1e300 times d = 0: an exact zero force without a compare; a measuring
-    // step still excludes it by r^2 for the energy.
-    constexpr bool SOFT_SELF = SELF_BY_R2 && PREC == 0 && KIND != NN;
+    // Ion pairs from lists: r^2 carries a floor below the last bit of any
+    // physical r^2 (1e-200 in FP64; 1e-20 in the mixed form, whose FP32 1/r^3
+    // must stay finite: the closest ions are fm apart, r^2 >= 1), so every
+    // other pair's r^2 is unchanged.  The self pair (d = 0 exactly) then has
+    // r^2 = floor and a finite factor times d = 0: an exact zero force
+    // without a compare; a measuring step still excludes it by r^2 for the
+    // energy.
+    constexpr bool SOFT_SELF = SELF_BY_R2 && KIND != NN;
     double r2;
     if constexpr (SOFT_SELF) {
-        r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, c.ec.r2_floor)));
-        if constexpr (MEAS) self = !(r2 > 1e-199);
+        const double fl = (PREC == 0) ? c.ec.r2_floor : c.ec.r2_floor_mixed;
+        r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, fl)));
+        if constexpr (MEAS) self = !(r2 > 10.0 * fl);
     } else {
         r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
         if constexpr (SELF_BY_R2) self = self || !(r2 > 0.0);

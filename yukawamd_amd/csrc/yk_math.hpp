@@ -34,6 +34,7 @@ constexpr int EXP_TN = 1 << EXP_TBITS;   // entries of the 2^(j/N) table
 struct ExpCoeffs {
     double three_eighths;         // rsqrt correction
     double r2_floor;              // 1e-200, added to r^2 by the FP64 list kernels
+    double r2_floor_mixed;        // 1e-20, the same for the mixed form (FP32 1/r^3 stays finite)
     double l2eN, ln2N_hi, ln2N_lo;   // table-driven reduction (N/ln2, ln2/N split)
     double t3[1];                 // 1/3! (1/2! is written inline)
 };
@@ -45,6 +46,7 @@ inline ExpCoeffs exp_coeffs()
     const double ln2_lo = 1.90821492927058770002e-10;
     e.three_eighths = 0.375;
     e.r2_floor = 1e-200;
+    e.r2_floor_mixed = 1e-20;
     e.l2eN = EXP_TN * 1.4426950408889634074;
     // exact: division by a power of two; ln2_hi has 21 trailing zero bits, so
     // k * ln2N_hi is exact for |k| < 2^21 (|x| < 1400)
