@@ -144,27 +144,6 @@ __device__ __forceinline__ void batch_item(const RepConsts &rc, const ExpCoeffs 
     }
 }
 
-// A cluster's list as four segments: entry v (< hi) of the concatenation.
-struct SegList {
-    const unsigned *base;
-    int segcap, hi, b1, b2, b3, nf0, nf1, nf2, nf3;
-    __device__ __forceinline__ unsigned entry(int v, bool &general) const
-    {
-        general = false;
-        if (v >= hi) return 0u;
-        // arithmetic selection of the segment (selects of addresses would put
-        // the fields on the stack)
-        const int s1 = v >= b1, s2 = v >= b2, s3 = v >= b3;
-        const int sg = s1 + s2 + s3;
-        const int start = s1 * (b1 + s2 * (b2 - b1 + s3 * (b3 - b2)));
-        const int nfs = nf0 + s1 * (nf1 - nf0 + s2 * (nf2 - nf1 + s3 * (nf3 - nf2)));
-        const int o = v - start;
-        const unsigned *r = base + (size_t)sg * segcap;
-        general = o >= nfs;
-        return general ? r[segcap - 1 - (o - nfs)] : r[o];
-    }
-};
-
 // One list item: the four 16-slot clusters of target tile it.tile against
 // share m of M of each cluster's list (the scheme of k_pairs_clist, with the
 // replica's constants and its region of the list arena).

@@ -993,6 +993,30 @@ static __global__ __launch_bounds__(CORDER_THREADS) void k_corder(
     for (int k = threadIdx.x; k < ncl; k += CORDER_THREADS) order[k] = (int)(key[k] & 0xfffu);
 }
 
+// A cluster's list as its NSEG segments (segment s begins at list index b_s,
+// b_0 = 0, and holds nf_s fast entries from the front of its region of segcap
+// entries, then general ones from the back): entry v and whether it is
+// general (0 past hi).
+struct SegList {
+    const unsigned *base;
+    int segcap, hi, b1, b2, b3, nf0, nf1, nf2, nf3;
+    __device__ __forceinline__ unsigned entry(int v, bool &general) const
+    {
+        general = false;
+        if (v >= hi) return 0u;
+        // arithmetic selection of the segment (selects of addresses would put
+        // the fields on the stack)
+        const int s1 = v >= b1, s2 = v >= b2, s3 = v >= b3;
+        const int sg = s1 + s2 + s3;
+        const int start = s1 * (b1 + s2 * (b2 - b1 + s3 * (b3 - b2)));
+        const int nfs = nf0 + s1 * (nf1 - nf0 + s2 * (nf2 - nf1 + s3 * (nf3 - nf2)));
+        const int o = v - start;
+        const unsigned *r = base + (size_t)sg * segcap;
+        general = o >= nfs;
+        return general ? r[segcap - 1 - (o - nfs)] : r[o];
+    }
+};
+
 // Pair sums over the cluster lists.  A wave takes four clusters (order[4s ..
 // 4s+3]), 16 lanes each, one target per lane, and item m of M: an equal share
 // of every cluster's list.  Each 16-lane group stages 16 of its own sources at
@@ -1055,20 +1079,10 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
     }
     iters = __builtin_amdgcn_readfirstlane(iters);
     const unsigned *mine = clist + (size_t)(cl >= 0 ? cl : 0) * NSEG * segcap;
-    // entry v of the list; bit 31 of the result's companion flag: general
-    auto entry_at = [&](int v, bool &general) -> unsigned {
-        general = false;
-        if (v >= hi) return 0u;
-        // segment of entry v (unrolled selects: no dynamic register indexing)
-        int o = v, nfs = sc[0].x, sg = 0;
-#pragma unroll
-        for (int q = 1; q < NSEG; ++q)
-            if (v >= sbeg[q]) { o = v - sbeg[q]; nfs = sc[q].x; sg = q; }
-        const unsigned *r = mine + (size_t)sg * segcap;
-        if (o < nfs) return r[o];
-        general = true;
-        return r[segcap - 1 - (o - nfs)];
-    };
+    static_assert(NSEG == 4, "SegList holds four segments");
+    const SegList sl{mine, segcap, hi, sbeg[1], sbeg[2], sbeg[3], sc[0].x, sc[1].x, sc[2].x,
+                     sc[3].x};
+    auto entry_at = [&](int v, bool &general) -> unsigned { return sl.entry(v, general); };
     Acc<MEAS> acc;
     double4 *sw = sj[wv];
     // software pipeline over batches of 16 entries per group
