@@ -373,20 +373,22 @@ __global__ __launch_bounds__(BLOCK) void k_batch_drift(double4 *__restrict__ pos
         }
     }
     const int tile = i / TILE;
+    unsigned long long *md = nullptr;   // wave-uniform
+    double d2 = 0.0;
     if (tile < nslots / TILE) {
         tile_box_store(x, valid, tile, box);
         // largest displacement since the replica's list build (list replicas)
         const int tr = tile_rep[tile];   // one replica per tile: wave-uniform
         if (MOVE && tr >= 0 && reps[tr].list_items > 0) {
-            double d2 = 0.0;
+            md = maxdisp + 2 * tr;
             if (valid) {
                 const double4 q = xref[i];
                 const double dx = x[0] - q.x, dy = x[1] - q.y, dz = x[2] - q.z;
                 d2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
             }
-            record_disp(maxdisp + 2 * tr, d2);
         }
     }
+    if (MOVE) record_disp(md, d2);   // every thread of the workgroup
 }
 
 // Cluster lists of the list replicas (grid: arena tiles x NSEG segments),

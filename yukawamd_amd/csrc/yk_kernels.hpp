@@ -400,14 +400,22 @@ __device__ __forceinline__ bool lists_stale(const unsigned long long *md, double
     return !(s * 1.00001f <= (float)skin);
 }
 
-// Adds one wave's squared displacements (d2 >= 0 per lane) to md.  The wave's
-// largest value m1 enters md[0], and whichever of m1 and md[0]'s previous
-// value is smaller enters md[1], as does the wave's second-largest value m2
-// (m2 <= m1 <= md[0] by then, so it never competes for md[0]).  Every value
-// but one copy of the final maximum thus reaches md[1], which ends as the
-// second-largest displacement.  Called by all 64 lanes of the wave.
+// Adds the squared displacements of one workgroup (d2 >= 0 per thread) to
+// the top-two pair md[0] >= md[1] that its wave names (md == nullptr: the
+// wave adds nothing).  Each wave reduces its values to their largest m1 and
+// second-largest m2, and the first thread merges the waves that name the same
+// md into one such pair (b1, b2) before it updates md: b1 enters md[0];
+// whichever of b1 and md[0]'s previous value is smaller enters md[1], as does
+// b2.  Every value but one copy of the final maximum is at most the second
+// largest, and the second largest always reaches md[1] (either with its own
+// pair, or as the loser against the maximum), so md[1] ends as the
+// second-largest displacement.  One update per workgroup and md keeps the
+// atomics few.  Called by every thread of a BLOCK-thread workgroup.
 __device__ __forceinline__ void record_disp(unsigned long long *md, double d2)
 {
+    constexpr int NW = BLOCK / 64;
+    __shared__ double top_s[NW][2];
+    __shared__ unsigned long long *md_s[NW];
     double m1 = d2;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -422,12 +430,37 @@ __device__ __forceinline__ void record_disp(unsigned long long *md, double d2)
         const double o = __shfl_xor(m2, off, 64);
         m2 = o > m2 ? o : m2;
     }
-    if (lane == 0 && m1 > 0.0) {
-        const unsigned long long b1 = as_bits(m1);
-        const unsigned long long old = atomicMax(&md[0], b1);
-        const unsigned long long lose = old < b1 ? old : b1;
-        if (lose) atomicMax(&md[1], lose);
-        if (m2 > 0.0) atomicMax(&md[1], as_bits(m2));
+    const int w = threadIdx.x >> 6;
+    if (lane == 0) {
+        top_s[w][0] = m1;
+        top_s[w][1] = m2;
+        md_s[w] = md;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w0 = 0; w0 < NW; ++w0) {
+        unsigned long long *m = md_s[w0];
+        bool first = m != nullptr;
+        for (int k = 0; k < w0; ++k) first = first && md_s[k] != m;
+        if (!first) continue;   // none, or merged with an earlier wave's
+        double b1 = 0.0, b2 = 0.0;
+        for (int k = w0; k < NW; ++k) {
+            if (md_s[k] != m) continue;
+            const double a1 = top_s[k][0], a2 = top_s[k][1];
+            if (a1 > b1) {
+                b2 = a2 > b1 ? a2 : b1;
+                b1 = a1;
+            } else {
+                b2 = a1 > b2 ? a1 : b2;
+            }
+        }
+        if (b1 > 0.0) {
+            const unsigned long long u1 = as_bits(b1);
+            const unsigned long long old = atomicMax(&m[0], u1);
+            const unsigned long long lose = old < u1 ? old : u1;
+            if (lose) atomicMax(&m[1], lose);
+            if (b2 > 0.0) atomicMax(&m[1], as_bits(b2));
+        }
     }
 }
 
