@@ -8,7 +8,9 @@ cancel_vcom every 7 steps, tnorm every 5, an x strain each step, energies
 every 3rd and the pressure tensor every 4th step.  Both start from the same
 bits (tests/test_ykmd_init.py); the device sums forces in another order, so
 the measurement records agree to rounding grown over the run, the final box
-(host arithmetic in both) exactly.
+(host arithmetic in both) exactly.  Two more runs add checkpoints every 4th
+step and 'xv8b' configurations every 5th, or an 'x4b' trajectory every 3rd
+step: both programs must write the same files, with the same headers.
 """
 import os
 import re
@@ -18,7 +20,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from yukawamd_amd.xvb import read_xvb
+from yukawamd_amd.xvb import read_xvb, read_xvb_all
 
 ROOT = Path(__file__).resolve().parent.parent
 MD_OMP = ROOT / "oracle" / "_ref" / "md_omp"
@@ -48,9 +50,9 @@ def read_meas(path):
     return data["istep"], data["v"]
 
 
-def run(cmd, workdir, env_extra=None):
+def run(cmd, workdir, env_extra=None, runmd=RUNMD):
     workdir.mkdir(parents=True, exist_ok=True)
-    (workdir / "runmd.in").write_text(RUNMD)
+    (workdir / "runmd.in").write_text(runmd)
     env = dict(os.environ, OMP_NUM_THREADS="1", **(env_extra or {}))
     out = subprocess.run(cmd, cwd=workdir, env=env, capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
@@ -102,3 +104,53 @@ def test_driver_mixed_precision_close_to_reference(tmp_path):
     assert list(gsteps) == list(rsteps)
     # ev, ek, px of the mixed kernel stay within 1e-5 of the FP64 reference
     np.testing.assert_allclose(gvals[:, 1:4], rvals[:, 1:4], rtol=1e-5)
+
+
+OUTPUT_CASES = {
+    "checkpoints_and_xv8b": " nckpt=4, nout=5, ftype='xv8b', append=.false.,",
+    "x4b_trajectory": " nout=3, ftype='x4b', append=.true.,",
+}
+
+
+def config_files(workdir):
+    return sorted(p.name for p in workdir.glob("md.*") if p.suffix not in (".log", ".meas"))
+
+
+@pytest.mark.parametrize("case", sorted(OUTPUT_CASES))
+def test_driver_configuration_outputs_match_reference(tmp_path, case):
+    runmd = RUNMD.replace("iseed=2024,", "iseed=2024,\n" + OUTPUT_CASES[case])
+    run([str(MD_OMP)], tmp_path / "ref", runmd=runmd)
+    run([str(YKMD)], tmp_path / "gpu", runmd=runmd)
+    names = config_files(tmp_path / "ref")
+    assert config_files(tmp_path / "gpu") == names
+    if case == "checkpoints_and_xv8b":
+        # initial configuration, 2 warm-up checkpoints + the one after the
+        # warm-up, 6 in the measurement phase (step 20 also a saved one),
+        # the saved configurations of steps 5, 10 and 15
+        assert len(names) == 1 + 3 + 6 + 3
+    else:
+        assert [n for n in names if n.startswith("md.traj.")] == [n for n in names if "x4b" in n]
+    for name in names:
+        cr = read_xvb_all(tmp_path / "ref" / name)
+        cg = read_xvb_all(tmp_path / "gpu" / name)
+        assert len(cg) == len(cr) > 0, name
+        if "traj" in name:
+            assert len(cr) == 8, name   # steps 3, 6, ..., 24
+        for g, r in zip(cg, cr):
+            assert (g["ftype"], g["n"], g["sim_type"], g["time"], g["xl"]) == \
+                   (r["ftype"], r["n"], r["sim_type"], r["time"], r["xl"]), name
+            # energies, pressure and tensor are zero after a step that does not measure
+            for key in ("ev", "ek", "px", "pp"):
+                zr = np.asarray(r[key]) == 0.0
+                assert (np.asarray(g[key])[zr] == 0.0).all(), (name, key)
+            for key in ("ev", "ek", "px", "pp"):
+                np.testing.assert_allclose(g[key], r[key], rtol=1e-9,
+                                           atol=1e-12 * np.abs(r[key]).max(), err_msg=name)
+            L = np.array(r["xl"])
+            dx = np.abs(g["x"] - r["x"])
+            dx = np.minimum(dx, L - dx)
+            tol = 2e-7 if g["ftype"].startswith("x4") or g["ftype"].startswith("xv4") else 1e-9
+            assert dx.max() <= tol * L.max(), name
+            if "v" in r:
+                np.testing.assert_allclose(g["v"], r["v"], rtol=1e-7,
+                                           atol=1e-10 * np.abs(r["v"]).max(), err_msg=name)

@@ -25,8 +25,13 @@
 !   thomas_fermi_mu        screening when no length is given (src/input1.F:262-273)
 !   auto_rncut             nuclear cutoff when rncut = 0 (src/input1.F:292-311)
 !   block_stats            statt (src/md.F:228-243)
-!   write_xvb / read_xvb   revision-b configuration files, types 'x8b',
-!                          'xv8b', 'xva8b' (src/rw_xvb.f:148-203)
+!   write_xvb / read_xvb   revision-b configuration files (header records,
+!                          then the particles), types 'x4b', 'xv4b', 'x8b',
+!                          'xv8b', 'xva8b' (src/rw_xvb.f:148-203); read_xvb
+!                          reads 'xv8b'
+!   write_xva              revision-a configuration files (time, then the
+!                          particles; no header), types 'md0', 'x4a', 'xv4a',
+!                          'x8a', 'xv8a', 'xva8a' (src/rw_xva.f:169-223)
 !
 ! Run with one thread of the reference when comparing: its OpenMP reductions
 ! sum in thread order.
@@ -62,7 +67,7 @@ module yk_host
 
   public :: mt_seed, mt_uniform, gauss_polar, random_positions, maxwell_velocities, &
             host_ttot, host_tnorm, host_cancel_vcom, random_box, thomas_fermi_mu, &
-            auto_rncut, block_stats, mdmod, write_xvb, read_xvb
+            auto_rncut, block_stats, mdmod, write_xvb, write_xva, read_xvb
 
 contains
 
@@ -255,11 +260,25 @@ contains
 
   ! Revision-b configuration file: type, code name and version, date, sim
   ! type, (time, box, ev, ek, px, pp(3,3), n), then the particle record.
-  subroutine write_xvb(file, ftype, sim_type, time, xl, ev, ek, px, pp, n, x, v, a)
+  ! Opens a configuration file for one more configuration: a new file, or the
+  ! end of an existing one when append (a trajectory file).
+  subroutine open_config(file, append, u)
+    character(*), intent(in) :: file
+    logical, intent(in) :: append
+    integer, intent(out) :: u
+    if (append) then
+      open(newunit=u, file=file, status='unknown', form='unformatted', position='append')
+    else
+      open(newunit=u, file=file, status='unknown', form='unformatted')
+    end if
+  end subroutine open_config
+
+  subroutine write_xvb(file, ftype, sim_type, time, xl, ev, ek, px, pp, n, x, v, a, append)
     character(*), intent(in) :: file, ftype, sim_type
     real(dp), intent(in) :: time, xl(3), ev, ek, px, pp(3, 3)
     integer, intent(in) :: n
     real(dp), intent(in) :: x(3, 0:n - 1), v(3, 0:n - 1), a(3, 0:n - 1)
+    logical, intent(in), optional :: append
     character(6) :: ft
     character(10) :: cname
     character(8) :: cver, date
@@ -267,18 +286,26 @@ contains
     character(5) :: zone
     character(20) :: st
     integer :: u, i
+    logical :: app
+    app = .false.
+    if (present(append)) app = append
     ft = ftype
     cname = 'MD'
     cver = '6.2.0'
     st = sim_type
     call date_and_time(date, daytime, zone)
-    open(newunit=u, file=file, status='unknown', form='unformatted')
+    call open_config(file, app, u)
     write(u) ft
     write(u) cname, cver
     write(u) date, daytime, zone
     write(u) st
     write(u) time, xl(1), xl(2), xl(3), ev, ek, px, pp, n
     select case (trim(ft))
+    case ('x4b')
+      write(u) (real(x(1, i)), real(x(2, i)), real(x(3, i)), i = 0, n - 1)
+    case ('xv4b')
+      write(u) (real(x(1, i)), real(x(2, i)), real(x(3, i)), &
+                real(v(1, i)), real(v(2, i)), real(v(3, i)), i = 0, n - 1)
     case ('x8b')
       write(u) (x(1, i), x(2, i), x(3, i), i = 0, n - 1)
     case ('xv8b')
@@ -289,6 +316,38 @@ contains
     end select
     close(u)
   end subroutine write_xvb
+
+  subroutine write_xva(file, ftype, time, n, x, v, a, append)
+    character(*), intent(in) :: file, ftype
+    real(dp), intent(in) :: time
+    integer, intent(in) :: n
+    real(dp), intent(in) :: x(3, 0:n - 1), v(3, 0:n - 1), a(3, 0:n - 1)
+    logical, intent(in) :: append
+    integer :: u, i
+    call open_config(file, append, u)
+    select case (trim(ftype))
+    case ('md0')
+      write(u) (x(1, i), x(2, i), x(3, i), v(1, i), v(2, i), v(3, i), i = 0, n - 1)
+    case ('x4a')
+      write(u) real(time)
+      write(u) (real(x(1, i)), real(x(2, i)), real(x(3, i)), i = 0, n - 1)
+    case ('xv4a')
+      write(u) real(time)
+      write(u) (real(x(1, i)), real(x(2, i)), real(x(3, i)), &
+                real(v(1, i)), real(v(2, i)), real(v(3, i)), i = 0, n - 1)
+    case ('x8a')
+      write(u) time
+      write(u) (x(1, i), x(2, i), x(3, i), i = 0, n - 1)
+    case ('xv8a')
+      write(u) time
+      write(u) (x(1, i), x(2, i), x(3, i), v(1, i), v(2, i), v(3, i), i = 0, n - 1)
+    case ('xva8a')
+      write(u) time
+      write(u) (x(1, i), x(2, i), x(3, i), v(1, i), v(2, i), v(3, i), &
+                a(1, i), a(2, i), a(3, i), i = 0, n - 1)
+    end select
+    close(u)
+  end subroutine write_xva
 
   ! Reads a revision-b 'xv8b' file: box, time and n, then positions and
   ! velocities (allocated here).  irtn = 0 on success.

@@ -16,9 +16,17 @@
 ! Output, in the reference's formats: the 'e', 'p' and 's' lines and the
 ! initial / final lines of the log (md.<tend>.log), the binary measurement
 ! records (md.<tend>.meas, src/md.F:151), the initial configuration
-! (md.<tstart>.xv8b, src/initconfig.f:385-390) and the end-of-run checkpoint
-! (md.<time>.xv8b, src/md_utils.f:19-38).  Configuration outputs every
-! nout steps and intermediate checkpoints are not written.
+! (md.<tstart>.xv8b, src/initconfig.f:385-390), the configurations saved
+! every nout steps of the measurement phase (save_config, src/md_utils.f:61-114:
+! the binary types of src/rw_xvb.f and src/rw_xva.f, one file per
+! configuration or appended to md.traj.<tend>.<ftype>) and the checkpoints
+! (md.<time>.xv8b, src/md_utils.f:19-38) every nckpt steps, after the warm-up
+! and at the end of the run (src/md.F:93, :104, :165, :176).  A run that saves
+! or checkpoints is split into device runs between those steps; their headers
+! carry what the reference's globals hold then: the energies, pressure and
+! tensor of the step just done, all zero after a step that does not measure
+! (every accel of the reference starts them from zero, src/int_ion_mix.f:132-137,
+! src/int_ion_pure.f:243-248, src/int_nn.f:334-339; src/vel_verlet_2.f:107-110).
 !
 ! YK_PRECISION=mixed selects the mixed-precision kernels; the mixture
 ! kernel's last-particle behaviour of the reference (src/int_ion_mix.f:158)
@@ -57,11 +65,12 @@ program ykmd
                             efield, q0, w0, tref, nuclear, xpacket, aa, bb, cc, rncut, &
                             irnd, iseed, rmax
 
-  integer :: n, nspec, i, j, k, ig, ios, irtn, istep, nrec, r, u8, u9
+  integer :: n, nspec, i, j, k, ig, ios, irtn, istep, nrec, r, u8, u9, nc, kd
   real(dp), allocatable :: x(:, :), v(:, :), a(:, :), zii(:), aii(:)
   real(dp) :: xlf, xl0(3), xl(3), ztot, xmuc, time, tend, ev, ek, px, e, pp(3, 3)
   real(dp) :: evave, ev2ave, pave, p2ave, eva, ev2a, pa, p2a, dev, dp_, devave, dpave
   real(dp) :: x1, x2, rhof
+  real(dp) :: hev, hek, hpx, hpp(3, 3)   ! ev, ek, px, pp of src/md_globals.f now
   character(40) :: suffix
   character(256) :: env, arg
   type(yk_params_t) :: p
@@ -78,6 +87,14 @@ program ykmd
   read(10, nml=run_parameters, iostat=ios)
   if (ios /= 0) call fail('error reading namelist run_parameters from runmd.in')
   close(10)
+
+  if (nout /= 0) then
+    select case (trim(ftype))
+    case ('x4b', 'xv4b', 'x8b', 'xv8b', 'xva8b', 'md0', 'x4a', 'xv4a', 'x8a', 'xv8a', 'xva8a')
+    case default
+      call fail('ftype not supported (binary revision-a and -b types only): '//trim(ftype))
+    end select
+  end if
 
   ! species from the internal list, in list order (src/initconfig.f:117-141)
   if (trim(spec_file) /= '' .and. trim(spec_file) /= 'internal') &
@@ -210,22 +227,37 @@ program ykmd
   e = m%ev + ek
   write(u8, 10050) '  Initial Vtot, Ttot, E, P = ', m%ev, ek, e, m%px
 10050 format(/, a, 4(2x, es16.9))
+  hev = m%ev
+  hek = eka
+  hpx = m%px
+  hpp = 0.0d0
 
   ! warm-up groups (src/md.F:80-101)
   istep = 0
   if (nwgroup >= 1) write(u8, '(/a)') '  Warmup phase'
   allocate(recs(max(1, ntot * nind)), rsteps(max(1, ntot * nind)))
   do ig = 1, nwgroup
-    call check(yk_run(h, dt, int(nwsteps, c_int32_t), 0_c_int32_t, recs), 'yk_run')
-    do k = 1, nwsteps
-      time = time + dt
+    kd = 0
+    do while (kd < nwsteps)
+      nc = min(nwsteps - kd, steps_to(istep, nckpt))
+      call check(yk_run(h, dt, int(nc, c_int32_t), 0_c_int32_t, recs), 'yk_run')
+      do k = 1, nc
+        time = time + dt
+      end do
+      istep = istep + nc
+      kd = kd + nc
+      hev = 0.0d0          ! accel(.false.) and newton(.false.)
+      hek = 0.0d0
+      hpx = 0.0d0
+      hpp = 0.0d0
+      if (mdmod(istep, nckpt) == 0) call checkpoint()
     end do
-    istep = istep + nwsteps
     call check(yk_cancel_vcom(h), 'yk_cancel_vcom')
     call check(yk_tnorm(h), 'yk_tnorm')
     write(u8, 10082) istep, time
   end do
 10082 format(2x, i7, 1x, f14.2, '   warm up')
+  if (istep > 0 .and. mdmod(istep, nckpt) /= 0) call checkpoint()
 
   ! measurement groups (src/md.F:105-168)
   if (ngroup >= 1) then
@@ -245,44 +277,65 @@ program ykmd
   ev = m%ev
   ek = eka
   px = m%px
+  hpp = 0.0d0
   do ig = 1, ngroup
     eva = 0.0d0
     ev2a = 0.0d0
     pa = 0.0d0
     p2a = 0.0d0
-    lp%dt = dt
-    lp%nsteps = int(ntot * nind, c_int32_t)
-    lp%istep0 = int(istep, c_int32_t)
-    lp%nind = int(nind, c_int32_t)
-    lp%nptensor = int(nptensor, c_int32_t)
-    lp%ncom = int(ncom, c_int32_t)
-    lp%ntnorm = int(ntnorm, c_int32_t)
-    lp%deps = deps
-    lp%xl0 = xl0
-    call check(yk_run_md(h, lp, recs, rsteps, int(size(recs), c_int32_t), nrec32, box), &
-               'yk_run_md')
-    nrec = nrec32
-    xl = box
-    r = 0
-    do k = 1, ntot * nind
-      time = time + dt
-      istep = istep + 1
-      if (mdmod(istep, nind) /= 0 .and. mdmod(istep, nptensor) /= 0) cycle
-      r = r + 1
-      if (r > nrec .or. rsteps(r) /= istep) call fail('measurement records out of step')
-      ev = recs(r)%ev
-      ek = recs(r)%ek
-      px = recs(r)%px
-      if (mdmod(istep, nind) == 0) then
-        eva = eva + ev
-        ev2a = ev2a + ev**2
-        pa = pa + px
-        p2a = p2a + px**2
-        e = ek + ev
-        write(u8, 10101) 'e', istep, time, ev, ek, e, px
-      end if
-      if (mdmod(istep, nptensor) == 0) write(u8, 10101) 'p', istep, time, recs(r)%pp
-      write(u9) istep, time, ev, ek, px, recs(r)%pp
+    kd = 0
+    do while (kd < ntot * nind)
+      ! one device run up to the next configuration output or checkpoint
+      nc = min(ntot * nind - kd, steps_to(istep, nout), steps_to(istep, nckpt))
+      lp%dt = dt
+      lp%nsteps = int(nc, c_int32_t)
+      lp%istep0 = int(istep, c_int32_t)
+      lp%nind = int(nind, c_int32_t)
+      lp%nptensor = int(nptensor, c_int32_t)
+      lp%ncom = int(ncom, c_int32_t)
+      lp%ntnorm = int(ntnorm, c_int32_t)
+      lp%deps = deps
+      lp%xl0 = xl0
+      call check(yk_run_md(h, lp, recs, rsteps, int(size(recs), c_int32_t), nrec32, box), &
+                 'yk_run_md')
+      nrec = nrec32
+      xl = box
+      r = 0
+      do k = 1, nc
+        time = time + dt
+        istep = istep + 1
+        if (mdmod(istep, nind) /= 0 .and. mdmod(istep, nptensor) /= 0) then
+          hev = 0.0d0
+          hek = 0.0d0
+          hpx = 0.0d0
+          hpp = 0.0d0
+          cycle
+        end if
+        r = r + 1
+        if (r > nrec .or. rsteps(r) /= istep) call fail('measurement records out of step')
+        ev = recs(r)%ev
+        ek = recs(r)%ek
+        px = recs(r)%px
+        hev = ev
+        hek = ek
+        hpx = px
+        hpp(1, 1) = recs(r)%pp(1); hpp(1, 2) = recs(r)%pp(2); hpp(1, 3) = recs(r)%pp(3)
+        hpp(2, 1) = recs(r)%pp(2); hpp(2, 2) = recs(r)%pp(4); hpp(2, 3) = recs(r)%pp(5)
+        hpp(3, 1) = recs(r)%pp(3); hpp(3, 2) = recs(r)%pp(5); hpp(3, 3) = recs(r)%pp(6)
+        if (mdmod(istep, nind) == 0) then
+          eva = eva + ev
+          ev2a = ev2a + ev**2
+          pa = pa + px
+          p2a = p2a + px**2
+          e = ek + ev
+          write(u8, 10101) 'e', istep, time, ev, ek, e, px
+        end if
+        if (mdmod(istep, nptensor) == 0) write(u8, 10101) 'p', istep, time, recs(r)%pp
+        write(u9) istep, time, ev, ek, px, recs(r)%pp
+      end do
+      kd = kd + nc
+      if (mdmod(istep, nout) == 0) call save_config()
+      if (mdmod(istep, nckpt) == 0) call checkpoint()
     end do
     call block_stats(eva, ev2a, ntot, dev, evave, ev2ave)
     call block_stats(pa, p2a, ntot, dp_, pave, p2ave)
@@ -302,19 +355,8 @@ program ykmd
     close(u9)
   end if
 10200 format(2(a, 1pe16.9))
-  ! end-of-run checkpoint, with the last measured energies (src/md.F:181)
-  if (istep > 0) then
-    call check(yk_get_state(h, x, v, a), 'yk_get_state')
-    pp = 0.0d0
-    if (nrec > 0) then
-      pp(1, 1) = recs(nrec)%pp(1); pp(1, 2) = recs(nrec)%pp(2); pp(1, 3) = recs(nrec)%pp(3)
-      pp(2, 1) = recs(nrec)%pp(2); pp(2, 2) = recs(nrec)%pp(4); pp(2, 3) = recs(nrec)%pp(5)
-      pp(3, 1) = recs(nrec)%pp(3); pp(3, 2) = recs(nrec)%pp(5); pp(3, 3) = recs(nrec)%pp(6)
-    end if
-    write(suffix, '(i5.5,i6.6)') int(time / 1000000.d0), nint(mod(time, 1000000.d0))
-    call write_xvb('md.'//trim(suffix)//'.xv8b', 'xv8b', sim_type, time, xl, ev, ek, px, pp, &
-                   n, x, v, a)
-  end if
+  ! end-of-run checkpoint (src/md.F:176)
+  if (istep > 0 .and. mdmod(istep, nckpt) /= 0) call checkpoint()
   call check(yk_accel(h, 1_c_int32_t, m), 'yk_accel')
   call check(yk_ttot(h, eka), 'yk_ttot')
   ek = eka
@@ -333,6 +375,47 @@ program ykmd
   call yk_destroy(h)
 
 contains
+
+  ! Steps from istep to the next step whose mdmod with q is 0 (q = 0: never).
+  integer function steps_to(istep, q)
+    integer, intent(in) :: istep, q
+    if (q == 0) then
+      steps_to = huge(0)
+    else
+      steps_to = abs(q) - mod(istep, abs(q))
+    end if
+  end function steps_to
+
+  ! checkpoint (src/md_utils.f:19-38): md.<time>.xv8b
+  subroutine checkpoint()
+    character(40) :: sfx
+    call check(yk_get_state(h, x, v, a), 'yk_get_state')
+    write(sfx, '(i5.5,i6.6)') int(time / 1000000.d0), nint(mod(time, 1000000.d0))
+    call write_xvb('md.'//trim(sfx)//'.xv8b', 'xv8b', sim_type, time, xl, hev, hek, hpx, hpp, &
+                   n, x, v, a)
+  end subroutine checkpoint
+
+  ! save_config(ftype, append) (src/md_utils.f:61-114): one file per
+  ! configuration, md.<time>.<ftype>, or appended to md.traj.<tend>.<ftype>
+  subroutine save_config()
+    character(40) :: sfx
+    character(300) :: fname
+    call check(yk_get_state(h, x, v, a), 'yk_get_state')
+    if (append) then
+      write(sfx, '(i5.5,i6.6)') int(tend / 1000000.d0), nint(mod(tend, 1000000.d0))
+      fname = 'md.traj.'//trim(sfx)//'.'//trim(ftype)
+    else
+      write(sfx, '(i5.5,i6.6)') int(time / 1000000.d0), nint(mod(time, 1000000.d0))
+      fname = 'md.'//trim(sfx)//'.'//trim(ftype)
+    end if
+    select case (trim(ftype))
+    case ('x4b', 'xv4b', 'x8b', 'xv8b', 'xva8b')
+      call write_xvb(trim(fname), ftype, sim_type, time, xl, hev, hek, hpx, hpp, n, x, v, a, &
+                     append)
+    case default
+      call write_xva(trim(fname), ftype, time, n, x, v, a, append)
+    end select
+  end subroutine save_config
 
   subroutine check(rc, what)
     integer(c_int), intent(in) :: rc

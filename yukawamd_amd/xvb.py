@@ -1,5 +1,5 @@
-"""Reader of the reference's revision-b configuration files ('x8b', 'xv8b',
-'xva8b'; src/rw_xvb.f:148-203): Fortran unformatted sequential records,
+"""Reader of the reference's revision-b configuration files ('x4b', 'xv4b',
+'x8b', 'xv8b', 'xva8b'; src/rw_xvb.f:148-203): Fortran unformatted sequential records,
 big-endian, each framed by 4-byte lengths -- file type; code name and
 version; date, time, zone; sim_type; (time, box, ev, ek, px, pp(3,3), n);
 the particle record."""
@@ -17,14 +17,17 @@ def _records(raw: bytes):
     return out
 
 
-def read_xvb(path) -> dict:
-    recs = _records(open(path, "rb").read())
+_COLS = {"x4b": (3, ">f4"), "xv4b": (6, ">f4"), "x8b": (3, ">f8"), "xv8b": (6, ">f8"),
+         "xva8b": (9, ">f8")}
+
+
+def _config(recs) -> dict:
     ftype = recs[0].decode().strip()
     head = recs[4]
     vals = np.frombuffer(head[:16 * 8], dtype=">f8")
     n = int(np.frombuffer(head[16 * 8:16 * 8 + 4], dtype=">i4")[0])
-    cols = {"x8b": 3, "xv8b": 6, "xva8b": 9}[ftype]
-    data = np.frombuffer(recs[5], dtype=">f8").reshape(n, cols).astype(np.float64)
+    cols, dt = _COLS[ftype]
+    data = np.frombuffer(recs[5], dtype=dt).reshape(n, cols).astype(np.float64)
     out = {"ftype": ftype, "code": recs[1].decode().strip(), "sim_type": recs[3].decode().strip(),
            "time": float(vals[0]), "xl": tuple(float(v) for v in vals[1:4]), "ev": float(vals[4]),
            "ek": float(vals[5]), "px": float(vals[6]), "pp": vals[7:16].reshape(3, 3).T.copy(),
@@ -36,4 +39,15 @@ def read_xvb(path) -> dict:
     return out
 
 
-__all__ = ["read_xvb"]
+def read_xvb(path) -> dict:
+    """The first (usually only) configuration of a revision-b file."""
+    return _config(_records(open(path, "rb").read())[:6])
+
+
+def read_xvb_all(path) -> list:
+    """Every configuration of a revision-b trajectory (six records each)."""
+    recs = _records(open(path, "rb").read())
+    return [_config(recs[k:k + 6]) for k in range(0, len(recs), 6)]
+
+
+__all__ = ["read_xvb", "read_xvb_all"]
