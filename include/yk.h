@@ -95,7 +95,7 @@ typedef struct yk_params {
     uint32_t flags;      /* enum yk_flags                                     */
     int32_t coulomb_on;  /* coulomb .eq. 'screened-coulomb' (int_nn.f:167)    */
     int32_t nuclear_on;  /* nuclear .eq. 'HPP'              (int_nn.f:168)    */
-    int32_t resort_every;/* steps between Morton re-sorts of the device slots */
+    int32_t resort_every;/* steps between Hilbert re-sorts of the device slots */
                          /* (0 = library default); results are identical up   */
                          /* to summation order whatever the value             */
     double xl[3];        /* box edge lengths (fm)                             */
@@ -290,7 +290,7 @@ int yk_batch_set_tuning(yk_batch *b, int32_t nbuckets, const int32_t *n_upper,
                         const int32_t *chunk_tiles, const int32_t *waves_per_block,
                         const int32_t *list_items);
 
-/* Re-sort a replica's slots into Morton order every `every` of its own steps
+/* Re-sort a replica's slots into Hilbert order every `every` of its own steps
  * (0: never; default 10), keeping tiles compact as particles move.  The
  * re-sort points depend only on the replica's own step count. */
 int yk_batch_set_resort(yk_batch *b, int32_t every);

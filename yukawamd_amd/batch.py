@@ -75,7 +75,7 @@ class Ensemble:
         self.tuning = list(buckets)
 
     def set_resort(self, every: int):
-        """Re-sort each replica into Morton order every `every` of its own steps
+        """Re-sort each replica into Hilbert order every `every` of its own steps
         (0: never)."""
         _lib.check_batch(self.lib.yk_batch_set_resort(self._b, int(every)))
 

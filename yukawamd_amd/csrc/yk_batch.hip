@@ -234,7 +234,7 @@ RepConsts rep_consts(const HostReplica &h)
     c.blk0 = h.off / BLOCK;
     c.nblk = (h.n + BLOCK - 1) / BLOCK;
     c.ref_f1 = (p.flags & YK_FLAG_REF_F1) ? 1 : 0;
-    int g = 1;   // about 2 particles per Morton cell, as at admission
+    int g = 1;   // about 2 particles per Hilbert cell, as at admission
     while (g < 10 && (double)(1 << (3 * g)) < 0.5 * h.n) ++g;
     c.grid_bits = g;
     c.list_items = h.list_items;
@@ -492,7 +492,7 @@ int capture(yk_batch *b, int steps, int nind)
 }
 
 // Re-sort the replicas whose own step count reached a multiple of
-// resort_every (and that still have steps to do) back into Morton order:
+// resort_every (and that still have steps to do) back into Hilbert order:
 // keys, one stable radix sort over the arena, gather into the second buffer
 // set, copy back (the captured step graph keeps its buffer addresses).  A
 // replica's re-sort points depend only on its own step count, so its
@@ -580,20 +580,12 @@ double wrap_into_box(double x, double L)
     return x;
 }
 
-// Morton order of one replica's particles (host; done once per admission).
-std::vector<int> morton_order(const double *x, int n, const double *L)
+// Hilbert order of one replica's particles (host; done once per admission).
+std::vector<int> sfc_order(const double *x, int n, const double *L)
 {
     int g = 1;
     while (g < 10 && (double)(1 << (3 * g)) < 0.5 * n) ++g;
     const double G = (double)(1 << g);
-    auto spread = [](unsigned v) {
-        v &= 0x3ffu;
-        v = (v | (v << 16)) & 0x030000ffu;
-        v = (v | (v << 8)) & 0x0300f00fu;
-        v = (v | (v << 4)) & 0x030c30c3u;
-        v = (v | (v << 2)) & 0x09249249u;
-        return v;
-    };
     std::vector<unsigned> key(n);
     for (int i = 0; i < n; ++i) {
         unsigned c[3];
@@ -604,7 +596,7 @@ std::vector<int> morton_order(const double *x, int n, const double *L)
             ci = ci < 0 ? 0 : (ci > (1 << g) - 1 ? (1 << g) - 1 : ci);
             c[k] = (unsigned)ci;
         }
-        key[i] = spread(c[0]) | (spread(c[1]) << 1) | (spread(c[2]) << 2);
+        key[i] = hilbert_key3(c[0], c[1], c[2], g);
     }
     std::vector<int> order(n);
     std::iota(order.begin(), order.end(), 0);
@@ -822,7 +814,7 @@ int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *
             return rc;
         }
     }
-    const std::vector<int> order = morton_order(x, n, params->xl);
+    const std::vector<int> order = sfc_order(x, n, params->xl);
     // sorted host staging
     const size_t nb = (size_t)nblk * BLOCK;
     std::vector<double4> p4(nb, make_double4(0, 0, 0, 0));

@@ -29,7 +29,7 @@ struct RepConsts {
     float three_n, n_f;
     int kind, prec, off, n;
     int tile0, ntiles, blk0, nblk;
-    int ref_f1, grid_bits;   // grid_bits: Morton cells per axis = 2^grid_bits
+    int ref_f1, grid_bits;   // grid_bits: Hilbert cells per axis = 2^grid_bits
     // cluster lists (list_items > 0): the replica's region of the list arena
     // (entries), segment capacity, list radius^2, skin and the FP32 bounds of
     // the candidate test (as ListConsts)
@@ -491,7 +491,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_permute(const int *__restrict__
 }
 
 // Keys of an arena re-sort (a stable radix sort of 64-bit keys over every
-// slot).  A slot of a replica in `due` gets (replica offset << 32 | Morton
+// slot).  A slot of a replica in `due` gets (replica offset << 32 | Hilbert
 // cell of its position), and the position is wrapped into the box (a
 // particle that crossed a face joins a tile on the side it is now on, so
 // tiles stay compact); every other slot, unused ones included, gets
@@ -526,8 +526,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_sort_keys(double4 *__restrict__
             c[d] = (unsigned)ci;
         }
         pos[i] = make_double4(xs[0], xs[1], xs[2], p.w);
-        k = ((unsigned long long)rc.off << 32) |
-            (unsigned long long)(spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2));
+        k = ((unsigned long long)rc.off << 32) | (unsigned long long)hilbert_key3(c[0], c[1], c[2], g);
     }
     key[i] = k;
     val[i] = i;

@@ -2,7 +2,7 @@
 // hot path for gfx950 (CDNA4, wave64).
 //
 // Layout in HBM (structure of arrays, one system per handle, particles kept
-// in Morton order of a 2^g x 2^g x 2^g cell grid so that 64 consecutive
+// in Hilbert-curve order of a 2^g x 2^g x 2^g cell grid so that 64 consecutive
 // particles -- one wave, one "tile" -- occupy a compact region):
 //   pos[n]    double4 {x, y, z, Z}   -- position and charge number, 32 B
 //   vel[3n]   component-major vx[n] | vy[n] | vz[n]
@@ -28,6 +28,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "yk_math.hpp"
+#include "yk_sfc.hpp"
 
 namespace yk {
 
@@ -1188,25 +1189,13 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// Morton keys of the current slots (cell grid 2^g per axis) and the gather
+// Hilbert keys of the current slots (cell grid 2^g per axis) and the gather
 // that applies a sort permutation to every per-particle array.
-__device__ __forceinline__ unsigned spread3(unsigned v)
-{
-    v &= 0x3ffu;
-    v = (v | (v << 16)) & 0x030000ffu;
-    v = (v | (v << 8)) & 0x0300f00fu;
-    v = (v | (v << 4)) & 0x030c30c3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
-
-// Also wraps every position into [0, L) (the slots are about to move and
-// every pair list is rebuilt afterwards).
-static __global__ __launch_bounds__(BLOCK) void k_morton(double4 *__restrict__ pos, int n,
-                                                  double gx, double gy, double gz, int gmax,
-                                                  double Lx, double Ly, double Lz,
-                                                  unsigned *__restrict__ key,
-                                                  int *__restrict__ val)
+static __global__ __launch_bounds__(BLOCK) void k_hilbert(double4 *__restrict__ pos, int n,
+                                                   double gx, double gy, double gz, int g,
+                                                   double Lx, double Ly, double Lz,
+                                                   unsigned *__restrict__ key,
+                                                   int *__restrict__ val)
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
@@ -1215,11 +1204,12 @@ static __global__ __launch_bounds__(BLOCK) void k_morton(double4 *__restrict__ p
     p.y -= Ly * __builtin_floor(p.y / Ly);
     p.z -= Lz * __builtin_floor(p.z / Lz);
     pos[i] = p;
+    const int gmax = (1 << g) - 1;
     int cx = (int)(p.x * gx), cy = (int)(p.y * gy), cz = (int)(p.z * gz);
     cx = cx < 0 ? 0 : (cx > gmax ? gmax : cx);
     cy = cy < 0 ? 0 : (cy > gmax ? gmax : cy);
     cz = cz < 0 ? 0 : (cz > gmax ? gmax : cz);
-    key[i] = spread3((unsigned)cx) | (spread3((unsigned)cy) << 1) | (spread3((unsigned)cz) << 2);
+    key[i] = hilbert_key3((unsigned)cx, (unsigned)cy, (unsigned)cz, g);
     val[i] = i;
 }
 
@@ -1408,7 +1398,7 @@ __global__ __launch_bounds__(BLOCK) void k_drift(double4 *__restrict__ pos,
             }
             // No periodic wrap here: positions stay continuous between
             // re-sorts so that the periodic image stored with every pair-list
-            // entry stays valid.  k_morton wraps them into the box at each
+            // entry stays valid.  k_hilbert wraps them into the box at each
             // re-sort and the host boundary wraps them on download with the
             // reference's rule (src/vel_verlet_2.f:39-40).
             x[k] = xk;

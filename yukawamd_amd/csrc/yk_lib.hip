@@ -58,7 +58,7 @@ struct yk_system {
     // caller-order host copies of the per-particle constants
     std::vector<double> zhost, ahost, dhost;
     hipStream_t stream = nullptr;
-    // device state, in Morton-sorted slot order (second set: permutation target)
+    // device state, in Hilbert-sorted slot order (second set: permutation target)
     double4 *pos = nullptr, *pos2 = nullptr;
     double *vel = nullptr, *vold = nullptr, *acc = nullptr;
     double *vel2 = nullptr, *vold2 = nullptr, *acc2 = nullptr;
@@ -329,7 +329,7 @@ void choose_decomposition(yk_system *s)
     if (want > s->nstages) want = s->nstages;
     s->stages_per_chunk = (s->nstages + want - 1) / want;
     s->nchunks = (s->nstages + s->stages_per_chunk - 1) / s->stages_per_chunk;
-    // Morton grid: about 1-2 particles per cell
+    // Hilbert grid: about 1-2 particles per cell
     int g = 1;
     while (g < 10 && (double)(1 << (3 * g)) < 0.5 * n) ++g;
     s->grid_bits = g;
@@ -575,15 +575,15 @@ void zero_meas(yk_meas *out)
     if (out) std::memset(out, 0, sizeof *out);
 }
 
-// Re-establish Morton order of the slots: keys, stable radix sort, gather
+// Re-establish Hilbert order of the slots: keys, stable radix sort, gather
 // of every per-particle array into the second buffer set, swap, tile AABBs.
 int resort(yk_system *s)
 {
     const int n = s->n;
     const int g = s->grid_bits;
     const double G = (double)(1 << g);
-    hipLaunchKernelGGL(k_morton, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos, n,
-                       G / s->p.xl[0], G / s->p.xl[1], G / s->p.xl[2], (1 << g) - 1,
+    hipLaunchKernelGGL(k_hilbert, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos, n,
+                       G / s->p.xl[0], G / s->p.xl[1], G / s->p.xl[2], g,
                        s->p.xl[0], s->p.xl[1], s->p.xl[2], s->keys, s->vals);
     YK_HIP(hipGetLastError());
     size_t bytes = s->sort_tmp_bytes;
@@ -899,7 +899,7 @@ int yk_put_state(yk_system *s, const double *x, const double *v)
     if (!finite3(x, n)) return fail(YK_ERR_NONFINITE, "positions contain NaN/Inf");
     if (v && !finite3(v, n)) return fail(YK_ERR_NONFINITE, "velocities contain NaN/Inf");
     YK_HIP(hipStreamSynchronize(s->stream));
-    // caller order first (orig = identity), then Morton-sort on the device
+    // caller order first (orig = identity), then Hilbert-sort on the device
     std::vector<double4> p4(nn);
     std::vector<int> iota(nn);
     for (size_t i = 0; i < nn; ++i) {
