@@ -43,12 +43,16 @@ def runmd(species, seed, extra=""):
 """
 
 
-def run(cmd, text, workdir):
+def run_only(cmd, text, workdir):
     workdir.mkdir(parents=True, exist_ok=True)
     (workdir / "runmd.in").write_text(text)
     env = dict(os.environ, OMP_NUM_THREADS="1")
     out = subprocess.run(cmd, cwd=workdir, env=env, capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+
+
+def run(cmd, text, workdir):
+    run_only(cmd, text, workdir)
     return read_xvb(next(workdir.glob("md.*.xv8b")))
 
 
@@ -84,3 +88,55 @@ def test_generator_against_numpy_mt19937(tmp_path, seed):
     y = ours["x"] / np.array(ours["xl"]) * 4294967295.0
     assert np.abs(y.ravel() - np.round(y.ravel())).max() < 1e-3
     assert np.array_equal(np.round(y.ravel()), raw)
+
+
+def echo(workdir):
+    """The log's parameter echo, 'SIMULATION TYPE' to the 'nckpt' line."""
+    lines = next(workdir.glob("md.*.log")).read_text().splitlines()
+    i = next(k for k, ln in enumerate(lines) if "SIMULATION TYPE" in ln)
+    j = next(k for k, ln in enumerate(lines) if ln.startswith("  nckpt:"))
+    return lines[i:j + 1]
+
+
+NUCLEON = """&run_parameters
+ sim_type='nucleon', start='random', tstart=0.0, dt=2.0,
+ nwgroup=0, nwsteps=0, ngroup=0, ntot=0, nind=0, nptensor=0, ngofr=1,
+ spec_list(1)%n=200, spec_list(1)%z=0.0, spec_list(1)%a=1.0,
+ spec_list(2)%n=100, spec_list(2)%z=1.0, spec_list(2)%a=1.0,
+ rho=0.05, kT=1.0, xmass=939.0, rccut=20.0,
+ coulomb='screened-coulomb', xlambda=10.0, nuclear='HPP',
+ iseed=77, irnd=4, nout=7, ftype='xv4b', append=.true., nckpt=11,
+/
+"""
+
+
+@pytest.mark.parametrize("case", ["mixture_thomas_fermi_strain", "nucleon_hpp"])
+def test_log_parameter_echo_matches_reference(tmp_path, case):
+    """The parameter echo of src/input1.F / src/initconfig.f, line for line:
+    Thomas-Fermi screening (no xlambda) with strain rates on two axes and a
+    B field, or the nucleon model with the automatic HPP cutoff, g(r)
+    requested (disabled), the MT19937 line and the output options."""
+    if case == "nucleon_hpp":
+        text = NUCLEON
+    else:
+        text = runmd(MIX, 2468, "deps(1)=1.0e-6, deps(3)=-2.0e-6, bfield=1.0e13, "
+                                "nout=3, ftype='x8b', nckpt=9,")
+    run([str(MD_OMP)], text, tmp_path / "ref")
+    run([str(YKMD), "--init-only"], text, tmp_path / "ours")
+    assert echo(tmp_path / "ours") == echo(tmp_path / "ref")
+
+
+def test_xv8b_start_echo_and_box_match_reference(tmp_path):
+    """A start from an xv8b file: the density is recomputed from the file's
+    box and n, and with no aspect given the gage box is the file's box
+    (src/initconfig.f:295-308); the echo names the file."""
+    run([str(MD_OMP)], runmd(MIX, 1357), tmp_path / "gen")
+    start = next((tmp_path / "gen").glob("md.*.xv8b"))
+    text = runmd(MIX, 1357).replace("start='random', tstart=0.0,", "start='start.xv8b',") \
+                           .replace("rho=7.18e-5,", "rho=5.0e-5,")
+    for d in ("ref", "ours"):
+        (tmp_path / d).mkdir()
+        (tmp_path / d / "start.xv8b").write_bytes(start.read_bytes())
+    run_only([str(MD_OMP)], text, tmp_path / "ref")
+    run_only([str(YKMD), "--init-only"], text, tmp_path / "ours")
+    assert echo(tmp_path / "ours") == echo(tmp_path / "ref")

@@ -13,8 +13,8 @@
 ! the ncom / ntnorm / strain / nind / nptensor cadences (one yk_run_md call
 ! per group), block statistics, the final accel and ttot.
 !
-! Output, in the reference's formats: the 'e', 'p' and 's' lines and the
-! initial / final lines of the log (md.<tend>.log), the binary measurement
+! Output, in the reference's formats: the log (md.<tend>.log: the parameter
+! echo, the 'e', 'p' and 's' lines and the initial / final lines), the binary measurement
 ! records (md.<tend>.meas, src/md.F:151), the initial configuration
 ! (md.<tstart>.xv8b, src/initconfig.f:385-390), the configurations saved
 ! every nout steps of the measurement phase (save_config, src/md_utils.f:61-114:
@@ -31,7 +31,7 @@
 ! YK_PRECISION=mixed selects the mixed-precision kernels; the mixture
 ! kernel's last-particle behaviour of the reference (src/int_ion_mix.f:158)
 ! is reproduced unless YK_FIX_F1=1.  `ykmd --init-only` stops after writing
-! the initial configuration (no device needed).
+! the initial configuration and the log's parameter echo (no device needed).
 !-------------------------------------------------------------------------------
 program ykmd
   use iso_c_binding
@@ -72,6 +72,7 @@ program ykmd
   real(dp) :: x1, x2, rhof
   real(dp) :: hev, hek, hpx, hpp(3, 3)   ! ev, ek, px, pp of src/md_globals.f now
   character(40) :: suffix
+  character(20) :: sim_type_in
   character(256) :: env, arg
   type(yk_params_t) :: p
   type(yk_meas_t) :: m
@@ -88,6 +89,7 @@ program ykmd
   if (ios /= 0) call fail('error reading namelist run_parameters from runmd.in')
   close(10)
 
+  sim_type_in = sim_type
   if (nout /= 0) then
     select case (trim(ftype))
     case ('x4b', 'xv4b', 'x8b', 'xv8b', 'xva8b', 'md0', 'x4a', 'xv4a', 'x8a', 'xv8a', 'xva8a')
@@ -140,10 +142,18 @@ program ykmd
     call read_xvb(trim(start), time, xl, k, x, v, irtn)
     if (irtn /= 0) call fail('cannot read start file (xv8b expected): '//trim(start))
     if (k /= n) call fail('start file holds another particle count than spec_list')
-    if (tstart >= 0.0d0) time = tstart
-    if (aspect(1) * aspect(2) * aspect(3) == 0.0d0) aspect = 1.0d0
-    xlf = (real(real(n), dp) / (aspect(1) * aspect(2) * aspect(3) * rho))**(1.0d0 / 3.0d0)
-    xl0 = aspect * xlf
+    if (tstart < 0.0d0) tstart = time
+    time = tstart
+    ! the density and the gage box follow the file's box (src/initconfig.f:299-308)
+    rho = real(real(n), dp) / (xl(1) * xl(2) * xl(3))
+    if (aspect(1) * aspect(2) * aspect(3) == 0.0d0) then
+      xlf = minval(xl)
+      aspect = xl / xlf
+      xl0 = xl
+    else
+      xlf = (real(real(n), dp) / (aspect(1) * aspect(2) * aspect(3) * rho))**(1.0d0 / 3.0d0)
+      xl0 = aspect * xlf
+    end if
     stretch = xl / xl0
   end select
   if (trim(start) == 'random') then
@@ -152,8 +162,6 @@ program ykmd
     call write_xvb('md.'//trim(suffix)//'.xv8b', 'xv8b', sim_type, time, xl, 0.0d0, &
                    0.0d0, 0.0d0, pp, n, x, v, a)
   end if
-  call get_command_argument(1, arg)
-  if (trim(arg) == '--init-only') stop
 
   ! derived parameters (src/input1.F:184-196, :222, :255-311)
   xl = stretch * xl0
@@ -173,6 +181,16 @@ program ykmd
     end if
   end if
   if (trim(nuclear) == 'HPP' .and. rncut == 0.0d0) rncut = auto_rncut(aa, bb, cc, xpacket)
+
+  ! log file (src/md.F:44-55)
+  tend = tstart + dt * (nwgroup * nwsteps + ngroup * ntot * nind)
+  write(suffix, '(i5.5,i6.6)') int(tend / 1000000.d0), nint(mod(tend, 1000000.d0))
+  open(newunit=u8, file='md.'//trim(suffix)//'.log', form='formatted', status='unknown')
+  call echo_parameters(u8)
+
+  call get_command_argument(1, arg)
+  if (trim(arg) == '--init-only') stop
+
 
   ! the device system
   select case (trim(sim_type))
@@ -210,15 +228,6 @@ program ykmd
   call check(yk_create(p, int(n, c_int32_t), zii, aii, h), 'yk_create')
   call check(yk_put_state(h, x, v), 'yk_put_state')
 
-  ! log file (src/md.F:44-55)
-  tend = tstart + dt * (nwgroup * nwsteps + ngroup * ntot * nind)
-  write(suffix, '(i5.5,i6.6)') int(tend / 1000000.d0), nint(mod(tend, 1000000.d0))
-  open(newunit=u8, file='md.'//trim(suffix)//'.log', form='formatted', status='unknown')
-  write(u8, '(a)') '  ykmd: MD run on the GPU library (libyukawa_hip.so)'
-  write(u8, '(a,a)') '  sim_type          = ', trim(sim_type)
-  write(u8, '(a,i10)') '  n                 = ', n
-  write(u8, '(a,3(2x,f17.9),a)') '  config dimensions =', xl, ' fm'
-  write(u8, '(a,f13.8,a)') '  screening length  = ', merge(1.0d0 / xmuc, 0.0d0, xmuc > 0.0d0), ' fm'
 
   ! initial state (src/md.F:66-77)
   call check(yk_accel(h, 1_c_int32_t, m), 'yk_accel')
@@ -375,6 +384,87 @@ program ykmd
   call yk_destroy(h)
 
 contains
+
+  ! The parameter echo that opens the reference's log (src/input1.F:82-373,
+  ! src/initconfig.f:197-200, :216, :260, :350, :366-368), with the values this
+  ! run uses; the first line names this program and the OpenMP line is left out.
+  subroutine echo_parameters(u)
+    integer, intent(in) :: u
+    character(8) :: date
+    character(10) :: daytime
+    character(5) :: zone
+    character(16) :: how
+    real(dp) :: lam
+    integer :: ks
+    character(6), parameter :: rngname(0:4) = [character(6) :: 'N/A', 'ran11', 'rannyu', &
+                                                'ran3', 'grnd']
+    write(u, '(2x,a)') 'ykmd F90+HIP'
+    call date_and_time(date, daytime, zone)
+    write(u, '(''  Start date/time = '',a8,2x,a10,2x,a5)') date, daytime, zone
+    write(u, '(''  Number of processes:  '',i6)') 1
+    write(u, '(a)') '  ----------------------------------------------------------'
+    write(u, '(/,''  SIMULATION TYPE:  '',a /''  ------------------------'')') trim(sim_type_in)
+    write(u, '(a)') '  Species  #particles    Z        A'
+    do ks = 1, nspec
+      if (spec_list(ks)%n > 0) write(u, '(3x,i4,4x,i9,2x,f7.2,2x,f7.2)') ks, spec_list(ks)%n, &
+                                     spec_list(ks)%z, spec_list(ks)%a
+    end do
+    write(u, '(''   Total:  '',i9)') n
+    if (irnd > 0) write(u, '(/''  irnd  = '',i9,2x,''('',a,'')''/''  iseed = '',i9)') &
+      irnd, trim(rngname(irnd)), iseed
+    write(u, '(/,''  INITIAL CONFIGURATION'',/''  ---------------------'')')
+    if (trim(start) == 'random') then
+      write(u, '(a)') '  Generating random coordinates in box'
+      write(u, '(a)') '  Generating velocities from Maxwell distribution'
+    else
+      write(u, '(''  file          = '',a)') trim(start)
+    end if
+    write(u, '(''  start time    =  '',f14.2,'' fm/c''/''  time step     =        '',f8.2,'' fm/c'')') &
+      tstart, dt
+    write(u, '(/,''  rho               ='',es16.8,'' fm-3''/''  L0                ='',2x,f17.9, &
+             &'' fm''/''  gage dimensions   ='',3(2x,f17.9),'' fm''/''  gage aspect       ='', &
+             &3(6x,f13.9)/''  strainrates       ='')', advance='no') rho, xlf, xl0, aspect
+    do ks = 1, 3
+      if (deps(ks) == XNOSTRAIN) then
+        write(u, '(19x)', advance='no')
+      else
+        write(u, '(2x,es17.9)', advance='no') deps(ks)
+      end if
+    end do
+    write(u, '('' c/fm'')')
+    write(u, '(''  config dimensions ='',3(2x,f17.9),'' fm''/''  config stretch    ='',3(2x,f17.9))') &
+      xl, stretch
+    write(u, '(/,''  kT                = '',f10.6,'' MeV''/''  nucleon mass      = '',f8.3,'' MeV'')') &
+      kT, xmass
+    if (ngofr > 0) write(u, '(a)') '  *** g(r) and S(q) disabled in this version ***'
+    write(u, '(/''  INTERACTIONS''/''  ------------'')')
+    if (trim(coulomb) == 'screened-coulomb') then
+      write(u, '(2x,a)') trim(coulomb)
+      if (xlambda > 0.0d0) then
+        lam = xlambda
+        how = '(defined)'
+      else
+        lam = 1.0d0 / xmuc
+        how = '(Thomas-Fermi)'
+      end if
+      write(u, '(''     screening length = '',f13.8,'' fm  '',a/''     ion form factor  = '',f13.8/ &
+               &''     rccut            = '',f13.8,'' fm'')') lam, trim(how), frp, rccut
+    end if
+    if (trim(nuclear) == 'HPP') then
+      write(u, '(2x,a)') trim(nuclear)
+      write(u, '(''     aa       = '',f8.3,'' MeV''/''     bb       = '',f8.3,'' MeV''/ &
+               &''     cc       = '',f8.3,'' MeV''/''     Lambda   = '',f8.3,'' fm^2''/ &
+               &''     rncut    = '',f8.3,'' fm'')') aa, bb, cc, xpacket, rncut
+    end if
+    if (bfield > 0.0d0) write(u, '(''  B-field:''/''     B0   = '',1pe10.3,'' Gauss'')') bfield
+    if (efield > 0.0d0) write(u, '(a)') '  *** external E field disabled in this version ***'
+    write(u, '(/,''  TIME STEP INTERVALS''/''  -------------------''/''  nwgroup, nwsteps:      '', &
+             &i8,1x,i8/''  ngroup, ntot, nind:    '',i8,1x,i8,1x,i8/''  ntnorm:                '', &
+             &i8/''  ncom:                  '',i8/''  nptensor:              '',i8)') &
+      nwgroup, nwsteps, ngroup, ntot, nind, ntnorm, ncom, nptensor
+    write(u, '(''  nout, ftype, append:   '',i8,7x,a5,5x,l1/''  nckpt:                 '',i8)') &
+      nout, ftype, append, nckpt
+  end subroutine echo_parameters
 
   ! Steps from istep to the next step whose mdmod with q is 0 (q = 0: never).
   integer function steps_to(istep, q)
