@@ -99,7 +99,18 @@ def tune_bucket(lo, hi, steps, warmup, seed, log):
     return best, results, len(reps), pairs
 
 
+SWEEP_DISABLED = (
+    "the candidate sweep is disabled: a sweep over the current kernels ended in a GPU memory "
+    "fault in the accel pass of one of its candidate ensembles, and the faulting shape and its "
+    "cause have not been found (the shapes not exercised elsewhere are 1-wave workgroups, 2 or "
+    "4 list items per tile and tile items of 2, 16, 32 or 64 source tiles).  "
+    "yukawamd_amd/tuning/batch_gfx950.json keeps the table of an earlier sweep; its shapes "
+    "run in the benchmark and the tests.")
+
+
 def main(argv=None):
+    print("autotune: " + SWEEP_DISABLED, file=sys.stderr)
+    return 2
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=None)
     ap.add_argument("--steps", type=int, default=8)
