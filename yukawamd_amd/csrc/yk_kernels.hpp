@@ -1278,26 +1278,36 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(
         double f[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) f[k] = 0.0;
-        if constexpr (LAYOUT == 0) {
-            for (int c = 0; c < nparts; ++c) {
-                const double *p = part + (size_t)c * NC * nn + i;
+        // part p's NC values for this slot start at base(p) and are stride
+        // apart; the sums keep part order.  Force-only sums (NC = 3) unroll by
+        // 8 parts so that 24 loads are in flight; measuring sums (NC = 11)
+        // keep two parts in flight (unrolled by 8 they ran four times slower).
+        auto sum_parts = [&](auto base, int stride) {
+            if constexpr (NC == 3) {
+#pragma unroll 8
+                for (int m = 0; m < nparts; ++m) {
+                    const double *p = base(m);
 #pragma unroll
-                for (int k = 0; k < NC; ++k) f[k] += p[k * nn];
+                    for (int k = 0; k < NC; ++k) f[k] += p[k * stride];
+                }
+            } else {
+#pragma unroll 2
+                for (int m = 0; m < nparts; ++m) {
+                    const double *p = base(m);
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) f[k] += p[k * stride];
+                }
             }
+        };
+        if constexpr (LAYOUT == 0) {
+            sum_parts([&](int c) { return part + (size_t)c * NC * nn + i; }, n);
         } else if constexpr (LAYOUT == 1) {
             const int t = i / TILE, lane = i % TILE;
-            for (int m = 0; m < nparts; ++m) {
-                const double *p = part + ((size_t)t * nparts + m) * NC * TILE + lane;
-#pragma unroll
-                for (int k = 0; k < NC; ++k) f[k] += p[k * TILE];
-            }
+            sum_parts([&](int m) { return part + ((size_t)t * nparts + m) * NC * TILE + lane; },
+                      TILE);
         } else {
             const int cl = i / CL, gl = i % CL;
-            for (int m = 0; m < nparts; ++m) {
-                const double *p = part + ((size_t)cl * nparts + m) * NC * CL + gl;
-#pragma unroll
-                for (int k = 0; k < NC; ++k) f[k] += p[k * CL];
-            }
+            sum_parts([&](int m) { return part + ((size_t)cl * nparts + m) * NC * CL + gl; }, CL);
         }
         double zi = pos[i].w;
         // the reference's mixture kernel never applies zii to the caller's last
