@@ -221,8 +221,7 @@ RepConsts rep_consts(const HostReplica &h)
     c.vol = p.xl[0] * p.xl[1] * p.xl[2];
     c.muf = (float)p.xmuc;
     c.mul2ef = (float)(p.xmuc * 1.4426950408889634074);
-    c.mul2e_c = (float)(-(p.xmuc * 1.4426950408889634074 - (double)c.mul2ef) *
-                        0.69314718055994530942);
+    c.mul2e_lo = (float)(p.xmuc * 1.4426950408889634074 - (double)c.mul2ef);
     c.three_n = 3.0f * (float)h.n;
     c.n_f = (float)h.n;
     c.kind = (p.sim_type == YK_PURE_ION) ? PURE : MIX;

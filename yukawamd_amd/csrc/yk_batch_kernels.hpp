@@ -25,7 +25,7 @@ struct RepConsts {
     double L[3], invL[3], halfL_m[3];
     double rc2, mu, evcut, cnum, margin, cull2;
     double dt, afac, rho, kT, xmass, vol;
-    float muf, mul2ef, mul2e_c;   // mul2e_c: as ForceConsts::mul2e_c
+    float muf, mul2ef, mul2e_lo;   // mul2e_lo: as ForceConsts::mul2e_lo
     float three_n, n_f;
     int kind, prec, off, n;
     int tile0, ntiles, blk0, nblk;
@@ -74,7 +74,7 @@ __device__ __forceinline__ void batch_item(const RepConsts &rc, const ExpCoeffs 
     c.evcut = rc.evcut;
     c.muf = rc.muf;
     c.mul2ef = rc.mul2ef;
-    c.mul2e_c = rc.mul2e_c;
+    c.mul2e_lo = rc.mul2e_lo;
     c.ec = ec;
     const int t = it.tile;
     const int i = t * TILE + lane;
@@ -169,7 +169,7 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
     c.evcut = rc.evcut;
     c.muf = rc.muf;
     c.mul2ef = rc.mul2ef;
-    c.mul2e_c = rc.mul2e_c;
+    c.mul2e_lo = rc.mul2e_lo;
     c.ec = ec;
     const int M = -it.s1, m = it.s0;
     const int g = lane / CL, gl = lane % CL;
@@ -238,12 +238,22 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
         const double4 *gs = sw + g * CL;
         if (!gen) {
             double4 pn = gs[0];
+            if (kb == CL) {   // a full batch: unrolled, immediate LDS offsets
+#pragma unroll
+                for (int k = 0; k < CL; ++k) {
+                    const double4 pj = pn;
+                    if (k + 1 < CL) pn = gs[k + 1];
+                    interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w, pi.x - pj.x, pi.y - pj.y,
+                                                     pi.z - pj.z, false, acc);
+                }
+            } else {
 #pragma unroll 4
-            for (int k = 0; k < kb; ++k) {
-                const double4 pj = pn;
-                pn = gs[min(k + 1, CL - 1)];
-                interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w, pi.x - pj.x, pi.y - pj.y,
-                                                 pi.z - pj.z, false, acc);
+                for (int k = 0; k < kb; ++k) {
+                    const double4 pj = pn;
+                    pn = gs[min(k + 1, CL - 1)];
+                    interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w, pi.x - pj.x, pi.y - pj.y,
+                                                     pi.z - pj.z, false, acc);
+                }
             }
         } else {
 #pragma unroll 2

@@ -51,10 +51,11 @@ struct ForceConsts {
     double margin;                       // absolute rounding margin for AABB tests
     double xp2i, xpi, aa, aa2, bpc, bmc, vc;
     float muf, mul2ef, xp2il2ef, xpif, aaf, aa2f, bpcf, bmcf, vcf;
-    // -(mu log2 e - mul2ef) ln 2: first-order correction of exp2(-mul2ef r)
-    // for the rounding of mu log2 e to float, which would otherwise bias every
-    // pair at distance r by the same relative amount (mu r x 6e-8 at most)
-    float mul2e_c;
+    // mu log2 e - mul2ef: the rounding error of mu log2 e in float.  The
+    // exponent -(mul2ef r + mul2e_lo r) is formed by one FMA, so the rounding
+    // of mul2ef does not bias every pair at distance r by the same relative
+    // amount (mu r x 6e-8 at most); one rounding of the exponent remains
+    float mul2e_lo;
     int n, ntiles, coul_on, nuc_on;
     ExpCoeffs ec;
     const double *etab_g; // the 2^(j/N) table of exp_neg64t (device memory)
@@ -177,7 +178,7 @@ __device__ __forceinline__ void interact(const ForceConsts &c, double zi, double
                 const float r2f = (float)r2;
                 const float rinv = rsqrt32(r2f);
                 const float r = r2f * rinv;
-                const float ex = exp2_32(-c.mul2ef * r) * __builtin_fmaf(c.mul2e_c, r, 1.0f);
+                const float ex = exp2_32(__builtin_fmaf(-c.mul2ef, r, -c.mul2e_lo * r));
                 const float exr = ex * rinv;
                 float fc = __builtin_fmaf(c.muf, r, 1.0f) * exr * (rinv * rinv);
                 if constexpr (KIND == MIX) fc = staged_zf(zj) * fc;
@@ -223,7 +224,7 @@ __device__ __forceinline__ void interact(const ForceConsts &c, double zi, double
                 if (hc) {
                     float rinv = rsqrt32(r2f);
                     float r = r2f * rinv;
-                    float ex = exp2_32(-c.mul2ef * r) * __builtin_fmaf(c.mul2e_c, r, 1.0f);
+                    float ex = exp2_32(__builtin_fmaf(-c.mul2ef, r, -c.mul2e_lo * r));
                     float exr = c.vcf * ex * rinv;
                     sf += __builtin_fmaf(c.muf, r, 1.0f) * exr * (rinv * rinv);
                     if constexpr (MEAS) phi += (double)exr - c.evcut;
@@ -1156,12 +1157,25 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
         const double4 *gs = sw + g * CL;
         if (!gen) {
             double4 pn = gs[0];
+            if (kb == CL) {
+                // a full batch (all but a share's last): fully unrolled, so
+                // the LDS reads take immediate offsets and a pair needs no
+                // address arithmetic; the next source is read one pair ahead
+#pragma unroll
+                for (int k = 0; k < CL; ++k) {
+                    const double4 pj = pn;
+                    if (k + 1 < CL) pn = gs[k + 1];
+                    interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w, pi.x - pj.x, pi.y - pj.y,
+                                                     pi.z - pj.z, false, acc);
+                }
+            } else {
 #pragma unroll 4
-            for (int k = 0; k < kb; ++k) {
-                const double4 pj = pn;
-                pn = gs[min(k + 1, CL - 1)];
-                interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w, pi.x - pj.x, pi.y - pj.y,
-                                                 pi.z - pj.z, false, acc);
+                for (int k = 0; k < kb; ++k) {
+                    const double4 pj = pn;
+                    pn = gs[min(k + 1, CL - 1)];
+                    interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w, pi.x - pj.x, pi.y - pj.y,
+                                                     pi.z - pj.z, false, acc);
+                }
             }
         } else {
 #pragma unroll 2

@@ -188,7 +188,7 @@ ForceConsts force_consts(const yk_system *s)
     const double LOG2E = 1.4426950408889634074;
     c.muf = (float)p.xmuc;
     c.mul2ef = (float)(p.xmuc * LOG2E);
-    c.mul2e_c = (float)(-(p.xmuc * LOG2E - (double)c.mul2ef) * 0.69314718055994530942);
+    c.mul2e_lo = (float)(p.xmuc * LOG2E - (double)c.mul2ef);
     c.xp2il2ef = (float)(c.xp2i * LOG2E);
     c.xpif = (float)c.xpi;
     c.aaf = (float)c.aa;
