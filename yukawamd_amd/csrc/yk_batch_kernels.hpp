@@ -107,7 +107,7 @@ __device__ __forceinline__ void batch_item(const RepConsts &rc, const ExpCoeffs 
         if (gap2 > rc.cull2) continue;
         const int j = st * TILE + lane;
         double4 pjs = (j - rc.off < rc.n) ? pos[j] : make_double4(nanv, nanv, nanv, 0.0);
-        pjs.w = stage_w<KIND, PREC>(pjs.w);
+        pjs.w = stage_w<KIND, PREC>(pjs.w, c.muf);
         sw[lane] = pjs;
         __builtin_amdgcn_wave_barrier();
         const int self_jj = (st == t) ? lane : -1;
@@ -222,7 +222,7 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
             } else {
                 genl = true;
             }
-            pj.w = stage_w<KIND, PREC>(pj.w);
+            pj.w = stage_w<KIND, PREC>(pj.w, c.muf);
             sw[lane] = pj;
         } else {
             const double q = __builtin_nan("");   // past the share: rejected by every pair test

@@ -109,22 +109,31 @@ __device__ __forceinline__ double min_image(double d, double L, double invL)
 }
 
 // Source charge as staged in LDS.  The mixed-precision ion forms read it as
-// a float: the staging copy keeps (float)Z in the low 32 bits of the w slot,
-// one conversion per staged source instead of one per pair.  Every other form
-// keeps the double.
+// two floats: the staging copy keeps (float)Z in the low 32 bits of the w slot
+// and (float)Z * muf in the high 32 bits, once per staged source, so that a
+// mixture pair forms Z_j (1 + mu r) with one FMA.  Every other form keeps the
+// double.
 template <int KIND, int PREC>
-__device__ __forceinline__ double stage_w(double z)
+__device__ __forceinline__ double stage_w(double z, float muf)
 {
-    if constexpr (PREC == 1 && KIND != NN)
-        return __builtin_bit_cast(double,
-                                  (unsigned long long)__builtin_bit_cast(unsigned, (float)z));
-    else
+    if constexpr (PREC == 1 && KIND != NN) {
+        const float zf = (float)z;
+        return __builtin_bit_cast(
+            double, ((unsigned long long)__builtin_bit_cast(unsigned, zf * muf) << 32) |
+                        (unsigned long long)__builtin_bit_cast(unsigned, zf));
+    } else {
         return z;
+    }
 }
 
 __device__ __forceinline__ float staged_zf(double w)
 {
     return __builtin_bit_cast(float, (unsigned)__builtin_bit_cast(unsigned long long, w));
+}
+
+__device__ __forceinline__ float staged_zmuf(double w)
+{
+    return __builtin_bit_cast(float, (unsigned)(__builtin_bit_cast(unsigned long long, w) >> 32));
 }
 
 // Pair interaction given the minimum-image displacement d = x_i - x_j.
@@ -180,8 +189,10 @@ __device__ __forceinline__ void interact(const ForceConsts &c, double zi, double
                 const float r = r2f * rinv;
                 const float ex = exp2_32(__builtin_fmaf(-c.mul2ef, r, -c.mul2e_lo * r));
                 const float exr = ex * rinv;
-                float fc = __builtin_fmaf(c.muf, r, 1.0f) * exr * (rinv * rinv);
-                if constexpr (KIND == MIX) fc = staged_zf(zj) * fc;
+                // Z_j (1 + mu r) for a mixture source (staged Z_j mu and Z_j)
+                const float zl = (KIND == MIX) ? __builtin_fmaf(staged_zmuf(zj), r, staged_zf(zj))
+                                               : __builtin_fmaf(c.muf, r, 1.0f);
+                const float fc = zl * exr * (rinv * rinv);
                 s = (double)fc;
                 if constexpr (MEAS) {
                     phi = (double)exr - c.evcut;
@@ -286,7 +297,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const double4 *__restrict__ pos
         const int jl = jb + threadIdx.x;
         // padded sources are NaN and never evaluated (loops stop at n)
         double4 pjs = (jl < n) ? pos[jl] : make_double4(nanv, nanv, nanv, 0.0);
-        pjs.w = stage_w<KIND, PREC>(pjs.w);
+        pjs.w = stage_w<KIND, PREC>(pjs.w, c.muf);
         sj[threadIdx.x] = pjs;
         __syncthreads();
         if (wave_live) {
@@ -599,7 +610,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_list(const double4 *__restrict_
                 pj.y += c.L[1] * (double)((int)((e_cur >> 28) & 3u) - 1);
                 pj.z += c.L[2] * (double)((int)((e_cur >> 30) & 3u) - 1);
             }
-            pj.w = stage_w<KIND, PREC>(pj.w);
+            pj.w = stage_w<KIND, PREC>(pj.w, c.muf);
             sw[lane] = pj;
         }
         __builtin_amdgcn_wave_barrier();
@@ -1137,7 +1148,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
             } else {
                 genl = true;
             }
-            pj.w = stage_w<KIND, PREC>(pj.w);
+            pj.w = stage_w<KIND, PREC>(pj.w, c.muf);
             sw[lane] = pj;
         } else {
             // past this group's share: a NaN source, which every pair test
