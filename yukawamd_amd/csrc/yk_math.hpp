@@ -9,12 +9,15 @@
 //             (y' = y + y*e*(1/2 + 3e/8), e = 1 - x*y^2).
 //   exp_neg64t e^x (x <= 0) by Tang's table-driven method (P. T. P. Tang,
 //             ACM TOMS 15 (1989) 144): x = (1024 q + j) ln2/1024 + r with
-//             |r| <= ln2/2048 (ln2/1024 split in two parts, from fdlibm's
-//             split of ln 2), e^x = 2^q * 2^(j/1024) * e^r, 2^(j/1024) from a
+//             |r| <= ln2/2048, e^x = 2^q * 2^(j/1024) * e^r, 2^(j/1024) from a
 //             1024-entry table (rounded to double on the host; the kernels
 //             keep it in LDS), e^r by its degree-3 Taylor polynomial
-//             (truncation |r|^4/4! < 5.5e-16 relative, a few ulp; the force
-//             and energy parity needs 1e-12 / 1e-13).  Nine fewer FP64
+//             (truncation |r|^4/4! < 5.5e-16 relative).  The reduction takes
+//             ln2/1024 as one double (one FMA; Tang splits it in two): its
+//             rounding, 5.4e-20 at most, adds |k| * 5.4e-20 = |x| * 8e-17 to
+//             r, a relative error of 1e-15 at the ion kernels' |x| = mu r <=
+//             12 and proportionally less below; the force and energy parity
+//             needs 1e-12 / 1e-13.  Ten fewer FP64
 //             operations per call than a Cody-Waite reduction to
 //             |r| <= ln2/2 with a degree-12 polynomial.
 //   FP32      v_rsq_f32 and v_exp_f32 (2^x) hardware transcendentals.
@@ -35,23 +38,18 @@ struct ExpCoeffs {
     double three_eighths;         // rsqrt correction
     double r2_floor;              // 1e-200, added to r^2 by the FP64 list kernels
     double r2_floor_mixed;        // 1e-20, the same for the mixed form (FP32 1/r^3 stays finite)
-    double l2eN, ln2N_hi, ln2N_lo;   // table-driven reduction (N/ln2, ln2/N split)
+    double l2eN, ln2N;            // table-driven reduction (N/ln2, ln2/N)
     double t3[1];                 // 1/3! (1/2! is written inline)
 };
 
 inline ExpCoeffs exp_coeffs()
 {
     ExpCoeffs e;
-    const double ln2_hi = 6.93147180369123816490e-01;   // fdlibm split of ln 2
-    const double ln2_lo = 1.90821492927058770002e-10;
     e.three_eighths = 0.375;
     e.r2_floor = 1e-200;
     e.r2_floor_mixed = 1e-20;
     e.l2eN = EXP_TN * 1.4426950408889634074;
-    // exact: division by a power of two; ln2_hi has 21 trailing zero bits, so
-    // k * ln2N_hi is exact for |k| < 2^21 (|x| < 1400)
-    e.ln2N_hi = ln2_hi / EXP_TN;
-    e.ln2N_lo = ln2_lo / EXP_TN;
+    e.ln2N = 0.69314718055994530942 / EXP_TN;   // exact division by a power of two
     e.t3[0] = 1.0 / 6.0;
     return e;
 }
@@ -78,8 +76,7 @@ __device__ __forceinline__ double exp_neg64t(double x, const ExpCoeffs &ec,
                                              const double *__restrict__ tab)
 {
     const double k = __builtin_rint(x * ec.l2eN);
-    double r = __builtin_fma(-k, ec.ln2N_hi, x);
-    r = __builtin_fma(-k, ec.ln2N_lo, r);
+    const double r = __builtin_fma(-k, ec.ln2N, x);
     double p = __builtin_fma(r, ec.t3[0], 0.5);   // 1/2 as an inline constant
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
