@@ -111,6 +111,10 @@ SWEEP_DISABLED = (
 def main(argv=None):
     print("autotune: " + SWEEP_DISABLED, file=sys.stderr)
     return 2
+
+
+def sweep(argv=None):
+    """The candidate sweep (disabled: main() does not call it)."""
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=None)
     ap.add_argument("--steps", type=int, default=8)

@@ -796,6 +796,16 @@ int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *
     for (size_t k = 0; k < (size_t)3 * n; ++k)
         if (!std::isfinite(x[k]) || (v && !std::isfinite(v[k])))
             return bfail(YK_ERR_NONFINITE, "positions or velocities contain NaN/Inf");
+    // charges and masses, checked before anything is allocated for the replica
+    auto z_of = [&](int i) {
+        return (params->sim_type == YK_PURE_ION) ? (zii ? zii[i] : params->zi) : zii[i];
+    };
+    auto a_of = [&](int i) {
+        return (params->sim_type == YK_PURE_ION) ? (aii ? aii[i] : params->ai) : aii[i];
+    };
+    for (int i = 0; i < n; ++i)
+        if (!(a_of(i) > 0.0) || !std::isfinite(z_of(i)))
+            return bfail(YK_ERR_ARG, "particle %d: A=%g Z=%g", i, a_of(i), z_of(i));
     HostReplica &h = b->table[id];
     h = HostReplica{};
     h.p = *params;
@@ -821,10 +831,7 @@ int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *
     std::vector<int> oo(nb, -1);
     for (int k = 0; k < n; ++k) {
         const int i = order[k];
-        const double zi = (params->sim_type == YK_PURE_ION) ? (zii ? zii[i] : params->zi) : zii[i];
-        const double ai = (params->sim_type == YK_PURE_ION) ? (aii ? aii[i] : params->ai) : aii[i];
-        if (!(ai > 0.0) || !std::isfinite(zi))
-            return bfail(YK_ERR_ARG, "particle %d: A=%g Z=%g", i, ai, zi);
+        const double zi = z_of(i), ai = a_of(i);
         double xs[3];
         for (int c = 0; c < 3; ++c) {
             double xc = x[3 * (size_t)i + c];
