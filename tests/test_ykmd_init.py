@@ -140,3 +140,35 @@ def test_xv8b_start_echo_and_box_match_reference(tmp_path):
     run_only([str(MD_OMP)], text, tmp_path / "ref")
     run_only([str(YKMD), "--init-only"], text, tmp_path / "ours")
     assert echo(tmp_path / "ours") == echo(tmp_path / "ref")
+
+
+NUCLEUS = """&run_parameters
+ sim_type='nucleon', start='nuclear', tstart=0.0, dt=2.0,
+ nwgroup=0, nwsteps=0, ngroup=0, ntot=0, nind=0,
+ spec_list(1)%n=126, spec_list(1)%z=0.0, spec_list(1)%a=1.0,
+ spec_list(2)%n=82, spec_list(2)%z=1.0, spec_list(2)%a=1.0,
+ rho=0.01, kT=1.0, xmass=939.0, rccut=20.0, rmax={rmax},
+ coulomb='screened-coulomb', xlambda=10.0, nuclear='HPP',
+ iseed={seed},
+/
+"""
+
+
+@pytest.mark.parametrize("rmax,seed", [(7.5, 208), (1.0e3, 31)])
+def test_nuclear_start_bitexact(tmp_path, rmax, seed):
+    """start='nuclear' (src/initconfig.f:227-253): positions uniform in a
+    sphere of radius min(rmax, half the shortest edge) about the box centre,
+    three draws per particle; the initial configuration bit for bit and the
+    echo (with the radius line) line for line.  rmax = 1000 fm is clamped to
+    half the box."""
+    text = NUCLEUS.format(rmax=rmax, seed=seed)
+    ref = run([str(MD_OMP)], text, tmp_path / "ref")
+    ours = run([str(YKMD), "--init-only"], text, tmp_path / "ours")
+    assert ours["n"] == ref["n"] == 208 and ours["xl"] == ref["xl"]
+    assert np.array_equal(ours["x"], ref["x"])
+    assert np.array_equal(ours["v"], ref["v"])
+    assert echo(tmp_path / "ours") == echo(tmp_path / "ref")
+    # the mathematics: every particle within the radius of the box centre
+    c = 0.5 * np.array(ref["xl"])
+    r = np.linalg.norm(ref["x"] - c, axis=1)
+    assert r.max() <= min(rmax, 0.5 * min(ref["xl"])) * (1 + 1e-12)

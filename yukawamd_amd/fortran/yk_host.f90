@@ -14,6 +14,9 @@
 !                          reference's arithmetic (src/randnums.f:5-25)
 !   random_positions       x(k,i) = xl(k) * u, particle by particle
 !                          (src/initconfig.f:218-223)
+!   sphere_positions       start='nuclear': uniform in a sphere of radius
+!                          rmax about the box centre, three draws per
+!                          particle (src/initconfig.f:227-253)
 !   maxwell_velocities     v(k,i) = sqrt(kT/(A_i xmass)) * g
 !                          (src/initconfig.f:366-372)
 !   host_ttot / host_tnorm / host_cancel_vcom
@@ -65,7 +68,8 @@ module yk_host
   logical, save :: have_spare = .false.
   real(dp), save :: spare = 0.0d0
 
-  public :: mt_seed, mt_uniform, gauss_polar, random_positions, maxwell_velocities, &
+  public :: mt_seed, mt_uniform, gauss_polar, random_positions, sphere_positions, &
+            maxwell_velocities, &
             host_ttot, host_tnorm, host_cancel_vcom, random_box, thomas_fermi_mu, &
             auto_rncut, block_stats, mdmod, write_xvb, write_xva, read_xvb
 
@@ -133,6 +137,25 @@ contains
       end do
     end do
   end subroutine random_positions
+
+  ! r = u1**0.3333333 rmax (the exponent a default-real literal, as in the
+  ! reference), cos(theta) = 2 u2 - 1, phi = 2 pi u3, about the box centre.
+  subroutine sphere_positions(n, xl, rmax, x)
+    integer, intent(in) :: n
+    real(dp), intent(in) :: xl(3), rmax
+    real(dp), intent(out) :: x(3, 0:n - 1)
+    integer :: i
+    real(dp) :: xrr, costheta, sintheta, xphi
+    do i = 0, n - 1
+      xrr = mt_uniform()**.3333333 * rmax
+      costheta = 2. * mt_uniform() - 1.
+      sintheta = sqrt(1.d0 - costheta * costheta)
+      xphi = 2.d0 * PI * mt_uniform()
+      x(1, i) = xrr * sintheta * cos(xphi) + 0.5 * xl(1)
+      x(2, i) = xrr * sintheta * sin(xphi) + 0.5 * xl(2)
+      x(3, i) = xrr * costheta + 0.5 * xl(3)
+    end do
+  end subroutine sphere_positions
 
   subroutine maxwell_velocities(n, aii, kT, xmass, v)
     integer, intent(in) :: n

@@ -5,9 +5,9 @@
 ! Reads the reference's run-parameter file (runmd.in, namelist
 ! run_parameters with the names of src/input1.F:28-53, so the same file
 ! serves both programs), derives the run parameters as src/input1.F and
-! src/initconfig.f do, builds the initial state of a start='random' run with
-! the reference's generator (irnd = 0 or 4: MT19937; yk_host.f90) or reads an
-! 'xv8b' start file, and runs the schedule of src/md.F:66-205 on the GPU:
+! src/initconfig.f do, builds the initial state of a start='random' or
+! start='nuclear' run with the reference's generator (irnd = 0 or 4: MT19937;
+! yk_host.f90) or reads an 'xv8b' start file, and runs the schedule of src/md.F:66-205 on the GPU:
 ! initial accel and ttot, nwgroup warm-up groups (nwsteps steps, then
 ! cancel_vcom and tnorm), ngroup measurement groups of ntot*nind steps with
 ! the ncom / ntnorm / strain / nind / nptensor cadences (one yk_run_md call
@@ -129,12 +129,17 @@ program ykmd
 
   ! initial configuration (src/initconfig.f:202-226, :331-373)
   select case (trim(start))
-  case ('random')
+  case ('random', 'nuclear')
     if (tstart < 0.0d0) call fail('tstart not specified')
     call random_box(n, rho, aspect, xlf, xl0)
     xl = xl0
     allocate(x(3, 0:n - 1), v(3, 0:n - 1))
-    call random_positions(n, xl, x)
+    if (trim(start) == 'random') then
+      call random_positions(n, xl, x)
+    else   ! a sphere about the box centre (src/initconfig.f:227-253)
+      rmax = min(rmax, min(xl(1), xl(2), xl(3)) * 0.5)
+      call sphere_positions(n, xl, rmax, x)
+    end if
     call maxwell_velocities(n, aii, kT, xmass, v)
     call host_cancel_vcom(n, v, aii, xmass, kT)
     time = tstart
@@ -156,7 +161,7 @@ program ykmd
     end if
     stretch = xl / xl0
   end select
-  if (trim(start) == 'random') then
+  if (trim(start) == 'random' .or. trim(start) == 'nuclear') then
     pp = 0.0d0
     write(suffix, '(i5.5,i6.6)') int(time / 1000000.d0), int(mod(time, 1000000.d0))
     call write_xvb('md.'//trim(suffix)//'.xv8b', 'xv8b', sim_type, time, xl, 0.0d0, &
@@ -413,8 +418,12 @@ contains
     if (irnd > 0) write(u, '(/''  irnd  = '',i9,2x,''('',a,'')''/''  iseed = '',i9)') &
       irnd, trim(rngname(irnd)), iseed
     write(u, '(/,''  INITIAL CONFIGURATION'',/''  ---------------------'')')
-    if (trim(start) == 'random') then
-      write(u, '(a)') '  Generating random coordinates in box'
+    if (trim(start) == 'random' .or. trim(start) == 'nuclear') then
+      if (trim(start) == 'random') then
+        write(u, '(a)') '  Generating random coordinates in box'
+      else
+        write(u, '(''  Generating random coordinates in sphere''/''  radius     = '',f8.4,'' fm'')') rmax
+      end if
       write(u, '(a)') '  Generating velocities from Maxwell distribution'
     else
       write(u, '(''  file          = '',a)') trim(start)
