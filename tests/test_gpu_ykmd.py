@@ -10,7 +10,9 @@ bits (tests/test_ykmd_init.py); the device sums forces in another order, so
 the measurement records agree to rounding grown over the run, the final box
 (host arithmetic in both) exactly.  Two more runs add checkpoints every 4th
 step and 'xv8b' configurations every 5th, or an 'x4b' trajectory every 3rd
-step: both programs must write the same files, with the same headers.
+step: both programs must write the same files, with the same headers.  A
+nucleon run from start='nuclear' takes the nucleon kernels through the same
+schedule.
 """
 import os
 import re
@@ -154,3 +156,34 @@ def test_driver_configuration_outputs_match_reference(tmp_path, case):
             if "v" in r:
                 np.testing.assert_allclose(g["v"], r["v"], rtol=1e-7,
                                            atol=1e-10 * np.abs(r["v"]).max(), err_msg=name)
+
+
+NUCLEUS = """&run_parameters
+ sim_type='nucleon', start='nuclear', tstart=0.0, dt=2.0,
+ nwgroup=1, nwsteps=4, ngroup=2, ntot=3, nind=2, nptensor=3, ncom=5, ntnorm=4,
+ spec_list(1)%n=126, spec_list(1)%z=0.0, spec_list(1)%a=1.0,
+ spec_list(2)%n=82, spec_list(2)%z=1.0, spec_list(2)%a=1.0,
+ rho=0.01, kT=1.0, xmass=939.0, rccut=20.0, rmax=7.5,
+ coulomb='screened-coulomb', xlambda=10.0, nuclear='HPP',
+ iseed=208,
+/
+"""
+
+
+def test_driver_nucleus_run_matches_reference(tmp_path):
+    """A nucleus (208 nucleons in a sphere of 7.5 fm, start='nuclear') through
+    the whole schedule: HPP + screened p-p Coulomb kernels, warm-up with
+    cancel_vcom/tnorm, two measurement groups.  Records to rounding grown over
+    the run, the end-of-run checkpoint to rounding."""
+    rlog, rsteps, rvals, rfin = run([str(MD_OMP)], tmp_path / "ref", runmd=NUCLEUS)
+    glog, gsteps, gvals, gfin = run([str(YKMD)], tmp_path / "gpu", runmd=NUCLEUS)
+    assert list(gsteps) == list(rsteps) and len(rsteps) > 0
+    np.testing.assert_allclose(gvals[:, 0], rvals[:, 0], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(gvals[:, 1:], rvals[:, 1:], rtol=1e-9,
+                               atol=1e-12 * np.abs(rvals[:, 1:]).max())
+    assert gfin["xl"] == rfin["xl"]
+    L = np.array(rfin["xl"])
+    dx = np.abs(gfin["x"] - rfin["x"])
+    dx = np.minimum(dx, L - dx)
+    assert dx.max() <= 1e-9 * L.max()
+    np.testing.assert_allclose(gfin["v"], rfin["v"], rtol=1e-7, atol=1e-10 * np.abs(rfin["v"]).max())
