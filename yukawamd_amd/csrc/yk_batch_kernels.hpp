@@ -472,7 +472,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_clist_build(
     }
     __syncthreads();
     int nf, ng;
-    clist_scan<W>(pos + rc.off, rc.n, ctiles, ncand, cv, live, tc, rc.L, rc.invL, rc.Lf, rc.invLf,
+    clist_scan<W>(pos + rc.off, rc.n, ctiles, ncand, cv, live, tc, rc.Lf, rc.invLf,
                   rc.rl2f, stage, clist,
                   rc.lists_off + ((size_t)(cvalid ? cll : 0) * NSEG + seg) * rc.segcap, rc.segcap,
                   lane, wv, nf, ng);

@@ -718,8 +718,10 @@ constexpr int CLIST_MAX_N = 65536;  // above this the tile lists are used
 constexpr int NSEG = 4;
 
 // Candidate test of the cluster-list builds.  A workgroup stages every
-// candidate source once: its offset from the target tile's centre, reduced to
-// the minimum image in FP64 and rounded to float, and in .w the entry's
+// candidate source once: its offset from the target tile's centre (the FP64
+// difference rounded to float, then reduced to the minimum image in FP32: an
+// error of a few float ulps of L, far inside the 1e-6 L and 1e-5 L widening of
+// the test's bounds), and in .w the entry's
 // per-axis image code for the shift that reduction applied (bits 26-31; the
 // source is seen at x + (code - 1) L), bit 0 set when a shift lies beyond one
 // box (such a source gives general entries only).  An absent source (a slot
@@ -728,17 +730,17 @@ constexpr unsigned CODE_MASK = ~SLOT_MASK;
 constexpr float FAR_AWAY = 3.0e38f;
 
 __device__ __forceinline__ float4 clist_stage(const double4 &p, bool present, const double tc[3],
-                                              const double L[3], const double invL[3])
+                                              const float Lf[3], const float invLf[3])
 {
     const double x[3] = {p.x, p.y, p.z};
     float o[3];
     unsigned w = 0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const double d = x[k] - tc[k];
-        const double m = __builtin_rint(d * invL[k]);
-        o[k] = present ? (float)__builtin_fma(-L[k], m, d) : FAR_AWAY;
-        const bool ok = __builtin_fabs(m) <= 1.0;
+        const float d = (float)(x[k] - tc[k]);
+        const float m = __builtin_rintf(d * invLf[k]);
+        o[k] = present ? __builtin_fmaf(-Lf[k], m, d) : FAR_AWAY;
+        const bool ok = __builtin_fabsf(m) <= 1.0f;
         w |= (unsigned)(ok ? 1 - (int)m : 1) << (26 + 2 * k);
         w |= ok ? 0u : 1u;
     }
@@ -878,8 +880,7 @@ __device__ __forceinline__ void clist_round(const ClusterView &v, const float4 (
 template <int W>
 __device__ __forceinline__ void clist_scan(const double4 *__restrict__ pos, int n,
                                            const int *ctiles, int ncand, const ClusterView &cv,
-                                           bool live, const double tc[3], const double L[3],
-                                           const double invL[3], const float Lf[3],
+                                           bool live, const double tc[3], const float Lf[3],
                                            const float invLf[3], float rl2f,
                                            float4 (*stage)[W][TILE], unsigned *__restrict__ clist,
                                            size_t seg_off, int segcap, int lane, int wv, int &nf,
@@ -898,7 +899,7 @@ __device__ __forceinline__ void clist_scan(const double4 *__restrict__ pos, int 
     double4 pa = load(ja), pb = load(jb);
     unsigned uf = 0, ug = 0;
     for (int r = 0; r < nrounds; ++r) {
-        stage[r & 1][wv][lane] = clist_stage(pa, ja < n, tc, L, invL);
+        stage[r & 1][wv][lane] = clist_stage(pa, ja < n, tc, Lf, invLf);
         pa = pb;
         ja = jb;
         jb = slot_of(r + 2);
@@ -991,7 +992,7 @@ static __global__ __launch_bounds__(BLOCK) void k_clist_build(
     }
     __syncthreads();
     int nf, ng;
-    clist_scan<W>(pos, lc.n, ctiles, ncand, cv, live, tc, lc.L, lc.invL, lc.Lf, lc.invLf, lc.rl2f,
+    clist_scan<W>(pos, lc.n, ctiles, ncand, cv, live, tc, lc.Lf, lc.invLf, lc.rl2f,
                   stage, clist, ((size_t)(cvalid ? cl : 0) * NSEG + seg) * segcap, segcap, lane,
                   wv, nf, ng);
     if (cvalid && lane == 0) ccounts[cl * NSEG + seg] = make_int2(nf, ng);
