@@ -60,12 +60,15 @@ def test_strain_matches_oracle(deps):
     assert np.array_equal(a1, a0)
 
 
-def test_run_md_matches_oracle_loop():
-    """13 steps with cancel_vcom every 5, tnorm every 3, an x strain each step,
-    measuring at the nind=4 and nptensor=6 steps (src/md.F:124-133)."""
+@pytest.mark.parametrize("deps", [(2e-6, X, X), (X, X, X)], ids=["strain", "no_strain"])
+def test_run_md_matches_oracle_loop(deps):
+    """13 steps with cancel_vcom every 5, tnorm every 3, with and without an x
+    strain each step, measuring at the nind=4 and nptensor=6 steps
+    (src/md.F:124-133).  Without strain the steps between the utilities run
+    with the next step's drift done by the finalize kernel."""
     w = W.ion_mixture(n=1500, seed=43)
     p = w.params
-    kw = dict(nsteps=13, istep0=0, nind=4, nptensor=6, ncom=5, ntnorm=3, deps=(2e-6, X, X),
+    kw = dict(nsteps=13, istep0=0, nind=4, nptensor=6, ncom=5, ntnorm=3, deps=deps,
               xl0=p.xl)
     with YukawaSystem(p, Z=w.Z, A=w.A, dt=w.dt) as s:
         s.put_state(w.x, w.v)

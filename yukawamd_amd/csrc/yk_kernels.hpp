@@ -1286,13 +1286,29 @@ __device__ __forceinline__ void block_reduce_store(double (&q)[NQ], double *__re
 // LAYOUT 0: partials part[c][NC][n] from k_pairs (nparts chunks);
 // LAYOUT 1: part[t][m][NC][TILE] from k_pairs_list (nparts items per tile);
 // LAYOUT 2: part[cl][m][NC][CL] from k_pairs_clist (nparts items per cluster).
-template <int KIND, bool MEAS, int MODE, int LAYOUT>
+// DRIFT (vel_verlet_2 only): the next step follows at once (same dt, no
+// re-sort or utility in between), so this kernel also does that step's drift
+// for its own slot with k_drift<1>'s arithmetic -- positions, vold, predicted
+// velocity, tile AABBs, displacement tracking -- and the next step launches
+// no k_drift (DriftArgs: the arrays only the drift writes).
+struct DriftArgs {
+    double4 *pos;
+    double *vold;
+    int ntiles;
+    TileBox *box;
+    const double4 *xref;
+    unsigned long long *maxdisp;
+};
+
+template <int KIND, bool MEAS, int MODE, int LAYOUT, bool DRIFT = false>
 __global__ __launch_bounds__(BLOCK) void k_finalize(
     const double *__restrict__ part, int nparts, const double4 *__restrict__ pos,
     const double *__restrict__ den, const double *__restrict__ aii,
     const int *__restrict__ orig, double *__restrict__ acc, double *__restrict__ vel,
-    const double *__restrict__ vold, StepConsts sc, double *__restrict__ blockpart)
+    const double *__restrict__ vold, StepConsts sc, double *__restrict__ blockpart,
+    DriftArgs dr = DriftArgs{})
 {
+    static_assert(!DRIFT || MODE == 1, "the fused drift is vel_verlet_2's");
     const int n = sc.n;
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     const size_t nn = (size_t)n;
@@ -1300,6 +1316,7 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(
     double q[NQ];
 #pragma unroll
     for (int k = 0; k < NQ; ++k) q[k] = 0.0;
+    double xn[3] = {0, 0, 0};   // DRIFT: the next step's positions
     if (i < n) {
         double f[NC];
 #pragma unroll
@@ -1335,7 +1352,9 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(
             const int cl = i / CL, gl = i % CL;
             sum_parts([&](int m) { return part + ((size_t)cl * nparts + m) * NC * CL + gl; }, CL);
         }
-        double zi = pos[i].w;
+        // DRIFT reads and writes positions and vold through dr only
+        const double4 p0 = DRIFT ? dr.pos[i] : pos[i];
+        double zi = p0.w;
         // the reference's mixture kernel never applies zii to the caller's last
         // particle (src/int_ion_mix.f:158, :201, :250); YK_FLAG_REF_F1 keeps that
         double fz_i = (KIND == MIX && !(sc.ref_f1 && orig[i] == n - 1)) ? zi : 1.0;
@@ -1356,9 +1375,9 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(
         if constexpr (MODE == 1) {   // v = 0.5*(v + (vold + dt*a))
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                double vp = vold[k * nn + i] + sc.dt * a[k];
+                double vp = (DRIFT ? dr.vold[k * nn + i] : vold[k * nn + i]) + sc.dt * a[k];
                 v[k] = 0.5 * (vel[k * nn + i] + vp);
-                vel[k * nn + i] = v[k];
+                if constexpr (!DRIFT) vel[k * nn + i] = v[k];
             }
         } else if constexpr (MODE == 2) {
 #pragma unroll
@@ -1391,8 +1410,35 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(
                 q[13] = ai * v[2] * v[2];
             }
         }
+        if constexpr (DRIFT) {   // k_drift<1> (src/vel_verlet_2.f:33-53)
+            const double xo[3] = {p0.x, p0.y, p0.z};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                xn[k] = xo[k] + v[k] * sc.dt + sc.afac * a[k];   // contraction is off
+                dr.vold[k * nn + i] = v[k];
+                vel[k * nn + i] = v[k] + sc.dt * a[k];
+            }
+            dr.pos[i] = make_double4(xn[0], xn[1], xn[2], p0.w);
+        }
     }
     if constexpr (MEAS) block_reduce_store(q, blockpart + (size_t)blockIdx.x * NQ);
+    if constexpr (DRIFT) {
+        const int tile = i / TILE;
+        if (tile < dr.ntiles) tile_box_store(xn, i < n, tile, dr.box);
+        if (dr.xref) {   // largest displacement since the list build
+            double d2 = 0.0;
+            if (i < n) {
+                const double4 r = dr.xref[i];
+                const double xr[3] = {r.x, r.y, r.z};
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    double d = xn[k] - xr[k];
+                    d2 = __builtin_fma(d, d, d2);
+                }
+            }
+            record_disp(dr.maxdisp, d2);
+        }
+    }
 }
 
 // Drift (+ velocity predictor or opening half kick), then the tile AABBs of

@@ -490,8 +490,18 @@ int launch_pairs(yk_system *s, bool meas)
 }
 
 template <int KIND, bool MEAS, int MODE>
-void launch_fin_k(yk_system *s, const StepConsts &sc)
+void launch_fin_k(yk_system *s, const StepConsts &sc, bool drift)
 {
+    if constexpr (MODE == 1) {
+        if (drift && s->use_clist) {   // with the next step's drift
+            const DriftArgs dr{s->pos, s->vold, s->ntiles, s->box,
+                               s->use_list ? s->xref : nullptr, s->maxdisp};
+            hipLaunchKernelGGL((k_finalize<KIND, MEAS, 1, 2, true>), dim3(s->nblocks),
+                               dim3(BLOCK), 0, s->stream, s->part, s->list_M, s->pos, s->den,
+                               s->aii, s->orig, s->acc, s->vel, s->vold, sc, s->blockpart, dr);
+            return;
+        }
+    }
     if (s->use_clist)
         hipLaunchKernelGGL((k_finalize<KIND, MEAS, MODE, 2>), dim3(s->nblocks), dim3(BLOCK), 0,
                            s->stream, s->part, s->list_M, s->pos, s->den, s->aii, s->orig,
@@ -507,29 +517,30 @@ void launch_fin_k(yk_system *s, const StepConsts &sc)
 }
 
 template <int KIND>
-void launch_fin_kind(yk_system *s, const StepConsts &sc, bool meas, int mode)
+void launch_fin_kind(yk_system *s, const StepConsts &sc, bool meas, int mode, bool drift)
 {
     if (meas) {
-        if (mode == 0) launch_fin_k<KIND, true, 0>(s, sc);
-        else if (mode == 1) launch_fin_k<KIND, true, 1>(s, sc);
-        else if (mode == 2) launch_fin_k<KIND, true, 2>(s, sc);
-        else launch_fin_k<KIND, true, 3>(s, sc);
+        if (mode == 0) launch_fin_k<KIND, true, 0>(s, sc, false);
+        else if (mode == 1) launch_fin_k<KIND, true, 1>(s, sc, drift);
+        else if (mode == 2) launch_fin_k<KIND, true, 2>(s, sc, false);
+        else launch_fin_k<KIND, true, 3>(s, sc, false);
     } else {
-        if (mode == 0) launch_fin_k<KIND, false, 0>(s, sc);
-        else if (mode == 1) launch_fin_k<KIND, false, 1>(s, sc);
-        else if (mode == 2) launch_fin_k<KIND, false, 2>(s, sc);
-        else launch_fin_k<KIND, false, 3>(s, sc);
+        if (mode == 0) launch_fin_k<KIND, false, 0>(s, sc, false);
+        else if (mode == 1) launch_fin_k<KIND, false, 1>(s, sc, drift);
+        else if (mode == 2) launch_fin_k<KIND, false, 2>(s, sc, false);
+        else launch_fin_k<KIND, false, 3>(s, sc, false);
     }
 }
 
-// finalize + optional measurement reduction into d_meas[slot]
-int launch_finalize(yk_system *s, double dt, bool meas, int mode, int slot)
+// finalize + optional measurement reduction into d_meas[slot]; drift: the
+// finalize also does the next step's vel_verlet_2 drift (cluster lists only)
+int launch_finalize(yk_system *s, double dt, bool meas, int mode, int slot, bool drift = false)
 {
     StepConsts sc = step_consts(s, dt);
     switch (s->p.sim_type) {
-    case YK_PURE_ION: launch_fin_kind<PURE>(s, sc, meas, mode); break;
-    case YK_ION_MIXTURE: launch_fin_kind<MIX>(s, sc, meas, mode); break;
-    default: launch_fin_kind<NN>(s, sc, meas, mode); break;   // nucleon and B-field test
+    case YK_PURE_ION: launch_fin_kind<PURE>(s, sc, meas, mode, drift); break;
+    case YK_ION_MIXTURE: launch_fin_kind<MIX>(s, sc, meas, mode, drift); break;
+    default: launch_fin_kind<NN>(s, sc, meas, mode, drift); break;   // nucleon and B-field test
     }
     YK_HIP(hipGetLastError());
     if (meas) {
@@ -701,18 +712,30 @@ int do_strain(yk_system *s, double dt, const double *deps, const double *xl0)
 int mdmod(int p, int q) { return q == 0 ? -1 : p % q; }   // src/md.F:248-256
 
 // One step on the device: (periodic re-sort), drift, pairs, finalize
-// (+ measurement into slot).
-int device_step(yk_system *s, double dt, bool meas, int slot)
+// (+ measurement into slot).  Inside one run loop the finalize may also do
+// the next step's drift: the caller passes next_plain when the next step of
+// the same loop follows with the same dt and no utility (cancel_vcom, tnorm,
+// strain) before it; the step fuses when, in addition, the integrator is
+// vel_verlet_2, the cluster lists are in use and no re-sort is due.  *fused
+// tells the caller, which passes it back as drift_done for that next step.
+int device_step(yk_system *s, double dt, bool meas, int slot, bool drift_done = false,
+                bool next_plain = false, bool *fused = nullptr)
 {
     int rc;
-    if (s->resort_every > 0 && s->steps_done > 0 && s->steps_done % s->resort_every == 0)
-        if ((rc = resort(s))) return rc;
+    if (fused) *fused = false;
+    if (!drift_done) {   // a fused drift is only done when no re-sort is due
+        if (s->resort_every > 0 && s->steps_done > 0 && s->steps_done % s->resort_every == 0)
+            if ((rc = resort(s))) return rc;
+    }
     s->steps_done += 1;
-    rc = launch_drift(s, dt);
-    if (rc) return rc;
+    if (!drift_done && (rc = launch_drift(s, dt))) return rc;
     if ((rc = launch_pairs(s, meas))) return rc;
     const int mode = (s->p.flags & YK_FLAG_SW) ? 3 : (s->p.flags & YK_FLAG_KDK) ? 2 : 1;
-    return launch_finalize(s, dt, meas, mode, slot);
+    const bool fuse = next_plain && mode == 1 && s->use_clist &&
+                      !(s->resort_every > 0 && s->steps_done % s->resort_every == 0);
+    if ((rc = launch_finalize(s, dt, meas, mode, slot, fuse))) return rc;
+    if (fused) *fused = fuse;
+    return YK_OK;
 }
 
 void release(yk_system *s)
@@ -1041,9 +1064,11 @@ int yk_run(yk_system *s, double dt, int32_t nsteps, int32_t nind, yk_meas *recor
         if (!s->step1) YK_HIP(hipEventCreate(&s->step1));
         YK_HIP(hipEventRecord(s->step0, s->stream));
     }
+    bool ahead = false;   // this step's drift was done by the previous finalize
     for (int st = 1; st <= nsteps; ++st) {
         const bool meas = (nind > 0) && (st % nind == 0);
-        if ((rc = device_step(s, dt, meas, meas ? st / nind - 1 : 0))) return rc;
+        if ((rc = device_step(s, dt, meas, meas ? st / nind - 1 : 0, ahead, st < nsteps, &ahead)))
+            return rc;
     }
     if (timing) {
         YK_HIP(hipEventRecord(s->step1, s->stream));
@@ -1135,13 +1160,20 @@ int yk_run_md(yk_system *s, const yk_loop *lp, yk_meas *records, int32_t *rec_st
         YK_HIP(hipEventRecord(s->step0, s->stream));
     }
     int slot = 0;
+    const bool straining = lp->deps[0] != YK_XNOSTRAIN || lp->deps[1] != YK_XNOSTRAIN ||
+                           lp->deps[2] != YK_XNOSTRAIN;
+    bool ahead = false;   // this step's drift was done by the previous finalize
     for (int st = 0; st < lp->nsteps; ++st) {   // src/md.F:124-133
         const int is = lp->istep0 + st;
+        // (with ahead set none of these is due: the previous step checked)
         if (mdmod(is, lp->ncom) == 0 && (rc = do_cancel_vcom(s))) return rc;
         if (mdmod(is, lp->ntnorm) == 0 && (rc = do_tnorm(s))) return rc;
         if ((rc = do_strain(s, lp->dt, lp->deps, lp->xl0))) return rc;
         const bool meas = mdmod(is + 1, lp->nind) == 0 || mdmod(is + 1, lp->nptensor) == 0;
-        if ((rc = device_step(s, lp->dt, meas, meas ? slot : 0))) return rc;
+        const bool next_plain = st + 1 < lp->nsteps && !straining &&
+                                mdmod(is + 1, lp->ncom) != 0 && mdmod(is + 1, lp->ntnorm) != 0;
+        if ((rc = device_step(s, lp->dt, meas, meas ? slot : 0, ahead, next_plain, &ahead)))
+            return rc;
         if (meas) {
             if (rec_steps && slot < max_records) rec_steps[slot] = is + 1;
             ++slot;
