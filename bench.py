@@ -186,8 +186,9 @@ def main():
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(independent systems shard one per GPU; see the sharding design)",
+        print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported "
+              "(a single system is not sharded across GPUs; independent replicas would run "
+              "one ensemble per GPU)",
               file=sys.stderr)
         return 2
     from yukawamd_amd import workloads as W

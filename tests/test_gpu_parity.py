@@ -176,6 +176,34 @@ def test_ion_mixture_27648_properties():
     assert np.array_equal(xa, xb) and np.array_equal(va, vb) and np.array_equal(aa, ab)
 
 
+@pytest.mark.parametrize("kind", ["pure", "mixture"])
+def test_run_is_repeated_newton_across_resorts(kind):
+    """A run loop does each step's drift inside the previous step's finalize
+    except before a re-sort and at its end; 23 steps with a re-sort every 5
+    and measurements every 3rd cross all of these and must equal 23 single
+    newton calls bit for bit, records included."""
+    w = W.ion_mixture(n=3000, seed=77) if kind == "mixture" else W.pure_ion(n=3000, seed=77)
+    p = w.params.replace(resort_every=5)
+    out = []
+    for loop in (False, True):
+        with YukawaSystem(p, Z=w.Z, A=w.A, dt=w.dt) as s:
+            s.put_state(w.x, w.v)
+            s.accel(True)
+            if loop:
+                recs = s.run(23, nind=3)
+            else:
+                recs = [s.newton(st % 3 == 0) for st in range(1, 24)]
+                recs = [r for st, r in enumerate(recs, 1) if st % 3 == 0]
+            out.append((s.get_state(), recs))
+    (sa, ra), (sb, rb) = out
+    for u, v in zip(sa, sb):
+        assert np.array_equal(u, v)
+    assert len(ra) == len(rb) == 7
+    for r1, r2 in zip(ra, rb):
+        for k in ("ev", "ek", "px", "vir", "pp"):
+            assert np.array_equal(r1[k], r2[k]), k
+
+
 def test_force_independent_of_measurement():
     """Unlike the reference's two branches (F2), the device force does not depend
     on do_measurements: both templates accumulate it with the same FMAs."""
