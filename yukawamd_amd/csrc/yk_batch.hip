@@ -207,8 +207,10 @@ RepConsts rep_consts(const HostReplica &h)
         c.invL[k] = 1.0 / p.xl[k];
         c.halfL_m[k] = 0.5 * p.xl[k] - 4.0 * c.margin;
     }
-    c.rc2 = p.rccut * p.rccut;
-    c.cull2 = c.rc2 * (1.0 + 1e-12) + c.margin;
+    c.cull2 = p.rccut * p.rccut * (1.0 + 1e-12) + c.margin;
+    // the cutoff the pair test uses (exp_cutoff2: unchanged for every pair
+    // that contributes)
+    c.rc2 = exp_cutoff2(p.rccut * p.rccut, p.xmuc, true);
     c.mu = p.xmuc;
     double hd = std::sqrt(p.xl[0] * p.xl[0] + p.xl[1] * p.xl[1] + p.xl[2] * p.xl[2]) / 2.0;
     c.evcut = (p.rccut <= hd) ? std::exp(-(p.xmuc * p.rccut)) / p.rccut : 0.0;

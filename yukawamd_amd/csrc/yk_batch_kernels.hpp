@@ -76,6 +76,7 @@ __device__ __forceinline__ void batch_item(const RepConsts &rc, const ExpCoeffs 
     c.mul2ef = rc.mul2ef;
     c.mul2e_lo = rc.mul2e_lo;
     c.ec = ec;
+    if constexpr (PREC == 0) pin_vgpr(c.ec.shifter);
     const int t = it.tile;
     const int i = t * TILE + lane;
     const int ilocal = i - rc.off;
@@ -171,6 +172,7 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
     c.mul2ef = rc.mul2ef;
     c.mul2e_lo = rc.mul2e_lo;
     c.ec = ec;
+    if constexpr (PREC == 0) pin_vgpr(c.ec.shifter);
     const int M = -it.s1, m = it.s0;
     const int g = lane / CL, gl = lane % CL;
     const int t = it.tile;

@@ -267,6 +267,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const double4 *__restrict__ pos
     int nswept = 0;   // source tiles this wave evaluated
     __shared__ double4 sj[STAGE];
     const int n = c.n;
+    if constexpr (PREC == 0) pin_vgpr(c.ec.shifter);
     const int lane = threadIdx.x & (TILE - 1);
     if constexpr (PREC == 0) {
         __shared__ double etab_s[EXP_TN];
@@ -565,6 +566,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_list(const double4 *__restrict_
 {
     __shared__ double4 sj[BLOCK / TILE][TILE];
     const int n = c.n;
+    if constexpr (PREC == 0) pin_vgpr(c.ec.shifter);
     // the build kernel (previous launch) has consumed the displacement flag
     if (blockIdx.x == 0 && threadIdx.x == 0 && lists_stale(maxdisp, skin)) {
         maxdisp[0] = 0ull;
@@ -1082,6 +1084,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
 {
     __shared__ double4 sj[BLOCK / TILE][TILE];
     const int n = c.n;
+    if constexpr (PREC == 0) pin_vgpr(c.ec.shifter);
     // the build kernels (previous launches) have consumed the displacement flag
     if (blockIdx.x == 0 && threadIdx.x == 0 && lists_stale(maxdisp, skin)) {
         maxdisp[0] = 0ull;

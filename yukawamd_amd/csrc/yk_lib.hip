@@ -174,11 +174,13 @@ ForceConsts force_consts(const yk_system *s)
         c.L[k] = p.xl[k];
         c.invL[k] = 1.0 / p.xl[k];
     }
-    c.rc2 = p.rccut * p.rccut;
-    c.rn2 = p.rncut * p.rncut;
+    // pair cutoffs as the kernels test them (exp_cutoff2: unchanged for every
+    // pair that contributes)
+    c.rc2 = exp_cutoff2(p.rccut * p.rccut, p.xmuc, true);
     c.mu = p.xmuc;
     c.evcut = evcut_of(p);
     c.xp2i = (p.xpacket > 0) ? 1.0 / (2.0 * p.xpacket) : 0.0;
+    c.rn2 = exp_cutoff2(p.rncut * p.rncut, c.xp2i, false);
     c.xpi = (p.xpacket > 0) ? 1.0 / p.xpacket : 0.0;
     c.aa = p.aa;
     c.aa2 = 2.0 * p.aa;

@@ -19,7 +19,12 @@
 //             12 and proportionally less below; the force and energy parity
 //             needs 1e-12 / 1e-13.  Ten fewer FP64
 //             operations per call than a Cody-Waite reduction to
-//             |r| <= ln2/2 with a degree-12 polynomial.
+//             |r| <= ln2/2 with a degree-12 polynomial.  k comes from one FMA
+//             against the shifter 1.5 * 2^52 (the sum is a double with unit
+//             spacing whose low 32 bits are k), so neither a rounding nor a
+//             conversion instruction is needed; that requires |x| <
+//             EXP_ARG_MAX, which the kernels' cutoff tests guarantee (see
+//             exp_cutoff2).
 //   FP32      v_rsq_f32 and v_exp_f32 (2^x) hardware transcendentals.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -39,6 +44,7 @@ struct ExpCoeffs {
     double r2_floor;              // 1e-200, added to r^2 by the FP64 list kernels
     double r2_floor_mixed;        // 1e-20, the same for the mixed form (FP32 1/r^3 stays finite)
     double l2eN, ln2N;            // table-driven reduction (N/ln2, ln2/N)
+    double shifter;               // 1.5 * 2^52 (exp_neg64t's rounding; kernels pin it in a VGPR)
     double t3[1];                 // 1/3! (1/2! is written inline)
 };
 
@@ -50,8 +56,28 @@ inline ExpCoeffs exp_coeffs()
     e.r2_floor_mixed = 1e-20;
     e.l2eN = EXP_TN * 1.4426950408889634074;
     e.ln2N = 0.69314718055994530942 / EXP_TN;   // exact division by a power of two
+    e.shifter = 6755399441055744.0;
     e.t3[0] = 1.0 / 6.0;
     return e;
+}
+
+// Largest |x| passed to exp_neg64t.  e^x underflows to zero below x = -745,
+// so a pair whose exponent lies beyond -EXP_ARG_MAX contributes exactly zero
+// to force, energy (whose shift e^{-mu rc}/rc is then zero too), virial and
+// pressure tensor; the kernels therefore test pairs against a cutoff no larger
+// than EXP_ARG_MAX / rate and never evaluate such an exponent.  (At 1e6 the
+// shifter's k stays within 32 bits: |k| <= 1e6 * 1024 / ln2 < 2^31.)
+constexpr double EXP_ARG_MAX = 1.0e6;
+
+// Square of the cutoff the kernels test for an exponent rate * d (d = r, or
+// d = r^2 for the nucleon Gaussian): min(cut, EXP_ARG_MAX / rate), squared
+// when d = r.  Equivalent to cut for every pair that contributes.
+inline double exp_cutoff2(double cut2, double rate, bool of_r)
+{
+    if (!(rate > 0.0)) return cut2;
+    const double dmax = EXP_ARG_MAX / rate;
+    const double lim2 = of_r ? dmax * dmax : dmax;
+    return cut2 < lim2 ? cut2 : lim2;
 }
 
 // The 2^(j/N) table, j = 0 .. N-1, rounded to double from long double.
@@ -70,19 +96,30 @@ __device__ __forceinline__ double rsqrt64(double x, const ExpCoeffs &ec)
     return __builtin_fma(m, p, y);
 }
 
-// e^x for x <= 0 (x = -0 included; large negative x underflows to 0 through
-// ldexp, so no clamp is needed); tab = the 2^(j/N) table (LDS).
+// e^x for -EXP_ARG_MAX < x <= 0 (x = -0 included; below -745 the result
+// underflows to 0 through ldexp); tab = the 2^(j/N) table (LDS).
 __device__ __forceinline__ double exp_neg64t(double x, const ExpCoeffs &ec,
                                              const double *__restrict__ tab)
 {
-    const double k = __builtin_rint(x * ec.l2eN);
+    // t = x N/ln2 + 1.5 * 2^52 rounded once: k = t - 1.5 * 2^52 is the integer
+    // nearest x N/ln2 (exact subtraction), and t's low 32 bits hold k in two's
+    // complement
+    const double t = __builtin_fma(x, ec.l2eN, ec.shifter);
+    const double k = t - ec.shifter;
     const double r = __builtin_fma(-k, ec.ln2N, x);
     double p = __builtin_fma(r, ec.t3[0], 0.5);   // 1/2 as an inline constant
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
-    const int ki = (int)k;   // ki = N (ki >> TBITS) + (ki & (N-1)), also for ki < 0
+    // ki = N (ki >> TBITS) + (ki & (N-1)), also for ki < 0
+    const int ki = (int)(unsigned)__builtin_bit_cast(unsigned long long, t);
     return __builtin_amdgcn_ldexp(tab[ki & (EXP_TN - 1)] * p, ki >> EXP_TBITS);
 }
+
+// Keeps a loop-invariant double in a VGPR pair for the rest of the kernel (an
+// empty asm with a VGPR operand; no instruction).  A VALU instruction reads at
+// most one SGPR operand, so without it the pair loop copies the shifter from
+// its kernel-argument SGPRs into a VGPR for every exp_neg64t call.
+__device__ __forceinline__ void pin_vgpr(double &v) { asm volatile("" : "+v"(v)); }
 
 // The workgroup's LDS copy of the table (every thread of the block calls it,
 // before any of them may return).
