@@ -9,6 +9,8 @@
   pair within the cutoff and, as in src/int_ion_mix.f:140-145, no energy
   shift).
 * Boxes with three different edges.
+* A cutoff of 1e8 fm, past the kernels' 1e6/mu limit, and screening so strong
+  that mu r exceeds that limit inside the cutoff (exact zeros).
 * The 'B-field' sim type: accelerations exactly zero (src/int_control.f:71-72).
 
 Every case runs the cluster lists, the 64-slot tile lists and the culled
@@ -53,6 +55,9 @@ CASES = {
     "rc_beyond_half_diagonal": (500, (200.0, 200.0, 200.0), 0.9 * 200.0),
     "three_edges": (900, (180.0, 260.0, 130.0), 60.0),
     "three_edges_rc_beyond_half": (600, (150.0, 240.0, 200.0), 90.0),
+    # beyond 1e6 / mu the kernels' cutoff test stops at 1e6 / mu (the FP64
+    # exp's range): here that is ~1.5e7 fm, still past every pair of the box
+    "rc_huge": (500, (200.0, 200.0, 200.0), 1.0e8),
 }
 
 
@@ -90,6 +95,20 @@ def test_edge_case_mixed_precision(case, path):
     print(f"{case} {path} mixed: max rel {rel:.2e} cond {cond:.2e}")
     assert cond <= 1e-6 and rel <= 1e-4
     assert m_g["ev"] == pytest.approx(m_o["ev"], rel=1e-6)
+
+
+@pytest.mark.parametrize("path", ["clist", "tilelist", "allpairs"])
+def test_exponent_beyond_range_contributes_zero(path):
+    """Screening so strong that mu r reaches 2e6 inside the cutoff: e^{-mu r}
+    is exactly 0 for every pair (so is the energy shift), and the kernels,
+    which test pairs against min(rc, 1e6/mu), must return exact zeros too."""
+    p, x, Z, A = mixture(400, (200.0, 200.0, 200.0), 150.0, seed=5, kappa=2.0e5)
+    assert p.xmuc * p.rccut > 1.5e6
+    a_o, m_o = oracle.accel(p, x, Z, A, True, ref_f1=0)
+    a_g, m_g = gpu_accel(p, x, Z, A, True, path)
+    assert np.array_equal(a_o, np.zeros_like(a_o)) and np.array_equal(a_g, a_o)
+    assert m_o["ev"] == 0.0 and m_g["ev"] == 0.0
+    assert m_g["px"] == pytest.approx(p.rho * p.kT, rel=1e-15)
 
 
 def test_bfield_sim_type_has_no_pair_force():
