@@ -52,7 +52,7 @@ FM_C_SECONDS = 3.33564e-24
 # from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scale factors from a plain
 # 256 MiB copy); counters cannot be read inside a timed run, so the figure is
 # the profiled one, with its commands, in the file named here.
-TRAFFIC_BYTES_PER_LAUNCH = 1.101e8
+TRAFFIC_BYTES_PER_LAUNCH = 9.835e7
 TRAFFIC_SOURCE = "profiles/hbm_traffic_k_pairs_clist_ion_mixture_27648.md"
 
 

@@ -48,8 +48,20 @@ int fail(int code, const char *fmt, ...)
 
 }  // namespace
 
-constexpr int LIST_ITEMS_FP64 = 48;
-constexpr int LIST_ITEMS_MIXED = 32;
+// Work items per cluster set of the cluster-list force kernels (-D overrides
+// them for tuning builds).  Measured on MI355X at N = 27,648, FP64 step time
+// (median of 6): 24 items 0.419 ms, 32 0.385, 40 0.406, 48 0.414, 64 0.449.
+// At 5 resident waves per SIMD (the kernel's 87 VGPRs) 32 items per set give
+// 13,824 waves, 2.7 per wave slot, where 48 left a nearly empty fifth round;
+// fewer items also mean fewer partial sums for the finalize to read.
+#ifndef YK_LIST_ITEMS_FP64
+#define YK_LIST_ITEMS_FP64 32
+#endif
+#ifndef YK_LIST_ITEMS_MIXED
+#define YK_LIST_ITEMS_MIXED 32
+#endif
+constexpr int LIST_ITEMS_FP64 = YK_LIST_ITEMS_FP64;
+constexpr int LIST_ITEMS_MIXED = YK_LIST_ITEMS_MIXED;
 
 struct yk_system {
     yk_params p{};
@@ -835,8 +847,7 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
     YK_HIPB(hipMalloc(&s->orig2, sizeof(int) * nn));
     YK_HIPB(hipMalloc(&s->box, sizeof(TileBox) * (size_t)s->ntiles));
     s->use_list = !(params->flags & YK_FLAG_ALLPAIRS);
-    // items per target tile (cluster set): the mixed form's shorter pair loop
-    // gains from longer items (measured on MI355X at N = 27,648)
+    // items per target tile (cluster set)
     s->list_M = (params->precision == YK_MIXED) ? LIST_ITEMS_MIXED : LIST_ITEMS_FP64;
     s->skin = 0.05 * list_cutoff(*params);
     if (!(s->skin < 0.05 * params->xl[0])) s->skin = 0.0;   // huge cutoff: every pair listed
