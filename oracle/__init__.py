@@ -65,6 +65,7 @@ def load():
         lib.yko_strain.argtypes = [P, C.c_int, _D, _D, _D, C.c_double]
         lib.yko_accel_truth.argtypes = [P, C.c_int, _D, _D, _D, _D]
         lib.yko_accel_omp.argtypes = [P, C.c_int, _D, _D, _D, _D, C.c_int]
+        lib.yko_pair_abs_sum.argtypes = [P, C.c_int, _D, _D, _D, _D, C.c_int]
         _lib = lib
     return _lib
 
@@ -240,13 +241,27 @@ def truth(p, x, Z):
 
 
 def accel_omp(p, x, Z, A, nthreads=0):
-    """OpenMP restatement of the force-only branch (CPU timing); returns (a, threads)."""
+    """OpenMP restatement of the ions' force-only branch with the reference's
+    last-particle behaviour (F1), for CPU timing; returns (a, threads)."""
+    if p.sim_type not in ("pure-ion", "ion-mixture"):
+        raise ValueError(f"accel_omp covers the ion kernels only, not {p.sim_type!r}")
     n = x.shape[0]
     a = np.zeros((n, 3))
     ps = params_struct(p, 0)
     used = load().yko_accel_omp(C.byref(ps), n, _ptr(_arr(x, (n, 3))), _ptr(_arr(Z, (n,))),
                                 _ptr(_arr(A, (n,))), _ptr(a), int(nthreads))
     return a, used
+
+
+def pair_abs_sum(p, x, Z, nthreads=0):
+    """Conditioning of each particle's force (test metric): (sum_j |F_ij|,
+    sqrt(sum_j |F_ij|^2)) in MeV/fm, over the pairs inside the cutoffs."""
+    n = x.shape[0]
+    sa, rms = np.zeros(n), np.zeros(n)
+    ps = params_struct(p, 0)
+    load().yko_pair_abs_sum(C.byref(ps), n, _ptr(_arr(x, (n, 3))), _ptr(_arr(Z, (n,))),
+                            _ptr(sa), _ptr(rms), int(nthreads))
+    return sa, rms
 
 
 # ---------------------------------------------------------------------------

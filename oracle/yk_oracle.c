@@ -708,3 +708,60 @@ int yko_accel_omp(const yko_params *p, int n, const double *x, const double *zii
     free(fj);
     return used;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Conditioning of each particle's force: abs_sum[i] = sum_j |F_ij| and      */
+/* rms[i] = sqrt(sum_j |F_ij|^2) over the pairs inside the cutoffs (physical */
+/* pair-force magnitudes in MeV/fm, the nucleon's nuclear and Coulomb terms  */
+/* counted separately).  A force computed with per-pair relative error e has */
+/* an error of at most e * abs_sum[i], and about e * rms[i] when the pair    */
+/* errors are independent.  Full N x N loop over i in parallel.             */
+int yko_pair_abs_sum(const yko_params *p, int n, const double *x, const double *zii,
+                     double *abs_sum, double *rms, int nthreads)
+{
+    const double *xl = p->xl;
+    double halfl[3];
+    const double mu = p->xmuc, rc2 = p->rccut * p->rccut, rn2 = p->rncut * p->rncut;
+    int used = 1;
+    for (int k = 0; k < 3; ++k) halfl[k] = 0.5 * xl[k];
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+    {
+#ifdef _OPENMP
+#pragma omp single
+        used = omp_get_num_threads();
+#endif
+#pragma omp for schedule(dynamic, 64)
+        for (int i = 0; i < n; ++i) {
+            double sa = 0.0, s2 = 0.0;
+            for (int j = 0; j < n; ++j) {
+                double r2 = 0.0, f1 = 0.0, f2 = 0.0;
+                if (j == i) continue;
+                for (int k = 0; k < 3; ++k) {
+                    const double dk = min_image(x[3 * i + k] - x[3 * j + k], halfl[k], xl[k]);
+                    r2 += dk * dk;
+                }
+                const double r = sqrt(r2);
+                if (p->kind == YKO_NN) {
+                    if (p->nuclear_on && r2 <= rn2) {
+                        const double g = exp(-r2 / (2.0 * p->xpacket));
+                        const double bc = (zii[i] == zii[j]) ? p->bb + p->cc : p->bb - p->cc;
+                        f1 = fabs((2.0 * p->aa * g + bc) * g / p->xpacket) * r;
+                    }
+                    if (p->coulomb_on && r2 <= rc2 && zii[i] == 1.0 && zii[j] == 1.0)
+                        f2 = p->vc * (1.0 + mu * r) * exp(-mu * r) / r2;
+                } else if (r2 <= rc2) {
+                    const double zz = (p->kind == YKO_PURE) ? p->zi * p->zi : zii[i] * zii[j];
+                    f1 = fabs(p->frp * p->vc * zz) * (1.0 + mu * r) * exp(-mu * r) / r2;
+                }
+                sa += f1 + f2;
+                s2 += f1 * f1 + f2 * f2;
+            }
+            abs_sum[i] = sa;
+            rms[i] = sqrt(s2);
+        }
+    }
+    return used;
+}

@@ -100,6 +100,12 @@ int yko_accel_truth(const yko_params *p, int n, const double *x,
 int yko_accel_omp(const yko_params *p, int n, const double *x, const double *zii,
                   const double *aii, double *a, int nthreads);
 
+/* Conditioning of each particle's force: abs_sum[i] = sum_j |F_ij|, rms[i] =
+ * sqrt(sum_j |F_ij|^2) (MeV/fm; test metrics, not a reference routine).
+ * Returns the thread count used. */
+int yko_pair_abs_sum(const yko_params *p, int n, const double *x, const double *zii,
+                     double *abs_sum, double *rms, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -162,3 +162,26 @@ def test_staggered_admission_with_resort_is_bitwise(mode):
         alone = e.get(rid)
     for key in ("x", "v", "a"):
         assert np.array_equal(mixed_in[key], alone[key]), key
+
+
+@pytest.mark.parametrize("shape", [
+    {"list_items": 8, "chunk_tiles": 8, "waves_per_block": 1},
+    {"list_items": 2, "chunk_tiles": 8, "waves_per_block": 4},
+    {"list_items": 4, "chunk_tiles": 8, "waves_per_block": 2},
+    {"list_items": 0, "chunk_tiles": 2, "waves_per_block": 4},
+    {"list_items": 0, "chunk_tiles": 16, "waves_per_block": 4},
+    {"list_items": 0, "chunk_tiles": 32, "waves_per_block": 2},
+    {"list_items": 0, "chunk_tiles": 64, "waves_per_block": 4},
+])
+def test_refused_tuning_shapes(shape):
+    """Shapes only an earlier tuning sweep ran (one of whose ensembles ended in
+    a GPU memory fault of undetermined cause) are refused with
+    YK_ERR_UNSUPPORTED; the table in force is left unchanged."""
+    with Ensemble(capacity=1024, max_replicas=1, tuning=MODES["tiles"]) as e:
+        with pytest.raises(YkError) as err:
+            e.set_tuning([dict(shape, n_upper=4096)])
+        assert err.value.code == 6
+        w = W.replica(81, 500, True, 175.0, 1.0, "fp64")
+        rid = e.add_workload(w, steps=2)
+        e.run(2)
+        assert e.status(rid)[0] == 0

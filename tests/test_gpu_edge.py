@@ -22,7 +22,7 @@ import numpy as np
 import pytest
 
 import oracle
-from test_gpu_parity import check_meas, force_errors, gpu_accel
+from _forces import check_meas, force_errors, gpu_accel
 from yukawamd_amd.params import SimParams
 from yukawamd_amd.system import YukawaSystem
 

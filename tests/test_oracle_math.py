@@ -327,3 +327,46 @@ def test_strain_wraps_only_beyond_the_new_box():
     np.testing.assert_array_equal(x1[moved, 0], w.x[moved, 0] - box[0])
     np.testing.assert_array_equal(x1[~moved], w.x[~moved])
     np.testing.assert_array_equal(x1[:, 1:], w.x[:, 1:])
+
+
+@pytest.mark.parametrize("make", [lambda: W.ion_mixture(n=900, seed=131),
+                                  lambda: W.pure_ion(n=700, seed=132)])
+@pytest.mark.parametrize("threads", [1, 3])
+def test_openmp_restatement_matches_the_force_branch(make, threads):
+    """yko_accel_omp (the CPU-baseline timing restatement: the force-only
+    branch with the reference's j > i loop split over threads) gives the
+    force-only branch's accelerations, with the reference's last-particle
+    behaviour (F1), to rounding: the per-thread reductions only reorder sums."""
+    w = make()
+    a_ref, _ = oracle.accel(w.params, w.x, w.Z, w.A, False, ref_f1=1)
+    a_omp, used = oracle.accel_omp(w.params, w.x, w.Z, w.A, threads)
+    assert used == threads
+    assert np.abs(a_omp - a_ref).max() <= 1e-13 * np.abs(a_ref).max()
+
+
+def test_openmp_restatement_is_ions_only():
+    w = W.nucleon(n=200, seed=133)
+    with pytest.raises(ValueError):
+        oracle.accel_omp(w.params, w.x, w.Z, w.A, 1)
+
+
+@pytest.mark.parametrize("make", [lambda: W.ion_mixture(n=600, seed=134),
+                                  lambda: W.pure_ion(n=500, seed=135),
+                                  lambda: W.nucleon(n=900, seed=136)])
+def test_pair_abs_sum_conditioning(make):
+    """oracle.pair_abs_sum (the parity tests' conditioning metric) against a
+    numpy restatement, and the bounds it must satisfy: |F_i| <= sum_j |F_ij|
+    (triangle inequality, F_i from the long-double truth) and
+    sqrt(sum_j |F_ij|^2) <= sum_j |F_ij|."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parent))
+    from _forces import abs_pair_force_sum_np
+    w = make()
+    sa, rms = oracle.pair_abs_sum(w.params, w.x, w.Z, 2)
+    ref = abs_pair_force_sum_np(w.params, w.x, w.Z)
+    np.testing.assert_allclose(sa, ref, rtol=1e-12, atol=0)
+    f, _, _, _ = oracle.truth(w.params, w.x, w.Z)
+    F = np.linalg.norm(f, axis=1)
+    assert np.all(F <= sa * (1 + 1e-12))
+    assert np.all(rms <= sa * (1 + 1e-12)) and np.all(rms > 0)

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <new>
 #include <string>
@@ -178,9 +179,8 @@ double evcut_of(const yk_params &p)
     return (p.sim_type == YK_NUCLEON) ? p.vc * std::exp(-(p.xmuc * p.rccut)) / p.rccut : e;
 }
 
-ForceConsts force_consts(const yk_system *s)
+ForceConsts force_consts_of(const yk_params &p, int n, int ntiles, const double *etab_g)
 {
-    const yk_params &p = s->p;
     ForceConsts c{};
     for (int k = 0; k < 3; ++k) {
         c.L[k] = p.xl[k];
@@ -210,12 +210,12 @@ ForceConsts force_consts(const yk_system *s)
     c.bpcf = (float)c.bpc;
     c.bmcf = (float)c.bmc;
     c.vcf = (float)p.vc;
-    c.n = s->n;
-    c.ntiles = s->ntiles;
+    c.n = n;
+    c.ntiles = ntiles;
     c.coul_on = p.coulomb_on ? 1 : 0;
     c.nuc_on = p.nuclear_on ? 1 : 0;
     c.ec = exp_coeffs();
-    c.etab_g = s->etab_g;
+    c.etab_g = etab_g;
     double lmax = std::max(p.xl[0], std::max(p.xl[1], p.xl[2]));
     c.margin = 1e-12 * lmax;
     for (int k = 0; k < 3; ++k) c.halfL_m[k] = 0.5 * p.xl[k] - 4.0 * c.margin;
@@ -228,6 +228,11 @@ ForceConsts force_consts(const yk_system *s)
     // tiles farther apart than the cutoff (plus rounding slack) are skipped
     c.cull2 = (cut >= 1e100) ? 1e300 : cut * cut * (1.0 + 1e-12) + c.margin;
     return c;
+}
+
+ForceConsts force_consts(const yk_system *s)
+{
+    return force_consts_of(s->p, s->n, s->ntiles, s->etab_g);
 }
 
 double list_cutoff(const yk_params &p)
@@ -290,9 +295,8 @@ int invalidate_list(yk_system *s)
     return YK_OK;
 }
 
-StepConsts step_consts(const yk_system *s, double dt)
+StepConsts step_consts_of(const yk_params &p, int n, double dt)
 {
-    const yk_params &p = s->p;
     StepConsts c{};
     const bool kdk = (p.flags & YK_FLAG_KDK) != 0;
     c.dt = dt;
@@ -309,27 +313,30 @@ StepConsts step_consts(const yk_system *s, double dt)
     case YK_ION_MIXTURE: c.cnum = p.frp * p.vc; break;              // int_ion_mix.f:275
     default: c.cnum = 1.0; break;                                    // int_nn.f:320
     }
-    c.n = s->n;
+    c.n = n;
     c.ref_f1 = (p.flags & YK_FLAG_REF_F1) ? 1 : 0;
     c.kind = p.sim_type;
     return c;
 }
 
-MeasConsts meas_consts(const yk_system *s, int mode)
+StepConsts step_consts(const yk_system *s, double dt) { return step_consts_of(s->p, s->n, dt); }
+
+MeasConsts meas_consts_of(const yk_params &p, int n, int mode)
 {
-    const yk_params &p = s->p;
-    StepConsts sc = step_consts(s, 0.0);
+    StepConsts sc = step_consts_of(p, n, 0.0);
     MeasConsts m{};
     m.cnum = sc.cnum;
     m.rho = p.rho;
     m.kT = p.kT;
     m.xmass = p.xmass;
     m.vol = p.xl[0] * p.xl[1] * p.xl[2];
-    m.three_n = 3.0f * (float)s->n;
-    m.n_f = (float)s->n;
+    m.three_n = 3.0f * (float)n;
+    m.n_f = (float)n;
     m.mode = mode;
     return m;
 }
+
+MeasConsts meas_consts(const yk_system *s, int mode) { return meas_consts_of(s->p, s->n, mode); }
 
 void choose_decomposition(yk_system *s)
 {
