@@ -241,6 +241,28 @@ int yk_run_md(yk_system *sys, const yk_loop *loop, yk_meas *records, int32_t *re
 int yk_get_timing(yk_system *sys, yk_timing *out);
 int yk_reset_timing(yk_system *sys);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: accel(do_measurements) (src/int_control.f:52-81) of n
+ * particles whose arrays already live in device memory, without a handle and
+ * without allocating.  x (3,n), zii (n), aii (n) in the reference's layout
+ * (zii/aii may be NULL for YK_PURE_ION, which uses params->zi/ai); the
+ * accelerations go to a (3,n) and, when meas is not NULL, the measurement
+ * (zeros when do_measurements == 0) to the yk_meas meas points at; all five
+ * pointers are device memory, each checked to lie in a device allocation of
+ * its size (YK_ERR_ARG otherwise).  Forces come from the tiled all-pairs
+ * kernel (the YK_FLAG_ALLPAIRS path; FP64 or mixed by params->precision);
+ * results equal yk_accel's to summation-order rounding.  The kernels are
+ * enqueued on `stream` (a hipStream_t; NULL: the null stream) and the call
+ * returns without synchronising.  Work memory is one static device array per
+ * GPU (about 24 MiB, held until the process ends) shared by every call on
+ * that GPU: calls that overlap on one GPU race.  n > YK_NOALLOC_MAX_N:
+ * YK_ERR_ARG. */
+#define YK_NOALLOC_MAX_N 16384
+int yk_accel_noalloc(const yk_params *params, int32_t n, const double *x, const double *zii,
+                     const double *aii, double *a, int32_t do_measurements, yk_meas *meas,
+                     void *stream);
+
 /* Batched ensembles and continuous batching ------------------------------
  *
  * No counterpart in the reference, which runs one system per process.  A

@@ -89,6 +89,8 @@ SIGNATURES = {
                             C.c_int32, C.POINTER(C.c_int32), D]),
     "yk_get_timing": (C.c_int, [P, C.POINTER(YkTiming)]),
     "yk_reset_timing": (C.c_int, [P]),
+    "yk_accel_noalloc": (C.c_int, [C.POINTER(YkParams), C.c_int32, P, P, P, P, C.c_int32, P,
+                                   P]),
     # batched ensembles / continuous batching
     "yk_batch_last_error": (C.c_char_p, []),
     "yk_batch_create": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(P)]),

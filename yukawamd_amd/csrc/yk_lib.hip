@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -787,6 +788,170 @@ void release(yk_system *s)
 }  // namespace
 
 // ===========================================================================
+namespace {
+
+// ---------------------------------------------------------------------------
+// yk_accel_noalloc: accel for device-resident arrays with no allocation.  Its
+// work memory is one file-scope device array per GPU (`scratch` below), laid
+// out for at most NA_MAX_N particles and NA_MAX_CHUNKS source chunks of the
+// tiled all-pairs kernel.  Host code reaches it only through the address
+// hipGetSymbolAddress returns on each device (na_scratch); every call on a
+// GPU uses the same memory, so calls that overlap on one GPU race.
+constexpr int NA_MAX_N = YK_NOALLOC_MAX_N;
+constexpr int NA_MAX_CHUNKS = 16;
+
+struct NaLayout {   // byte offsets inside scratch, 256-aligned
+    size_t pos, den, orig, box, part, blockpart, acc, etab, swept, total;
+};
+
+constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
+
+constexpr NaLayout na_layout()
+{
+    constexpr size_t n = NA_MAX_N;
+    NaLayout l{};
+    size_t o = 0;
+    l.pos = o;       o += na_align(sizeof(double4) * n);
+    l.den = o;       o += na_align(sizeof(double) * n);
+    l.orig = o;      o += na_align(sizeof(int) * n);
+    l.box = o;       o += na_align(sizeof(TileBox) * ((n + TILE - 1) / TILE));
+    l.part = o;      o += na_align(sizeof(double) * 11 * n * NA_MAX_CHUNKS);
+    l.blockpart = o; o += na_align(sizeof(double) * NQ * ((n + BLOCK - 1) / BLOCK));
+    l.acc = o;       o += na_align(sizeof(double) * 3 * n);
+    l.etab = o;      o += na_align(sizeof(double) * EXP_TN);
+    l.swept = o;     o += na_align(sizeof(unsigned long long));
+    l.total = o;
+    return l;
+}
+constexpr NaLayout NA = na_layout();
+
+static_assert(sizeof(yk_meas) == sizeof(MeasOut) && offsetof(yk_meas, pp) == offsetof(MeasOut, pp),
+              "k_meas_final writes a yk_meas");
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA.total];
+
+namespace {
+
+std::mutex na_mutex;
+void *na_ptr[256];   // scratch's device address, per device ordinal
+
+// The address of this GPU's scratch (looked up once per device, with the
+// device current).
+int na_scratch(unsigned char **out)
+{
+    int dev = 0;
+    YK_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 256) return fail(YK_ERR_UNSUPPORTED, "device ordinal %d", dev);
+    std::lock_guard<std::mutex> lock(na_mutex);
+    if (!na_ptr[dev]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(YK_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                        hipGetErrorString(e));
+        na_ptr[dev] = ptr;
+    }
+    *out = static_cast<unsigned char *>(na_ptr[dev]);
+    return YK_OK;
+}
+
+// ptr must lie in one device allocation holding at least `bytes` from ptr.
+int na_check_range(const void *ptr, size_t bytes, const char *name)
+{
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    hipError_t e = hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)ptr);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();   // the lookup's error is not the caller's
+        return fail(YK_ERR_ARG, "%s is not device memory", name);
+    }
+    const char *b = static_cast<const char *>(base), *p = static_cast<const char *>(ptr);
+    if (p < b || (size_t)(p - b) + bytes > size)
+        return fail(YK_ERR_ARG, "%s: its device allocation holds fewer than %zu bytes from it",
+                    name, bytes);
+    return YK_OK;
+}
+
+// Caller's (3,n) positions and per-particle Z, A -> pos {x,y,z,Z}, the
+// acceleration denominator (as yk_create forms it) and the identity slot map.
+__global__ __launch_bounds__(BLOCK) void k_na_stage(const double *__restrict__ x,
+                                                   const double *__restrict__ zii,
+                                                   const double *__restrict__ aii, int n,
+                                                   int kind, double zi, double ai, double xmass,
+                                                   double4 *__restrict__ pos,
+                                                   double *__restrict__ den, int *__restrict__ orig)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double z = zii ? zii[i] : zi;
+    pos[i] = make_double4(x[3 * (size_t)i], x[3 * (size_t)i + 1], x[3 * (size_t)i + 2], z);
+    den[i] = (kind == YK_PURE_ION) ? ai * xmass
+                                   : (kind == YK_ION_MIXTURE ? aii[i] * xmass : xmass);
+    orig[i] = i;
+}
+
+// The 2^(j/N) table of exp_neg64t, from the device exp2.
+__global__ void k_na_exp_table(double *__restrict__ t)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < EXP_TN) t[j] = exp2((double)j / EXP_TN);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_na_fill(double *__restrict__ p, size_t count)
+{
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < count) p[i] = 0.0;
+}
+
+// Component-major accelerations -> the caller's (3,n) array.
+__global__ __launch_bounds__(BLOCK) void k_na_out(const double *__restrict__ acc, int n,
+                                                 double *__restrict__ a)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const size_t nn = (size_t)n;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) a[3 * (size_t)i + k] = acc[k * nn + i];
+}
+
+__global__ void k_na_zero_meas(yk_meas *m)
+{
+    if (threadIdx.x == 0) *m = yk_meas{};
+}
+
+template <int KIND, int PREC>
+void na_launch_pairs(hipStream_t st, const double4 *pos, const TileBox *box, const ForceConsts &c,
+                     int nblocks, int nchunks, int spc, double *part, unsigned long long *swept,
+                     bool meas)
+{
+    const dim3 grid(nblocks, nchunks);
+    if (meas)
+        hipLaunchKernelGGL((k_pairs<KIND, PREC, true>), grid, dim3(BLOCK), 0, st, pos, box, c,
+                           spc, part, swept);
+    else
+        hipLaunchKernelGGL((k_pairs<KIND, PREC, false>), grid, dim3(BLOCK), 0, st, pos, box, c,
+                           spc, part, swept);
+}
+
+template <int KIND>
+void na_launch_finalize(hipStream_t st, int nblocks, const double *part, int nchunks,
+                        const double4 *pos, const double *den, const int *orig, double *acc,
+                        const StepConsts &sc, double *blockpart, bool meas)
+{
+    if (meas)
+        hipLaunchKernelGGL((k_finalize<KIND, true, 0, 0>), dim3(nblocks), dim3(BLOCK), 0, st,
+                           part, nchunks, pos, den, (const double *)nullptr, orig, acc,
+                           (double *)nullptr, (const double *)nullptr, sc, blockpart);
+    else
+        hipLaunchKernelGGL((k_finalize<KIND, false, 0, 0>), dim3(nblocks), dim3(BLOCK), 0, st,
+                           part, nchunks, pos, den, (const double *)nullptr, orig, acc,
+                           (double *)nullptr, (const double *)nullptr, sc, blockpart);
+}
+
+}  // namespace
+
 extern "C" {
 
 int32_t yk_abi_version(void) { return YK_ABI_VERSION; }
@@ -1239,6 +1404,98 @@ int yk_reset_timing(yk_system *s)
     if (rc) return rc;
     if ((rc = read_swept(s, &s->swept_base))) return rc;
     s->timing = yk_timing{};
+    return YK_OK;
+}
+
+int yk_accel_noalloc(const yk_params *params, int32_t n, const double *x, const double *zii,
+                     const double *aii, double *a, int32_t do_measurements, yk_meas *meas,
+                     void *stream)
+{
+    int rc = check_params(params);
+    if (rc) return rc;
+    if (n < 1 || n > NA_MAX_N)
+        return fail(YK_ERR_ARG, "n=%d is outside [1, %d], the no-allocation limit", n, NA_MAX_N);
+    if (!x || !a) return fail(YK_ERR_ARG, "x and a are required");
+    const int kind = params->sim_type;
+    if (kind != YK_PURE_ION && (!zii || !aii))
+        return fail(YK_ERR_ARG, "zii and aii are required for sim_type %d", kind);
+    const size_t nn = (size_t)n;
+    if ((rc = na_check_range(x, sizeof(double) * 3 * nn, "x")) ||
+        (rc = na_check_range(a, sizeof(double) * 3 * nn, "a")) ||
+        (zii && (rc = na_check_range(zii, sizeof(double) * nn, "zii"))) ||
+        (aii && (rc = na_check_range(aii, sizeof(double) * nn, "aii"))) ||
+        (meas && (rc = na_check_range(meas, sizeof(yk_meas), "meas"))))
+        return rc;
+    unsigned char *base = nullptr;
+    if ((rc = na_scratch(&base))) return rc;
+    double4 *pos = reinterpret_cast<double4 *>(base + NA.pos);
+    double *den = reinterpret_cast<double *>(base + NA.den);
+    int *orig = reinterpret_cast<int *>(base + NA.orig);
+    TileBox *box = reinterpret_cast<TileBox *>(base + NA.box);
+    double *part = reinterpret_cast<double *>(base + NA.part);
+    double *blockpart = reinterpret_cast<double *>(base + NA.blockpart);
+    double *acc = reinterpret_cast<double *>(base + NA.acc);
+    double *etab = reinterpret_cast<double *>(base + NA.etab);
+    auto *swept = reinterpret_cast<unsigned long long *>(base + NA.swept);
+    // the tiled all-pairs decomposition of choose_decomposition, at most
+    // NA_MAX_CHUNKS source chunks
+    const int nblocks = (n + BLOCK - 1) / BLOCK, ntiles = (n + TILE - 1) / TILE;
+    const int nstages = (n + STAGE - 1) / STAGE;
+    int want = (2048 + nblocks - 1) / nblocks;
+    want = std::max(1, std::min(want, std::min(nstages, NA_MAX_CHUNKS)));
+    const int spc = (nstages + want - 1) / want;
+    const int nchunks = (nstages + spc - 1) / spc;
+    const bool m = do_measurements != 0;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_na_stage, dim3(nblocks), dim3(BLOCK), 0, st, x, zii, aii, n, kind,
+                       params->zi, params->ai, params->xmass, pos, den, orig);
+    hipLaunchKernelGGL(k_tile_boxes, dim3(nblocks), dim3(BLOCK), 0, st, pos, n, ntiles, box);
+    const bool mixed = params->precision == YK_MIXED;
+    if (n < 2 || kind == YK_BFIELD_TEST) {   // no pairs: zero partial sums
+        const size_t cnt = (size_t)nchunks * 11 * nn;
+        hipLaunchKernelGGL(k_na_fill, dim3((unsigned)((cnt + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
+                           st, part, cnt);
+    } else {
+        if (!mixed)
+            hipLaunchKernelGGL(k_na_exp_table, dim3(EXP_TN / BLOCK), dim3(BLOCK), 0, st, etab);
+        const ForceConsts c = force_consts_of(*params, n, ntiles, etab);
+        switch (kind) {
+        case YK_PURE_ION:
+            mixed ? na_launch_pairs<PURE, 1>(st, pos, box, c, nblocks, nchunks, spc, part, swept, m)
+                  : na_launch_pairs<PURE, 0>(st, pos, box, c, nblocks, nchunks, spc, part, swept, m);
+            break;
+        case YK_ION_MIXTURE:
+            mixed ? na_launch_pairs<MIX, 1>(st, pos, box, c, nblocks, nchunks, spc, part, swept, m)
+                  : na_launch_pairs<MIX, 0>(st, pos, box, c, nblocks, nchunks, spc, part, swept, m);
+            break;
+        default:
+            mixed ? na_launch_pairs<NN, 1>(st, pos, box, c, nblocks, nchunks, spc, part, swept, m)
+                  : na_launch_pairs<NN, 0>(st, pos, box, c, nblocks, nchunks, spc, part, swept, m);
+            break;
+        }
+    }
+    const StepConsts sc = step_consts_of(*params, n, 0.0);
+    switch (kind) {
+    case YK_PURE_ION:
+        na_launch_finalize<PURE>(st, nblocks, part, nchunks, pos, den, orig, acc, sc, blockpart, m);
+        break;
+    case YK_ION_MIXTURE:
+        na_launch_finalize<MIX>(st, nblocks, part, nchunks, pos, den, orig, acc, sc, blockpart, m);
+        break;
+    default:
+        na_launch_finalize<NN>(st, nblocks, part, nchunks, pos, den, orig, acc, sc, blockpart, m);
+        break;
+    }
+    if (meas) {
+        if (m)
+            hipLaunchKernelGGL(k_meas_final, dim3(1), dim3(BLOCK), 0, st, blockpart, nblocks,
+                               meas_consts_of(*params, n, MEAS_ACCEL),
+                               reinterpret_cast<MeasOut *>(meas));
+        else
+            hipLaunchKernelGGL(k_na_zero_meas, dim3(1), dim3(64), 0, st, meas);
+    }
+    hipLaunchKernelGGL(k_na_out, dim3(nblocks), dim3(BLOCK), 0, st, acc, n, a);
+    YK_HIP(hipGetLastError());
     return YK_OK;
 }
 

@@ -78,7 +78,7 @@ module yk_bind
   public :: yk_abi_version, yk_create, yk_destroy, yk_set_params, yk_put_state, &
             yk_put_accel, yk_get_state, yk_accel, yk_newton, yk_run, yk_ttot, &
             yk_tnorm, yk_cancel_vcom, yk_strain, yk_run_md, &
-            yk_get_timing, yk_reset_timing, yk_error_message
+            yk_get_timing, yk_reset_timing, yk_accel_noalloc, yk_error_message
   public :: yk_batch_create, yk_batch_destroy, yk_batch_footprint, yk_batch_add, &
             yk_batch_remove, yk_batch_set_tuning, yk_batch_set_resort, yk_batch_run, &
             yk_batch_status, yk_batch_get, yk_batch_last_run, yk_batch_error_message
@@ -197,6 +197,19 @@ module yk_bind
     integer(c_int) function yk_reset_timing(sys) bind(C, name='yk_reset_timing')
       import :: c_int, c_ptr
       type(c_ptr), value :: sys
+    end function
+
+    ! accel on arrays already in device memory, with no allocation (no
+    ! reference counterpart; for callers that cannot allocate): x, zii, aii, a
+    ! and meas are device addresses (zii/aii may be c_null_ptr for
+    ! 'pure-ion'), stream a hipStream_t or c_null_ptr; returns before the
+    ! device work is done.
+    integer(c_int) function yk_accel_noalloc(params, n, x, zii, aii, a, do_measurements, &
+                                             meas, stream) bind(C, name='yk_accel_noalloc')
+      import :: c_int, c_int32_t, c_ptr, yk_params_t
+      type(yk_params_t), intent(in) :: params
+      integer(c_int32_t), value :: n, do_measurements
+      type(c_ptr), value :: x, zii, aii, a, meas, stream
     end function
 
     type(c_ptr) function yk_batch_last_error_c() bind(C, name='yk_batch_last_error')
