@@ -85,15 +85,17 @@ def test_edge_case_against_oracle(case, path):
 @pytest.mark.parametrize("case", sorted(c for c in CASES if c != "n1"))
 @pytest.mark.parametrize("path", ["clist", "tilelist", "allpairs"])
 def test_edge_case_mixed_precision(case, path):
-    """The mixed form on the same cases, with the mixed-precision bars
-    (condition-aware force error <= 1e-6, per-particle <= 1e-4, ev <= 1e-6)."""
+    """The mixed form on the same cases, with the mixed-precision bars:
+    condition-aware force error <= 1e-6, ev <= 1e-6, and per particle <= 1e-6
+    on the cluster lists (FP64 near field) and <= 1e-4 on the tile lists and
+    the all-pairs kernel (every pair in FP32)."""
     n, xl, rc = CASES[case]
     p, x, Z, A = mixture(n, xl, rc, seed=900 + len(case) * 7 + n)
     a_o, m_o = oracle.accel(p, x, Z, A, True, ref_f1=0)
     a_g, m_g = gpu_accel(p.replace(precision="mixed"), x, Z, A, True, path)
     rel, cond = force_errors(p, a_g, a_o, x, Z, A)
     print(f"{case} {path} mixed: max rel {rel:.2e} cond {cond:.2e}")
-    assert cond <= 1e-6 and rel <= 1e-4
+    assert cond <= 1e-6 and rel <= (1e-6 if path == "clist" else 1e-4)
     assert m_g["ev"] == pytest.approx(m_o["ev"], rel=1e-6)
 
 

@@ -87,7 +87,10 @@ def test_mixed_precision_error(make):
     rel, cond = force_errors(p, a_g, a_o, w.x, w.Z, w.A)
     print(f"{w.name} mixed: max rel {rel:.2e} cond {cond:.2e} "
           f"ev rel {abs(m_g['ev'] / m_o['ev'] - 1):.2e}")
-    assert cond <= 1e-6 and rel <= 1e-4
+    if p.sim_type == "nucleon":     # no near field: every pair in FP32
+        assert cond <= 1e-6 and rel <= 1e-4
+    else:                           # FP64 near field (cluster lists)
+        assert cond <= 1e-7 and rel <= 1e-6
     assert m_g["ev"] == pytest.approx(m_o["ev"], rel=1e-6)
 
 

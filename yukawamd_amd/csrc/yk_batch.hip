@@ -86,6 +86,7 @@ struct yk_batch {
     RepState *state = nullptr;
     MeasOut *records = nullptr, *init_records = nullptr;
     BatchItem *items = nullptr;
+    int *qorder = nullptr;    // queue position -> item (list shares 0 first)
     int2 *tile_items = nullptr;
     double *part = nullptr;
     size_t part_items_cap = 0;
@@ -254,7 +255,12 @@ RepConsts rep_consts(const HostReplica &h)
         const double rlf = rl * (1.0 + 5e-6) + 1e-6 * lm;   // as the single-system bound
         c.rl2f = (float)(rlf * rlf);
         c.skinf = (float)c.skin;
+        const double rn = near_radius(1, p.precision == YK_MIXED, p.rho, c.skin);
+        c.rn2f = rn < 0.0 ? -1.0f : (float)(rn * rn);
     }
+    c.rc2f_lo = (float)(c.rc2 * (1.0 - 4e-6));
+    c.rc2f_hi = (float)(c.rc2 * (1.0 + 4e-6));
+    c.r2_floor_f = 1e-20f;
     return c;
 }
 
@@ -393,22 +399,38 @@ int rebuild(yk_batch *b)
     if ((size_t)b->nitems > b->part_items_cap) {
         if (b->part) YB_HIP(hipFree(b->part));
         if (b->items) YB_HIP(hipFree(b->items));
+        if (b->qorder) YB_HIP(hipFree(b->qorder));
         b->part = nullptr;
         b->items = nullptr;
+        b->qorder = nullptr;
         size_t want = std::max<size_t>((size_t)b->nitems + b->nitems / 4, 1024);
         YB_HIP(hipMalloc(&b->part, sizeof(double) * 11 * TILE * want));
         YB_HIP(hipMalloc(&b->items, sizeof(BatchItem) * want));
+        YB_HIP(hipMalloc(&b->qorder, sizeof(int) * want));
         b->part_items_cap = want;
     }
+    // queue order: a list item's share m leads with the list's back entries
+    // (FP64 in the mixed form), so shares go out by m -- every list's share 0
+    // first, tile items with them -- and the longest items start first;
+    // otherwise the build order (largest replica first)
+    std::vector<int> qorder(items.size());
+    std::iota(qorder.begin(), qorder.end(), 0);
+    std::stable_sort(qorder.begin(), qorder.end(), [&](int a, int c) {
+        const int ma = items[a].s1 < 0 ? items[a].s0 : 0, mc = items[c].s1 < 0 ? items[c].s0 : 0;
+        return ma < mc;
+    });
     YB_HIP(hipMemcpyAsync(b->reps, rc.data(), sizeof(RepConsts) * b->max_rep,
                           hipMemcpyHostToDevice, b->stream));
     YB_HIP(hipMemcpyAsync(b->tile_rep, tile_rep.data(), sizeof(int) * ntiles_cap,
                           hipMemcpyHostToDevice, b->stream));
     YB_HIP(hipMemcpyAsync(b->tile_items, tile_items.data(), sizeof(int2) * ntiles_cap,
                           hipMemcpyHostToDevice, b->stream));
-    if (b->nitems)
+    if (b->nitems) {
         YB_HIP(hipMemcpyAsync(b->items, items.data(), sizeof(BatchItem) * b->nitems,
                               hipMemcpyHostToDevice, b->stream));
+        YB_HIP(hipMemcpyAsync(b->qorder, qorder.data(), sizeof(int) * b->nitems,
+                              hipMemcpyHostToDevice, b->stream));
+    }
     hipLaunchKernelGGL(k_fill_rep, dim3((b->cap + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, b->stream,
                        b->rep, b->tile_rep, b->reps, b->cap);
     YB_HIP(hipGetLastError());
@@ -440,11 +462,11 @@ int enqueue_step(yk_batch *b)
         hipLaunchKernelGGL(k_batch_clist_build, dim3(b->n_list_tiles, NSEG), dim3(BLOCK), 0,
                            b->stream, b->pos, b->box, b->list_tiles, b->tile_rep, b->reps,
                            b->rep_maxdisp, b->clist, b->ccounts, b->xref);
-    const size_t lds = sizeof(double4) * TILE * b->waves;
+    const size_t lds = (sizeof(double4) + sizeof(float4)) * TILE * b->waves;
     hipLaunchKernelGGL((k_batch_pairs<MEAS>), dim3(persistent_blocks(b)), dim3(TILE * b->waves),
-                       lds, b->stream, b->pos, b->box, b->reps, b->state, b->items, b->nitems,
-                       b->queue, ec, b->etab_g, MODE == 0 ? 1 : 0, b->clist, b->ccounts,
-                       b->part);
+                       lds, b->stream, b->pos, b->box, b->reps, b->state, b->items, b->qorder,
+                       b->nitems, b->queue, ec, b->etab_g, MODE == 0 ? 1 : 0, b->clist,
+                       b->ccounts, b->part);
     hipLaunchKernelGGL((k_batch_finalize<MEAS, MODE>), slots, dim3(BLOCK), 0, b->stream, b->part,
                        b->tile_items, b->pos, b->den, b->aii, b->orig, b->rep, b->reps,
                        b->state, b->span, b->cap, b->acc, b->vel, b->vold, b->blockpart);
@@ -739,7 +761,7 @@ void yk_batch_destroy(yk_batch *b)
                     (void *)b->tile_rep, (void *)b->box, (void *)b->tile_items,
                     (void *)b->blockpart, (void *)b->reps, (void *)b->state,
                     (void *)b->records, (void *)b->init_records, (void *)b->queue,
-                    (void *)b->part, (void *)b->items, (void *)b->keys, (void *)b->keys2,
+                    (void *)b->part, (void *)b->items, (void *)b->qorder, (void *)b->keys, (void *)b->keys2,
                     (void *)b->vals, (void *)b->vals2, (void *)b->due, b->sort_tmp,
                     (void *)b->pos2, (void *)b->vel2, (void *)b->vold2, (void *)b->acc2,
                     (void *)b->aii2, (void *)b->den2, (void *)b->orig2, (void *)b->xref,

@@ -37,6 +37,8 @@ struct RepConsts {
     int list_items, segcap;
     double rl2, skin;
     float Lf[3], invLf[3], halfLf_m[3], rl2f, skinf;
+    float rn2f;                    // near field of the mixed form (ListConsts::rn2f)
+    float rc2f_lo, rc2f_hi, r2_floor_f;   // as ForceConsts
 };
 
 struct RepState {            // device-updated per replica
@@ -146,7 +148,7 @@ __device__ __forceinline__ void batch_item(const RepConsts &rc, const ExpCoeffs 
 }
 
 // One list item: the four 16-slot clusters of target tile it.tile against
-// share m of M of each cluster's list (the scheme of k_pairs_clist, with the
+// share m of M of each cluster's list (clist_share, as k_pairs_clist, with the
 // replica's constants and its region of the list arena).
 template <int KIND, int PREC, bool MEAS>
 __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCoeffs &ec,
@@ -155,7 +157,8 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
                                                 const unsigned *__restrict__ clist,
                                                 const int2 *__restrict__ ccounts,
                                                 const BatchItem &it, double4 *__restrict__ sw,
-                                                int lane, double *__restrict__ out)
+                                                float4 *__restrict__ sf, int lane,
+                                                double *__restrict__ out)
 {
     ForceConsts c{};
     c.etab = etab;
@@ -171,105 +174,43 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
     c.muf = rc.muf;
     c.mul2ef = rc.mul2ef;
     c.mul2e_lo = rc.mul2e_lo;
+    c.rc2f_lo = rc.rc2f_lo;
+    c.rc2f_hi = rc.rc2f_hi;
+    c.r2_floor_f = rc.r2_floor_f;
     c.ec = ec;
-    if constexpr (PREC == 0) pin_vgpr(c.ec.shifter);
+    pin_vgpr(c.ec.shifter);   // both forms have an FP64 pair path
     const int M = -it.s1, m = it.s0;
-    const int g = lane / CL, gl = lane % CL;
+    const int g = lane / CL;
     const int t = it.tile;
     const int i = t * TILE + lane;
     const bool ivalid = i - rc.off < rc.n;
     const double nanv = __builtin_nan("");
     const double4 pi = ivalid ? pos[i] : make_double4(nanv, nanv, nanv, 0.0);
+    double cc[3] = {0.0, 0.0, 0.0};
+    if constexpr (PREC == 1) group_centre(pi, ivalid, cc);
     const int clg = t * CPW + g;                    // global cluster (ccounts index)
     const int cll = (t - rc.tile0) * CPW + g;       // cluster within the replica
     const bool cvalid = cll * CL < rc.n;            // clusters past n have no list
-    // segment boundaries and fast counts as scalars (NSEG = 4), so nothing
-    // is indexed dynamically
-    static_assert(NSEG == 4, "segment selection below is written for four segments");
-    int2 c0 = make_int2(0, 0), c1 = c0, c2 = c0, c3 = c0;
-    if (cvalid) {
-        const int2 *cp = ccounts + (size_t)clg * NSEG;
-        c0 = cp[0]; c1 = cp[1]; c2 = cp[2]; c3 = cp[3];
+    int2 sc[NSEG];
+    int total = 0;
+#pragma unroll
+    for (int sg = 0; sg < NSEG; ++sg) {
+        sc[sg] = cvalid ? ccounts[(size_t)clg * NSEG + sg] : make_int2(0, 0);
+        total += sc[sg].x + sc[sg].y;
     }
-    const int b1 = c0.x + c0.y, b2 = b1 + c1.x + c1.y, b3 = b2 + c2.x + c2.y;
-    const int total = b3 + c3.x + c3.y;
     const int per = (total + M - 1) / M;
     const int lo = min(m * per, total), hi = min(lo + per, total);
-    const int share = hi - lo;
-    int iters = share;
+    int iters = hi - lo;
 #pragma unroll
     for (int off = CL; off < TILE; off <<= 1) {
         const int o = __shfl_xor(iters, off, 64);
         iters = o > iters ? o : iters;
     }
     iters = __builtin_amdgcn_readfirstlane(iters);
-    const int segcap = rc.segcap;
-    const unsigned *mine = clist + rc.lists_off + (size_t)cll * NSEG * segcap;
-    const SegList sl{mine, segcap, hi, b1, b2, b3, c0.x, c1.x, c2.x, c3.x};
+    const unsigned *mine = clist + rc.lists_off + (size_t)cll * NSEG * rc.segcap;
+    const SegList sl = seg_list(mine, rc.segcap, sc, hi);
     Acc<MEAS> acc;
-    bool g_cur, g_nxt;
-    unsigned e_cur = sl.entry(lo + gl, g_cur);
-    double4 pj_cur = pos[rc.off + (e_cur & SLOT_MASK)];
-    unsigned e_nxt = sl.entry(lo + CL + gl, g_nxt);
-    for (int b0 = 0; b0 < iters; b0 += CL) {
-        const int v = lo + b0 + gl;
-        bool genl = false;
-        __builtin_amdgcn_wave_barrier();   // the previous batch's reads are done
-        if (v < hi) {
-            double4 pj = pj_cur;
-            if (!g_cur) {   // apply the entry's periodic image
-                pj.x += c.L[0] * (double)((int)((e_cur >> 26) & 3u) - 1);
-                pj.y += c.L[1] * (double)((int)((e_cur >> 28) & 3u) - 1);
-                pj.z += c.L[2] * (double)((int)((e_cur >> 30) & 3u) - 1);
-            } else {
-                genl = true;
-            }
-            pj.w = stage_w<KIND, PREC>(pj.w, c.muf);
-            sw[lane] = pj;
-        } else {
-            const double q = __builtin_nan("");   // past the share: rejected by every pair test
-            sw[lane] = make_double4(q, q, q, 0.0);
-        }
-        const bool gen = __ballot(genl) != 0ull;
-        __builtin_amdgcn_wave_barrier();
-        e_cur = e_nxt;
-        g_cur = g_nxt;
-        pj_cur = pos[rc.off + (e_cur & SLOT_MASK)];
-        e_nxt = sl.entry(lo + b0 + 2 * CL + gl, g_nxt);
-        const int kb = min(CL, iters - b0);
-        const double4 *gs = sw + g * CL;
-        if (!gen) {
-            double4 pn = gs[0];
-            if (kb == CL) {   // a full batch: unrolled, immediate LDS offsets
-#pragma unroll
-                for (int k = 0; k < CL; ++k) {
-                    const double4 pj = pn;
-                    if (k + 1 < CL) pn = gs[k + 1];
-                    interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w, pi.x - pj.x, pi.y - pj.y,
-                                                     pi.z - pj.z, false, acc);
-                }
-            } else {
-#pragma unroll 4
-                for (int k = 0; k < kb; ++k) {
-                    const double4 pj = pn;
-                    pn = gs[min(k + 1, CL - 1)];
-                    interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w, pi.x - pj.x, pi.y - pj.y,
-                                                     pi.z - pj.z, false, acc);
-                }
-            }
-        } else {
-#pragma unroll 2
-            for (int k = 0; k < kb; ++k) {
-                const double4 pj = gs[k];
-                interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w,
-                                                 min_image(pi.x - pj.x, c.L[0], c.invL[0]),
-                                                 min_image(pi.y - pj.y, c.L[1], c.invL[1]),
-                                                 min_image(pi.z - pj.z, c.L[2], c.invL[2]),
-                                                 false, acc);
-            }
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
+    clist_share<KIND, PREC, MEAS>(c, pi, cc, sl, lo, iters, pos + rc.off, sw, sf, lane, acc);
     double *o = out + lane;
     o[0] = acc.fx; o[TILE] = acc.fy; o[2 * TILE] = acc.fz;
     if constexpr (MEAS) {
@@ -287,16 +228,20 @@ __global__ __launch_bounds__(512) void k_batch_pairs(const double4 *__restrict__
                                                      const TileBox *__restrict__ box,
                               const RepConsts *__restrict__ reps,
                               const RepState *__restrict__ state,
-                              const BatchItem *__restrict__ items, int nitems,
+                              const BatchItem *__restrict__ items,
+                              const int *__restrict__ qorder, int nitems,
                               unsigned *__restrict__ queue, ExpCoeffs ec,
                               const double *__restrict__ etab_g, int accel_pass,
                               const unsigned *__restrict__ clist,
                               const int2 *__restrict__ ccounts, double *__restrict__ part)
 {
+    // dynamic LDS: per wave 64 double4 sources, then per wave 64 float4 (the
+    // mixed list items' FP32 offsets)
     extern __shared__ double4 sj_dyn[];
     __shared__ double etab_s[EXP_TN];   // FP64 replicas' exp table, one per workgroup
     const int lane = threadIdx.x & (TILE - 1);
     double4 *sw = sj_dyn + (threadIdx.x / TILE) * TILE;
+    float4 *sf = reinterpret_cast<float4 *>(sj_dyn + blockDim.x) + (threadIdx.x / TILE) * TILE;
     load_exp_table(etab_s, etab_g);
     const double *etab = etab_s;
     constexpr int NC = MEAS ? 11 : 3;
@@ -305,8 +250,9 @@ __global__ __launch_bounds__(512) void k_batch_pairs(const double4 *__restrict__
         if (lane == 0) k = atomicAdd(queue, 1u);
         k = (unsigned)__builtin_amdgcn_readfirstlane((int)__shfl(k, 0, 64));
         if (k >= (unsigned)nitems) break;   // every wave reaches this exit
-        const BatchItem it = items[k];
-        double *out = part + (size_t)k * NC * TILE;
+        const int q = qorder[k];             // a permutation of [0, nitems)
+        const BatchItem it = items[q];
+        double *out = part + (size_t)q * NC * TILE;
         const RepConsts &rc = reps[it.rep];
         if (accel_pass ? state[it.rep].fresh == 0 : state[it.rep].steps_left <= 0) {
             // replica not part of this pass: leave zero partials
@@ -320,14 +266,14 @@ __global__ __launch_bounds__(512) void k_batch_pairs(const double4 *__restrict__
         if (it.s1 < 0) {
             if (kind == PURE) {
                 if (prec == 0)
-                    batch_list_item<PURE, 0, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, lane, out);
+                    batch_list_item<PURE, 0, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, sf, lane, out);
                 else
-                    batch_list_item<PURE, 1, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, lane, out);
+                    batch_list_item<PURE, 1, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, sf, lane, out);
             } else {
                 if (prec == 0)
-                    batch_list_item<MIX, 0, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, lane, out);
+                    batch_list_item<MIX, 0, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, sf, lane, out);
                 else
-                    batch_list_item<MIX, 1, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, lane, out);
+                    batch_list_item<MIX, 1, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, sf, lane, out);
             }
             continue;
         }
@@ -475,7 +421,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_clist_build(
     __syncthreads();
     int nf, ng;
     clist_scan<W>(pos + rc.off, rc.n, ctiles, ncand, cv, live, tc, rc.Lf, rc.invLf,
-                  rc.rl2f, stage, clist,
+                  rc.rl2f, rc.rn2f, stage, clist,
                   rc.lists_off + ((size_t)(cvalid ? cll : 0) * NSEG + seg) * rc.segcap, rc.segcap,
                   lane, wv, nf, ng);
     if (cvalid && lane == 0) ccounts[(t * CPW + wv) * NSEG + seg] = make_int2(nf, ng);

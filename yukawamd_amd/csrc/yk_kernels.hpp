@@ -56,6 +56,11 @@ struct ForceConsts {
     // of mul2ef does not bias every pair at distance r by the same relative
     // amount (mu r x 6e-8 at most); one rounding of the exponent remains
     float mul2e_lo;
+    // the mixed ion form's far pairs (FP32 throughout): cutoff^2 widened and
+    // narrowed by 4e-6 relative (an FP32 r^2 within that band of rc^2 takes
+    // the FP64 test, so a pair is in or out of the cutoff exactly as in FP64),
+    // and the r^2 floor that keeps a self pair's 1/r^3 finite
+    float rc2f_lo, rc2f_hi, r2_floor_f;
     int n, ntiles, coul_on, nuc_on;
     ExpCoeffs ec;
     const double *etab_g; // the 2^(j/N) table of exp_neg64t (device memory)
@@ -388,9 +393,31 @@ struct ListConsts {
     // double test would: rl2f = (rl (1 + 5e-6) + 1e-6 max L)^2 (the float
     // offsets are within a few ulp of L); halfLf_m = halfL_m - 1e-5 L
     float Lf[3], invLf[3], halfLf_m[3], rl2f, skinf;
+    // cluster lists of the mixed ion form: sources whose distance to the
+    // cluster's AABB is at most sqrt(rn2f) (the near field) are stored with
+    // the general entries, which the mixed pair kernels evaluate in FP64;
+    // rn2f < 0: no near field
+    float rn2f;
 };
 
 constexpr unsigned SLOT_MASK = (1u << 26) - 1u;
+
+// Near field of the mixed ion form's cluster lists: sources within
+// NEAR_WS Wigner-Seitz radii a = (3 / (4 pi rho))^(1/3) of a cluster's AABB,
+// plus the skin (so the pairs within NEAR_WS a stay near until the next
+// build), are evaluated in FP64.  Per-pair FP32 errors are about 1e-7 of a
+// pair force; the nearest pairs carry the largest forces, and a particle whose
+// net force is a small fraction of them would otherwise see its relative
+// error grow past 1e-6.  Returns the radius, or -1 (no near field: FP64
+// systems, nucleons, or no density given).
+constexpr double NEAR_WS = 4.0;
+
+inline double near_radius(int sim_type_is_ion, int mixed, double rho, double skin)
+{
+    if (!sim_type_is_ion || !mixed || !(rho > 0.0)) return -1.0;
+    const double a = std::cbrt(3.0 / (4.0 * 3.14159265358979323846 * rho));
+    return NEAR_WS * a + skin;
+}
 
 __device__ __forceinline__ unsigned long long as_bits(double v)
 {
@@ -798,8 +825,8 @@ __device__ __forceinline__ ClusterView cluster_view(const double c[3], const dou
 // (FAR_AWAY); tier 0 callers test presence.
 template <int TIER>
 __device__ __forceinline__ void clist_test(const ClusterView &v, const float4 &s, const float Lf[3],
-                                           const float invLf[3], float rl2f, bool &cand,
-                                           bool &fast, unsigned long long &mcand,
+                                           const float invLf[3], float rl2f, float rn2f,
+                                           bool &cand, bool &fast, unsigned long long &mcand,
                                            unsigned long long &mfast)
 {
     const float sx[3] = {s.x, s.y, s.z};
@@ -830,6 +857,11 @@ __device__ __forceinline__ void clist_test(const ClusterView &v, const float4 &s
     }
     cand = d2 <= rl2f;
     mcand = __builtin_amdgcn_ballot_w64(cand);
+    // a near-field source is stored with the general entries (evaluated in
+    // FP64 by the mixed kernels)
+    const bool near = d2 <= rn2f;
+    fast = fast & !near;
+    mfast &= ~__builtin_amdgcn_ballot_w64(near);
 }
 
 // Lanes below this one whose bit is set in m.
@@ -849,8 +881,9 @@ template <int TIER, int W>
 __device__ __forceinline__ void clist_round(const ClusterView &v, const float4 (*st)[TILE],
                                             const int *base, int n, bool live, int lane,
                                             const float Lf[3], const float invLf[3], float rl2f,
-                                            unsigned *__restrict__ clist, size_t seg_off,
-                                            unsigned segcap, unsigned &nf, unsigned &ng)
+                                            float rn2f, unsigned *__restrict__ clist,
+                                            size_t seg_off, unsigned segcap, unsigned &nf,
+                                            unsigned &ng)
 {
 #pragma unroll
     for (int w = 0; w < W; ++w) {
@@ -858,7 +891,7 @@ __device__ __forceinline__ void clist_round(const ClusterView &v, const float4 (
         const float4 sv = st[w][lane];
         bool cand, fast;
         unsigned long long mcand, mfast;
-        clist_test<TIER>(v, sv, Lf, invLf, rl2f, cand, fast, mcand, mfast);
+        clist_test<TIER>(v, sv, Lf, invLf, rl2f, rn2f, cand, fast, mcand, mfast);
         if constexpr (TIER == 0) {
             const bool present = j < n;
             cand = cand & live & present;
@@ -883,7 +916,7 @@ template <int W>
 __device__ __forceinline__ void clist_scan(const double4 *__restrict__ pos, int n,
                                            const int *ctiles, int ncand, const ClusterView &cv,
                                            bool live, const double tc[3], const float Lf[3],
-                                           const float invLf[3], float rl2f,
+                                           const float invLf[3], float rl2f, float rn2f,
                                            float4 (*stage)[W][TILE], unsigned *__restrict__ clist,
                                            size_t seg_off, int segcap, int lane, int wv, int &nf,
                                            int &ng)
@@ -911,14 +944,14 @@ __device__ __forceinline__ void clist_scan(const double4 *__restrict__ pos, int 
 #pragma unroll
         for (int w = 0; w < W; ++w) base[w] = (r * W + w < ncand) ? ctiles[r * W + w] * TILE : n;
         if (cv.tier == 2)
-            clist_round<2, W>(cv, stage[r & 1], base, n, live, lane, Lf, invLf, rl2f, clist,
-                              seg_off, (unsigned)segcap, uf, ug);
+            clist_round<2, W>(cv, stage[r & 1], base, n, live, lane, Lf, invLf, rl2f, rn2f,
+                              clist, seg_off, (unsigned)segcap, uf, ug);
         else if (cv.tier == 1)
-            clist_round<1, W>(cv, stage[r & 1], base, n, live, lane, Lf, invLf, rl2f, clist,
-                              seg_off, (unsigned)segcap, uf, ug);
+            clist_round<1, W>(cv, stage[r & 1], base, n, live, lane, Lf, invLf, rl2f, rn2f,
+                              clist, seg_off, (unsigned)segcap, uf, ug);
         else
-            clist_round<0, W>(cv, stage[r & 1], base, n, live, lane, Lf, invLf, rl2f, clist,
-                              seg_off, (unsigned)segcap, uf, ug);
+            clist_round<0, W>(cv, stage[r & 1], base, n, live, lane, Lf, invLf, rl2f, rn2f,
+                              clist, seg_off, (unsigned)segcap, uf, ug);
     }
     nf = (int)uf;
     ng = (int)ug;
@@ -994,7 +1027,7 @@ static __global__ __launch_bounds__(BLOCK) void k_clist_build(
     }
     __syncthreads();
     int nf, ng;
-    clist_scan<W>(pos, lc.n, ctiles, ncand, cv, live, tc, lc.Lf, lc.invLf, lc.rl2f,
+    clist_scan<W>(pos, lc.n, ctiles, ncand, cv, live, tc, lc.Lf, lc.invLf, lc.rl2f, lc.rn2f,
                   stage, clist, ((size_t)(cvalid ? cl : 0) * NSEG + seg) * segcap, segcap, lane,
                   wv, nf, ng);
     if (cvalid && lane == 0) ccounts[cl * NSEG + seg] = make_int2(nf, ng);
@@ -1042,35 +1075,262 @@ static __global__ __launch_bounds__(CORDER_THREADS) void k_corder(
     for (int k = threadIdx.x; k < ncl; k += CORDER_THREADS) order[k] = (int)(key[k] & 0xfffu);
 }
 
-// A cluster's list as its NSEG segments (segment s begins at list index b_s,
-// b_0 = 0, and holds nf_s fast entries from the front of its region of segcap
-// entries, then general ones from the back): entry v and whether it is
-// general (0 past hi).
+// A cluster's list as its NSEG segments, read back entries first: each
+// segment's region holds its fast entries from the front and the others
+// (general ones and, in the mixed ion form, the near field) from the back; the
+// list is the back entries of segments 0..3, then their fast entries, so that
+// the entries the mixed kernels evaluate in FP64 come first in every list.
+// entry(v): entry v and whether it is a back entry (0 at or past hi).
 struct SegList {
     const unsigned *base;
-    int segcap, hi, b1, b2, b3, nf0, nf1, nf2, nf3;
-    __device__ __forceinline__ unsigned entry(int v, bool &general) const
+    int segcap, hi;
+    int nback;           // back entries of all segments
+    int g1, g2, g3;      // back entries of segments below 1, 2, 3
+    int f1, f2, f3;      // fast entries of segments below 1, 2, 3
+    __device__ __forceinline__ unsigned entry(int v, bool &back) const
     {
-        general = false;
+        back = false;
         if (v >= hi) return 0u;
+        back = v < nback;
         // arithmetic selection of the segment (selects of addresses would put
         // the fields on the stack)
-        const int s1 = v >= b1, s2 = v >= b2, s3 = v >= b3;
+        const int w = back ? v : v - nback;
+        const int c1 = back ? g1 : f1, c2 = back ? g2 : f2, c3 = back ? g3 : f3;
+        const int s1 = w >= c1, s2 = w >= c2, s3 = w >= c3;
         const int sg = s1 + s2 + s3;
-        const int start = s1 * (b1 + s2 * (b2 - b1 + s3 * (b3 - b2)));
-        const int nfs = nf0 + s1 * (nf1 - nf0 + s2 * (nf2 - nf1 + s3 * (nf3 - nf2)));
-        const int o = v - start;
+        const int start = s1 * (c1 + s2 * (c2 - c1 + s3 * (c3 - c2)));
+        const int o = w - start;
         const unsigned *r = base + (size_t)sg * segcap;
-        general = o >= nfs;
-        return general ? r[segcap - 1 - (o - nfs)] : r[o];
+        return back ? r[segcap - 1 - o] : r[o];
     }
 };
 
+// A cluster's segment counts (fast, back) as a SegList limited to [0, hi).
+__device__ __forceinline__ SegList seg_list(const unsigned *base, int segcap, const int2 (&sc)[NSEG],
+                                            int hi)
+{
+    static_assert(NSEG == 4, "SegList holds four segments");
+    SegList s;
+    s.base = base;
+    s.segcap = segcap;
+    s.hi = hi;
+    s.g1 = sc[0].y;
+    s.g2 = s.g1 + sc[1].y;
+    s.g3 = s.g2 + sc[2].y;
+    s.nback = s.g3 + sc[3].y;
+    s.f1 = sc[0].x;
+    s.f2 = s.f1 + sc[1].x;
+    s.f3 = s.f2 + sc[2].x;
+    return s;
+}
+
+// The far pairs of the mixed ion form, FP32 throughout: target and source as
+// FP32 offsets from the cluster's centre (each the rounding of an FP64
+// difference), displacement, r^2 (with a floor that keeps a self pair's 1/r^3
+// finite), cutoff test, 1/r, exp and the force factor; the force accumulates
+// in FP32 over one 16-source batch (the caller adds it to the FP64 sums), the
+// measured quantities in FP64.  s.w: the source's Z.  An FP32 r^2 within the
+// band [rc2f_lo, rc2f_hi] of the cutoff decides by the FP64 r^2 of the target
+// pi and the source's FP64 image sd.
+template <int KIND, bool MEAS>
+__device__ __forceinline__ void interact_far(const ForceConsts &c, float xi, float yi, float zi,
+                                             const float4 &s, const double4 &pi,
+                                             const double4 *__restrict__ sd, float &fx, float &fy,
+                                             float &fz, Acc<MEAS> &acc)
+{
+    const float dx = xi - s.x, dy = yi - s.y, dz = zi - s.z;
+    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, c.r2_floor_f)));
+    bool in = r2 <= c.rc2f_hi;
+    if (in && r2 > c.rc2f_lo) {   // rare: FP32 cannot tell
+        const double4 pj = *sd;
+        const double ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+        in = __builtin_fma(ex, ex, __builtin_fma(ey, ey, __builtin_fma(ez, ez, c.ec.r2_floor))) <=
+             c.rc2;
+    }
+    if (in) {
+        const float rinv = rsqrt32(r2);
+        const float r = r2 * rinv;
+        const float ex = exp2_32(__builtin_fmaf(-c.mul2ef, r, -c.mul2e_lo * r));
+        float q = __builtin_fmaf(c.muf, r, 1.0f) * ex;
+        if constexpr (KIND == MIX) q *= s.w;
+        const float fc = q * ((rinv * rinv) * rinv);
+        fx = __builtin_fmaf(fc, dx, fx);
+        fy = __builtin_fmaf(fc, dy, fy);
+        fz = __builtin_fmaf(fc, dz, fz);
+        if constexpr (MEAS) {
+            if (r2 > 10.0f * c.r2_floor_f) {   // not the self pair
+                double phi = (double)(ex * rinv) - c.evcut;
+                if constexpr (KIND == MIX) phi *= (double)s.w;
+                const double sd = (double)fc, ddx = dx, ddy = dy, ddz = dz;
+                const double sx = sd * ddx, sy = sd * ddy, sz = sd * ddz;
+                acc.e += phi;
+                acc.w = __builtin_fma(sd, (double)r2, acc.w);
+                acc.pxx = __builtin_fma(ddx, sx, acc.pxx);
+                acc.pxy = __builtin_fma(ddx, sy, acc.pxy);
+                acc.pxz = __builtin_fma(ddx, sz, acc.pxz);
+                acc.pyy = __builtin_fma(ddy, sy, acc.pyy);
+                acc.pyz = __builtin_fma(ddy, sz, acc.pyz);
+                acc.pzz = __builtin_fma(ddz, sz, acc.pzz);
+            }
+        }
+    }
+}
+
+// The pairs of a 16-lane group's target (pi, one per lane; NaN for none)
+// with its share [lo, hi) of its cluster's list (sl), the wave stepping
+// through its four groups' shares together for iters entries; sources are
+// slots of pos.  Each group stages 16 of its sources per batch in its part of
+// the wave's LDS slice sw (the next batch's entries and positions in flight);
+// lanes past their share stage NaN sources, which every pair test rejects.
+// A batch holding a back entry (general, or near field) takes the per-pair
+// minimum image -- in FP64 also in the mixed ion form (PREC 1) --; the other
+// batches of the mixed ion form run interact_far on FP32 offsets from the
+// group's cluster centre cc.  Nucleons keep the double staging in both
+// precisions.
+template <int KIND, int PREC, bool MEAS>
+__device__ __forceinline__ void clist_share(const ForceConsts &c, const double4 &pi,
+                                            const double cc[3], const SegList &sl, int lo,
+                                            int iters, const double4 *__restrict__ pos,
+                                            double4 *__restrict__ sw, float4 *__restrict__ sf,
+                                            int lane, Acc<MEAS> &acc)
+{
+    constexpr bool FAR32 = PREC == 1 && KIND != NN;
+    const int g = lane / CL, gl = lane % CL;
+    const int hi = sl.hi;
+    float xi_f = 0.f, yi_f = 0.f, zi_f = 0.f;
+    if constexpr (FAR32) {
+        xi_f = (float)(pi.x - cc[0]);
+        yi_f = (float)(pi.y - cc[1]);
+        zi_f = (float)(pi.z - cc[2]);
+    }
+    bool g_cur, g_nxt;
+    unsigned e_cur = sl.entry(lo + gl, g_cur);
+    double4 pj_cur = pos[e_cur & SLOT_MASK];
+    unsigned e_nxt = sl.entry(lo + CL + gl, g_nxt);
+    for (int b0 = 0; b0 < iters; b0 += CL) {
+        const int v = lo + b0 + gl;
+        const bool live = v < hi;
+        // a batch holding a back entry takes the per-pair minimum image for all
+        // of its pairs (a no-op for the already shifted ones)
+        const bool gen = __ballot(live && g_cur) != 0ull;
+        double4 pj = pj_cur;
+        if (live && !g_cur) {   // apply the entry's periodic image
+            pj.x += c.L[0] * (double)((int)((e_cur >> 26) & 3u) - 1);
+            pj.y += c.L[1] * (double)((int)((e_cur >> 28) & 3u) - 1);
+            pj.z += c.L[2] * (double)((int)((e_cur >> 30) & 3u) - 1);
+        }
+        const double q = __builtin_nan("");
+        if (FAR32 && !gen) {
+            const float qf = (float)q;
+            sf[lane] = live ? make_float4((float)(pj.x - cc[0]), (float)(pj.y - cc[1]),
+                                          (float)(pj.z - cc[2]), (float)pj.w)
+                            : make_float4(qf, qf, qf, 0.f);
+        }
+        // the FP64 images: general batches, and the far batches' cutoff band
+        pj.w = FAR32 ? pj.w : stage_w<KIND, PREC>(pj.w, c.muf);
+        sw[lane] = live ? pj : make_double4(q, q, q, 0.0);
+        __builtin_amdgcn_wave_barrier();
+        e_cur = e_nxt;
+        g_cur = g_nxt;
+        pj_cur = pos[e_cur & SLOT_MASK];
+        e_nxt = sl.entry(lo + b0 + 2 * CL + gl, g_nxt);
+        const int kb = min(CL, iters - b0);
+        // FP64 (the mixed ion form's too), minimum image per pair
+        auto general_batch = [&]() {
+            constexpr int GP = FAR32 ? 0 : PREC;
+            const double4 *gs = sw + g * CL;
+#pragma unroll 2
+            for (int k = 0; k < kb; ++k) {
+                const double4 pjk = gs[k];
+                interact<KIND, GP, MEAS, true>(c, pi.w, pjk.w,
+                                               min_image(pi.x - pjk.x, c.L[0], c.invL[0]),
+                                               min_image(pi.y - pjk.y, c.L[1], c.invL[1]),
+                                               min_image(pi.z - pjk.z, c.L[2], c.invL[2]),
+                                               false, acc);
+            }
+        };
+        if constexpr (FAR32) {
+            if (!gen) {
+                const float4 *gs = sf + g * CL;
+                const double4 *gd = sw + g * CL;
+                float fx = 0.f, fy = 0.f, fz = 0.f;
+                float4 pn = gs[0];
+                if (kb == CL) {
+#pragma unroll
+                    for (int k = 0; k < CL; ++k) {
+                        const float4 pf = pn;
+                        if (k + 1 < CL) pn = gs[k + 1];
+                        interact_far<KIND, MEAS>(c, xi_f, yi_f, zi_f, pf, pi, gd + k, fx, fy, fz,
+                                                 acc);
+                    }
+                } else {
+#pragma unroll 4
+                    for (int k = 0; k < kb; ++k) {
+                        const float4 pf = pn;
+                        pn = gs[min(k + 1, CL - 1)];
+                        interact_far<KIND, MEAS>(c, xi_f, yi_f, zi_f, pf, pi, gd + k, fx, fy, fz,
+                                                 acc);
+                    }
+                }
+                acc.fx += (double)fx;
+                acc.fy += (double)fy;
+                acc.fz += (double)fz;
+            } else {
+                general_batch();
+            }
+        } else {
+            if (!gen) {
+                const double4 *gs = sw + g * CL;
+                double4 pn = gs[0];
+                if (kb == CL) {
+                    // a full batch (all but a share's last): fully unrolled, so
+                    // the LDS reads take immediate offsets and a pair needs no
+                    // address arithmetic; the next source is read one pair ahead
+#pragma unroll
+                    for (int k = 0; k < CL; ++k) {
+                        const double4 pjk = pn;
+                        if (k + 1 < CL) pn = gs[k + 1];
+                        interact<KIND, PREC, MEAS, true>(c, pi.w, pjk.w, pi.x - pjk.x,
+                                                         pi.y - pjk.y, pi.z - pjk.z, false, acc);
+                    }
+                } else {
+#pragma unroll 4
+                    for (int k = 0; k < kb; ++k) {
+                        const double4 pjk = pn;
+                        pn = gs[min(k + 1, CL - 1)];
+                        interact<KIND, PREC, MEAS, true>(c, pi.w, pjk.w, pi.x - pjk.x,
+                                                         pi.y - pjk.y, pi.z - pjk.z, false, acc);
+                    }
+                }
+            } else {
+                general_batch();
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// The centre of the AABB of a 16-lane group's valid targets (every lane of
+// the group gets it; 0 for a group without targets).
+__device__ __forceinline__ void group_centre(const double4 &pi, bool valid, double cc[3])
+{
+    const double x[3] = {pi.x, pi.y, pi.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double lo = valid ? x[k] : __builtin_inf(), hi = valid ? x[k] : -__builtin_inf();
+#pragma unroll
+        for (int off = 1; off < CL; off <<= 1) {
+            const double l2 = __shfl_xor(lo, off, 64), h2 = __shfl_xor(hi, off, 64);
+            lo = l2 < lo ? l2 : lo;
+            hi = h2 > hi ? h2 : hi;
+        }
+        cc[k] = (lo <= hi) ? 0.5 * (lo + hi) : 0.0;
+    }
+}
+
 // Pair sums over the cluster lists.  A wave takes four clusters (order[4s ..
 // 4s+3]), 16 lanes each, one target per lane, and item m of M: an equal share
-// of every cluster's list.  Each 16-lane group stages 16 of its own sources at
-// a time (already shifted to their image) in its LDS slice; the wave then
-// steps through the four slices together, lanes of shorter shares idle.
+// of every cluster's list (clist_share).
 template <int KIND, int PREC, bool MEAS>
 __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict__ pos,
                                                        const unsigned *__restrict__ clist,
@@ -1083,8 +1343,10 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
                                                        unsigned long long *__restrict__ swept)
 {
     __shared__ double4 sj[BLOCK / TILE][TILE];
+    __shared__ float4 sfj[BLOCK / TILE][(PREC == 1 && KIND != NN) ? TILE : 1];
+    constexpr bool TAB = PREC == 0 || KIND != NN;   // an FP64 pair path with the exp table
     const int n = c.n;
-    if constexpr (PREC == 0) pin_vgpr(c.ec.shifter);
+    if constexpr (TAB) pin_vgpr(c.ec.shifter);
     // the build kernels (previous launches) have consumed the displacement flag
     if (blockIdx.x == 0 && threadIdx.x == 0 && lists_stale(maxdisp, skin)) {
         maxdisp[0] = 0ull;
@@ -1092,36 +1354,38 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
     }
     const int lane = threadIdx.x & (TILE - 1);
     const int wv = threadIdx.x / TILE;
-    if constexpr (PREC == 0) {
+    if constexpr (TAB) {
         __shared__ double etab_s[EXP_TN];
         load_exp_table(etab_s, c.etab_g);
         c.etab = etab_s;
     }
     const int g = lane / CL, gl = lane % CL;
+    // items in launch order share-index first (every set's share 0, then share
+    // 1, ...): the back entries -- FP64 in the mixed ion form -- lead every
+    // list, so the first shares are the longest, and they start first
     const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / TILE) + wv);
-    const int set = item / M, m = item % M;
     const int nsets = (ncl + CPW - 1) / CPW;
-    if (set >= nsets) return;   // whole wave; this kernel has no block barrier
+    const int m = item / nsets, set = item % nsets;
+    if (m >= M) return;   // whole wave; no block barrier follows
     const int ci = set * CPW + g;
     const int cl = (ci < ncl) ? order[ci] : -1;
     const int i = cl * CL + gl;
     const bool ivalid = cl >= 0 && i < n;
     const double nanv = __builtin_nan("");
     const double4 pi = ivalid ? pos[i] : make_double4(nanv, nanv, nanv, 0.0);
-    // this group's share of its cluster's list (NSEG segments in order)
+    double cc[3] = {0.0, 0.0, 0.0};
+    if constexpr (PREC == 1 && KIND != NN) group_centre(pi, ivalid, cc);
+    // this group's share of its cluster's list
     int2 sc[NSEG];
-    int sbeg[NSEG + 1];
-    sbeg[0] = 0;
+    int total = 0;
 #pragma unroll
     for (int sg = 0; sg < NSEG; ++sg) {
         sc[sg] = (cl >= 0) ? ccounts[cl * NSEG + sg] : make_int2(0, 0);
-        sbeg[sg + 1] = sbeg[sg] + sc[sg].x + sc[sg].y;
+        total += sc[sg].x + sc[sg].y;
     }
-    const int total = sbeg[NSEG];
     const int per = (total + M - 1) / M;
     const int lo = min(m * per, total), hi = min(lo + per, total);
-    const int share = hi - lo;
-    int iters = share;
+    int iters = hi - lo;
 #pragma unroll
     for (int off = CL; off < TILE; off <<= 1) {
         const int o = __shfl_xor(iters, off, 64);
@@ -1129,82 +1393,9 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
     }
     iters = __builtin_amdgcn_readfirstlane(iters);
     const unsigned *mine = clist + (size_t)(cl >= 0 ? cl : 0) * NSEG * segcap;
-    static_assert(NSEG == 4, "SegList holds four segments");
-    const SegList sl{mine, segcap, hi, sbeg[1], sbeg[2], sbeg[3], sc[0].x, sc[1].x, sc[2].x,
-                     sc[3].x};
-    auto entry_at = [&](int v, bool &general) -> unsigned { return sl.entry(v, general); };
+    const SegList sl = seg_list(mine, segcap, sc, hi);
     Acc<MEAS> acc;
-    double4 *sw = sj[wv];
-    // software pipeline over batches of 16 entries per group
-    bool g_cur, g_nxt;
-    unsigned e_cur = entry_at(lo + gl, g_cur);
-    double4 pj_cur = pos[e_cur & SLOT_MASK];
-    unsigned e_nxt = entry_at(lo + CL + gl, g_nxt);
-    for (int b0 = 0; b0 < iters; b0 += CL) {
-        const int v = lo + b0 + gl;
-        bool genl = false;
-        if (v < hi) {
-            double4 pj = pj_cur;
-            if (!g_cur) {   // apply the entry's periodic image
-                pj.x += c.L[0] * (double)((int)((e_cur >> 26) & 3u) - 1);
-                pj.y += c.L[1] * (double)((int)((e_cur >> 28) & 3u) - 1);
-                pj.z += c.L[2] * (double)((int)((e_cur >> 30) & 3u) - 1);
-            } else {
-                genl = true;
-            }
-            pj.w = stage_w<KIND, PREC>(pj.w, c.muf);
-            sw[lane] = pj;
-        } else {
-            // past this group's share: a NaN source, which every pair test
-            // rejects (NaN r^2 fails the cutoff), so no lane count is compared
-            const double q = __builtin_nan("");
-            sw[lane] = make_double4(q, q, q, 0.0);
-        }
-        // a batch holding a general entry takes the per-pair minimum image for
-        // all of its pairs (a no-op for the already shifted ones)
-        const bool gen = __ballot(genl) != 0ull;
-        __builtin_amdgcn_wave_barrier();
-        e_cur = e_nxt;
-        g_cur = g_nxt;
-        pj_cur = pos[e_cur & SLOT_MASK];
-        e_nxt = entry_at(lo + b0 + 2 * CL + gl, g_nxt);
-        const int kb = min(CL, iters - b0);
-        const double4 *gs = sw + g * CL;
-        if (!gen) {
-            double4 pn = gs[0];
-            if (kb == CL) {
-                // a full batch (all but a share's last): fully unrolled, so
-                // the LDS reads take immediate offsets and a pair needs no
-                // address arithmetic; the next source is read one pair ahead
-#pragma unroll
-                for (int k = 0; k < CL; ++k) {
-                    const double4 pj = pn;
-                    if (k + 1 < CL) pn = gs[k + 1];
-                    interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w, pi.x - pj.x, pi.y - pj.y,
-                                                     pi.z - pj.z, false, acc);
-                }
-            } else {
-#pragma unroll 4
-                for (int k = 0; k < kb; ++k) {
-                    const double4 pj = pn;
-                    pn = gs[min(k + 1, CL - 1)];
-                    interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w, pi.x - pj.x, pi.y - pj.y,
-                                                     pi.z - pj.z, false, acc);
-                }
-            }
-        } else {
-#pragma unroll 2
-            for (int k = 0; k < kb; ++k) {
-                const double4 pj = gs[k];
-                interact<KIND, PREC, MEAS, true>(c, pi.w, pj.w,
-                                                 min_image(pi.x - pj.x, c.L[0], c.invL[0]),
-                                                 min_image(pi.y - pj.y, c.L[1], c.invL[1]),
-                                                 min_image(pi.z - pj.z, c.L[2], c.invL[2]),
-                                                 false, acc);
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-    }
+    clist_share<KIND, PREC, MEAS>(c, pi, cc, sl, lo, iters, pos, sj[wv], sfj[wv], lane, acc);
     if (lane == 0 && iters > 0) atomicAdd(swept, (unsigned long long)iters * TILE);
     if (cl < 0) return;
     constexpr int NC = MEAS ? 11 : 3;

@@ -211,6 +211,9 @@ ForceConsts force_consts_of(const yk_params &p, int n, int ntiles, const double 
     c.bpcf = (float)c.bpc;
     c.bmcf = (float)c.bmc;
     c.vcf = (float)p.vc;
+    c.rc2f_lo = (float)(c.rc2 * (1.0 - 4e-6));
+    c.rc2f_hi = (float)(c.rc2 * (1.0 + 4e-6));
+    c.r2_floor_f = 1e-20f;
     c.n = n;
     c.ntiles = ntiles;
     c.coul_on = p.coulomb_on ? 1 : 0;
@@ -275,6 +278,9 @@ ListConsts list_consts(const yk_system *s)
         l.rl2f = (float)(rlf * rlf);
     }
     l.skinf = (float)s->skin;
+    const double rn = near_radius(p.sim_type == YK_PURE_ION || p.sim_type == YK_ION_MIXTURE,
+                                  p.precision == YK_MIXED, p.rho, s->skin);
+    l.rn2f = rn < 0.0 ? -1.0f : (float)(rn * rn);
     return l;
 }
 
