@@ -78,7 +78,12 @@ def test_fp64_against_oracle(make, path):
 
 @pytest.mark.parametrize("make", [lambda: W.ion_mixture(n=3000, seed=61),
                                   lambda: W.pure_ion(n=3456, seed=1000),
-                                  lambda: W.nucleon(n=6000, seed=62)])
+                                  lambda: W.nucleon(n=6000, seed=62),
+                                  # the ensemble's screening range: the FP64 near
+                                  # field's radius follows the force's decay
+                                  lambda: W.ion_mixture(n=4000, seed=63, kappa=0.5),
+                                  lambda: W.ion_mixture(n=4000, seed=64, kappa=2.0),
+                                  lambda: W.pure_ion(n=4000, seed=65, kappa=2.0)])
 def test_mixed_precision_error(make):
     w = make()
     p = w.params

@@ -255,7 +255,7 @@ RepConsts rep_consts(const HostReplica &h)
         const double rlf = rl * (1.0 + 5e-6) + 1e-6 * lm;   // as the single-system bound
         c.rl2f = (float)(rlf * rlf);
         c.skinf = (float)c.skin;
-        const double rn = near_radius(1, p.precision == YK_MIXED, p.rho, c.skin);
+        const double rn = near_radius(1, p.precision == YK_MIXED, p.rho, p.xmuc, p.rccut, c.skin);
         c.rn2f = rn < 0.0 ? -1.0f : (float)(rn * rn);
     }
     c.rc2f_lo = (float)(c.rc2 * (1.0 - 4e-6));

@@ -27,6 +27,8 @@
 // are deterministic.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <vector>
+
 #include "yk_math.hpp"
 #include "yk_sfc.hpp"
 
@@ -402,21 +404,45 @@ struct ListConsts {
 
 constexpr unsigned SLOT_MASK = (1u << 26) - 1u;
 
-// Near field of the mixed ion form's cluster lists: sources within
-// NEAR_WS Wigner-Seitz radii a = (3 / (4 pi rho))^(1/3) of a cluster's AABB,
-// plus the skin (so the pairs within NEAR_WS a stay near until the next
-// build), are evaluated in FP64.  Per-pair FP32 errors are about 1e-7 of a
-// pair force; the nearest pairs carry the largest forces, and a particle whose
-// net force is a small fraction of them would otherwise see its relative
-// error grow past 1e-6.  Returns the radius, or -1 (no near field: FP64
-// systems, nucleons, or no density given).
-constexpr double NEAR_WS = 4.0;
+// Near field of the mixed ion form's cluster lists: sources within the near
+// radius (plus the skin, so those pairs stay near until the next build) of a
+// cluster's AABB are evaluated in FP64.  Per-pair FP32 errors are about 1e-7
+// of a pair force and add up in a particle's force like a random walk, with
+// weight sum over shells of r^2 F(r)^2 (F = (1 + mu r) e^{-mu r} / r^2 in a
+// uniform fluid); the nearest pairs carry most of it, and a particle whose net
+// force is a small fraction of them would otherwise see its relative error
+// grow past 1e-6.  The near radius leaves a fraction NEAR_TAIL of that weight
+// (between 0.5 a and the cutoff, a = (3 / (4 pi rho))^(1/3)) to FP32:
+// 0.0019 is the fraction beyond 4 a at mu a = 0.7, where 4 a brings the
+// largest relative force error of 27,648 ions to about 3e-7; the radius is
+// then 5.1 a at mu a = 0.5, 3.1 a at 1 and 1.9 a at 2.  Returns -1 (no near
+// field) for FP64 systems, nucleons, or no density.
+constexpr double NEAR_TAIL = 0.0019;
 
-inline double near_radius(int sim_type_is_ion, int mixed, double rho, double skin)
+inline double near_radius(int sim_type_is_ion, int mixed, double rho, double mu, double rc,
+                          double skin)
 {
     if (!sim_type_is_ion || !mixed || !(rho > 0.0)) return -1.0;
     const double a = std::cbrt(3.0 / (4.0 * 3.14159265358979323846 * rho));
-    return NEAR_WS * a + skin;
+    const double k = mu * a, x0 = 0.5, xc = std::min(std::max(rc / a, x0 * 1.001), 1e9);
+    // the integral of x^2 F(x)^2 dx = x^3 F(x)^2 d(ln x) on a logarithmic grid
+    // over [x0, xc] (Simpson per interval), accumulated from the top
+    const int NI = 4096;
+    const double h = std::log(xc / x0) / NI;
+    auto w = [&](double t) {
+        const double x = x0 * std::exp(t);
+        const double f = (1.0 + k * x) * std::exp(-k * x) / (x * x);
+        return x * x * x * f * f;
+    };
+    std::vector<double> tail(NI + 1, 0.0);
+    for (int i = NI - 1; i >= 0; --i) {
+        const double ta = i * h, tb = ta + h;
+        tail[i] = tail[i + 1] + h / 6.0 * (w(ta) + 4.0 * w(0.5 * (ta + tb)) + w(tb));
+    }
+    const double want = NEAR_TAIL * tail[0];
+    int i = 0;
+    while (i < NI && tail[i] > want) ++i;
+    return x0 * std::exp(i * h) * a + skin;
 }
 
 __device__ __forceinline__ unsigned long long as_bits(double v)

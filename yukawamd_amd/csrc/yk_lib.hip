@@ -279,7 +279,7 @@ ListConsts list_consts(const yk_system *s)
     }
     l.skinf = (float)s->skin;
     const double rn = near_radius(p.sim_type == YK_PURE_ION || p.sim_type == YK_ION_MIXTURE,
-                                  p.precision == YK_MIXED, p.rho, s->skin);
+                                  p.precision == YK_MIXED, p.rho, p.xmuc, p.rccut, s->skin);
     l.rn2f = rn < 0.0 ? -1.0f : (float)(rn * rn);
     return l;
 }
