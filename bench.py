@@ -47,6 +47,12 @@ PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector
 PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector (packed)
 FLOPS_PER_PAIR_FP64 = 90     # directed pair, force-only step, special functions costed
 FLOPS_PER_PAIR_MEAS = 109    # measurement step
+FLOPS_PER_PAIR_PURE = 89     # pure ions (no Z_j factor)
+FLOPS_PER_PAIR_PURE_MEAS = 108
+# mixed far pair, FP32 (FMA = 2): displacement 3, r^2 6, cutoff 1, r 1, exponent
+# 3, (1 + mu r) e 3, Z_j 1, r^-3 2, force factor 1, accumulate 6; the hardware
+# rsq and exp2 counted as one each
+FLOPS_PER_PAIR_MIXED_FP32 = 29
 FM_C_SECONDS = 3.33564e-24
 # HBM bytes per launch of the FP64 force-only cluster-list kernel at N = 27,648,
 # from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scale factors from a plain
@@ -54,6 +60,16 @@ FM_C_SECONDS = 3.33564e-24
 # the profiled one, with its commands, in the file named here.
 TRAFFIC_BYTES_PER_LAUNCH = 9.835e7
 TRAFFIC_SOURCE = "profiles/hbm_traffic_k_pairs_clist_ion_mixture_27648.md"
+# VALU issue of the pair kernels at N = 27,648 from rocprofv3 counters:
+# SQ_INSTS_VALU x 4 cycles (a wave64 VALU instruction on a 16-lane SIMD, FP64 or
+# unpacked FP32) / (GRBM_GUI_ACTIVE x 1,024 SIMDs); read from the profile named
+# here, since counters cannot be collected inside a timed run
+VALU_ISSUE_FRAC_FP64 = None
+VALU_ISSUE_FRAC_MIXED = None
+COUNTER_SOURCE = "profiles/valu_counters_k_pairs_clist_ion_mixture_27648.md"
+# bytes per particle and average durations of the O(N) step kernels at
+# N = 27,648 (rocprofv3 kernel trace of this benchmark), and their HBM rate
+INTEGRATORS = None
 
 
 def parse():
@@ -64,15 +80,21 @@ def parse():
     ap.add_argument("--nind", type=int, default=10, help="measure every nind steps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-calls", type=int, default=25)
+    ap.add_argument("--cpu-calls-1", type=int, default=2, help="calls timed on one core")
     ap.add_argument("--forms", default="fp64,mixed")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-replicas", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--batch-crust", type=int, default=4,
                     help="replicas of the 27,648-ion mixture in the batched mixed form")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the pure-ion (N=3,456) and nucleon (N=51,200) lines")
+    ap.add_argument("--config-steps", type=int, default=100)
+    ap.add_argument("--no-drift", action="store_true",
+                    help="skip the energy drift over 1,000 steps (mixture and nucleons)")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--serve-rate", type=float, default=0.35, help="Poisson arrivals per s")
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-rate", type=float, default=1.5, help="Poisson arrivals per s")
     ap.add_argument("--serve-round", type=int, default=50, help="steps per serving round")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (reference "
@@ -99,6 +121,74 @@ def run_form(w, precision, steps, warmup, nind, timing):
         return {"seconds": t, "timing": tim, "records": recs}
     finally:
         s.close()
+
+
+def form_figures(w, prec, steps, warmup, nind, flops_pair=None, flops_pair_meas=None):
+    """Headline-style figures of one workload in one precision: the product
+    run (no events) for the rates, a second event-timed run of the same steps
+    for the force kernel's average duration and the pairs it evaluated."""
+    n = w.n
+    pairs = n * (n - 1) // 2
+    r = run_form(w, prec, steps, warmup, nind, timing=False)
+    t = r["seconds"]
+    tim = run_form(w, prec, steps, warmup, nind, timing=True)["timing"]
+    launches = max(tim["force_launches"], 1)
+    avg_ms = tim["force_ms"] / launches
+    swept = tim["pairs_swept"] / launches     # directed pairs evaluated per launch
+    out = {
+        "pair_interactions_per_s": pairs * steps / t,
+        "ms_per_step": 1e3 * t / steps,
+        "steps_per_s": steps / t,
+        "ns_per_day": w.dt * FM_C_SECONDS * 1e9 * (steps / t) * 86400,
+        "omega_p_t_per_day": 0.05 * (steps / t) * 86400,
+        "force_kernel_avg_ms": avg_ms,
+        "directed_pairs_evaluated_per_launch": swept,
+        "evaluated_fraction_of_all_pairs": swept / (n * (n - 1)),
+        "last_record": r["records"][-1] if r["records"] else None,
+    }
+    if flops_pair:
+        nmeas = steps // nind if nind else 0
+        flops = (steps - nmeas) * flops_pair + nmeas * flops_pair_meas
+        out["force_kernel_tflops"] = swept * flops / steps / (avg_ms * 1e-3) / 1e12
+    return out
+
+
+def in_cutoff_fraction(p):
+    """Fraction of all pairs inside the cutoff in a uniform fluid (rc <= L/2):
+    the cutoff sphere's share of the box."""
+    import math
+    return 4.0 / 3.0 * math.pi * p.rccut ** 3 / (p.xl[0] * p.xl[1] * p.xl[2])
+
+
+def energy_drift(w, warm_groups=20, warm_steps=25, steps=1000, nind=10, forms=("fp64", "mixed")):
+    """Relative energy drift over `steps` NVE steps: a thermostatted warm-up
+    from the random start in FP64 (warm_groups groups of warm_steps steps, each
+    followed by cancel_vcom + tnorm, as src/md.F:88-101), then from that state
+    `steps` steps in each precision, E = ev + ek per particle recorded every
+    nind steps; drift = max |E(t) - E(0)| / |E(0)|."""
+    from yukawamd_amd.system import from_workload
+    s = from_workload(w, precision="fp64")
+    try:
+        s.accel(True)
+        for _ in range(warm_groups):
+            s.run(warm_steps)
+            s.cancel_vcom()
+        x, v, _ = s.get_state()
+    finally:
+        s.close()
+    out = {"warmup_steps": warm_groups * warm_steps, "steps": steps, "nind": nind}
+    for prec in forms:
+        s = from_workload(w, precision=prec)
+        try:
+            s.put_state(x, v)
+            e0 = s.accel(True)["ev"] + s.ttot()
+            recs = s.run(steps, nind)
+        finally:
+            s.close()
+        e = [e0] + [r["ev"] + r["ek"] for r in recs]
+        out[prec] = {"relative_drift": max(abs(ei - e0) for ei in e) / abs(e0),
+                     "e0_mev": e0, "e_end_mev": e[-1]}
+    return out
 
 
 def run_batch(ws, steps, warmup, nind):
@@ -147,6 +237,7 @@ def run_serving(count, rate, round_steps):
         out.update({"rounds": srv.rounds, "peak_concurrent_runs": srv.peak_live,
                     "peak_footprint_gib": srv.peak_bytes / 2 ** 30,
                     "device_utilization": srv.device_ms * 1e-3 / wall})
+    out["window"] = f"requests 0-{count - 1} of the 2,000-request stream"
     out["workload"] = (f"first {count} requests of the serving stream (seed 1004): crust "
                        "mixture, N log-uniform [2048, 262144], steps U[100, 2000], Gamma "
                        f"U[150,220], kappa U[0.5,2], 80% mixed / 20% FP64; Poisson arrivals "
@@ -155,10 +246,8 @@ def run_serving(count, rate, round_steps):
     return out
 
 
-def cpu_baseline(calls):
+def cpu_baseline(calls, threads):
     env = dict(os.environ)
-    threads = len(os.sched_getaffinity(0))
-    threads = min(threads, 16)
     env["OMP_NUM_THREADS"] = str(threads)
     env.setdefault("OMP_PROC_BIND", "close")
     cmd = [sys.executable, "-m", "oracle.cpu_bench", "--workload", "ion_mixture_27648",
@@ -197,27 +286,8 @@ def main():
     pairs = n * (n - 1) // 2
     forms = {}
     for prec in args.forms.split(","):
-        r = run_form(w, prec, args.steps, args.warmup, args.nind, timing=False)
-        t = r["seconds"]
-        tim = run_form(w, prec, args.steps, args.warmup, args.nind, timing=True)["timing"]
-        launches = max(tim["force_launches"], 1)
-        avg_ms = tim["force_ms"] / launches
-        nmeas = args.steps // args.nind if args.nind else 0
-        flops = (args.steps - nmeas) * FLOPS_PER_PAIR_FP64 + nmeas * FLOPS_PER_PAIR_MEAS
-        swept = tim["pairs_swept"] / launches     # directed pairs evaluated per launch
-        achieved = swept * flops / args.steps / (avg_ms * 1e-3) / 1e12
-        forms[prec] = {
-            "pair_interactions_per_s": pairs * args.steps / t,
-            "ms_per_step": 1e3 * t / args.steps,
-            "steps_per_s": args.steps / t,
-            "ns_per_day": w.dt * FM_C_SECONDS * 1e9 * (args.steps / t) * 86400,
-            "omega_p_t_per_day": 0.05 * (args.steps / t) * 86400,
-            "force_kernel_avg_ms": avg_ms,
-            "force_kernel_tflops": achieved,
-            "directed_pairs_evaluated_per_launch": swept,
-            "evaluated_fraction_of_all_pairs": swept / (n * (n - 1)),
-            "last_record": r["records"][-1] if r["records"] else None,
-        }
+        forms[prec] = form_figures(w, prec, args.steps, args.warmup, args.nind,
+                                   FLOPS_PER_PAIR_FP64, FLOPS_PER_PAIR_MEAS)
     f64 = forms.get("fp64") or next(iter(forms.values()))
     line = {
         "metric": "pair interactions/s at N=27,648 (FP64 drop-in; mixed beside it)",
@@ -242,6 +312,24 @@ def main():
     if "mixed" in forms and "fp64" in forms:
         line["mixed_over_fp64"] = (forms["mixed"]["pair_interactions_per_s"]
                                    / forms["fp64"]["pair_interactions_per_s"])
+    if not args.no_configs:
+        cfg = {}
+        for wc, fl, flm, steps in ((W.pure_ion(), FLOPS_PER_PAIR_PURE, FLOPS_PER_PAIR_PURE_MEAS,
+                                    args.config_steps),
+                                   (W.nucleon(), None, None, max(args.config_steps // 2, 10))):
+            cfg[wc.name] = {"n": wc.n, "steps": steps, "dt_fm_per_c": wc.dt,
+                            "in_cutoff_fraction": in_cutoff_fraction(wc.params)}
+            for prec in args.forms.split(","):
+                cfg[wc.name][prec] = form_figures(wc, prec, steps, args.warmup, args.nind, fl, flm)
+                if fl:
+                    cfg[wc.name][prec]["roofline_frac_fp64"] = (
+                        cfg[wc.name][prec]["force_kernel_tflops"] / PEAK_FP64_TFLOPS)
+        line["configs"] = cfg
+    if not args.no_drift:
+        line["energy_drift_1000_steps"] = {
+            "ion_mixture_27648": energy_drift(w),
+            "nucleon_51200": energy_drift(W.nucleon()),
+        }
     if not args.no_batch:
         from yukawamd_amd.batch import TUNING_FILE
         crust = [W.ion_mixture(seed=2001 + k, precision="mixed") for k in range(args.batch_crust)]
@@ -261,6 +349,8 @@ def main():
         line["batched_ensemble"] = b
     if not args.no_serve:
         line["serving"] = run_serving(args.serve_requests, args.serve_rate, args.serve_round)
+    incut = in_cutoff_fraction(w.params) * n * (n - 1)    # directed pairs inside rc
+    swept = f64["directed_pairs_evaluated_per_launch"]
     line["roofline"] = {
         "bound": "valu",
         "peak_kind": "FP64 vector",
@@ -272,10 +362,29 @@ def main():
         "traffic": TRAFFIC_BYTES_PER_LAUNCH,
         "traffic_source": TRAFFIC_SOURCE,
         "flops_per_directed_pair": FLOPS_PER_PAIR_FP64,
-        "directed_pairs_per_launch": f64["directed_pairs_evaluated_per_launch"],
+        "directed_pairs_per_launch": swept,
+        # the pairs inside the cutoff (a uniform fluid's share) over those the
+        # lists make the kernel evaluate, and the rate counted on them alone
+        "in_cutoff_fraction_of_evaluated": incut / swept,
+        "useful_frac": f64["force_kernel_tflops"] / PEAK_FP64_TFLOPS * incut / swept,
+        # counter cross-check (rocprofv3 --pmc, file named below): VALU
+        # instructions x 4 cycles / (GPU-active cycles x SIMDs)
+        "valu_issue_frac": VALU_ISSUE_FRAC_FP64,
+        "valu_issue_frac_mixed": VALU_ISSUE_FRAC_MIXED,
+        "counter_source": COUNTER_SOURCE,
     }
+    if "mixed" in forms:
+        mx = forms["mixed"]
+        line["roofline"]["mixed_fp32_tflops"] = (
+            mx["directed_pairs_evaluated_per_launch"] * FLOPS_PER_PAIR_MIXED_FP32
+            / (mx["force_kernel_avg_ms"] * 1e-3) / 1e12)
+        line["roofline"]["mixed_fp32_frac"] = line["roofline"]["mixed_fp32_tflops"] / PEAK_FP32_TFLOPS
+        line["roofline"]["flops_per_directed_pair_mixed_fp32"] = FLOPS_PER_PAIR_MIXED_FP32
+    line["integrators"] = INTEGRATORS
     if not args.no_cpu:
-        cb = cpu_baseline(args.cpu_calls)
+        share = min(len(os.sched_getaffinity(0)), 16)
+        cb = cpu_baseline(args.cpu_calls, share)
+        c1 = cpu_baseline(args.cpu_calls_1, 1)
         if "error" in cb:
             line["cpu_baseline"] = {"value": None, "error": cb["error"]}
         else:
@@ -283,7 +392,14 @@ def main():
                 "value": cb["pair_interactions_per_s"], "unit": "pair interactions/s",
                 "cores": cb["threads"], "kind": cb["kind"],
                 "sample": f"{cb['calls']} force-only accel calls of ion_mixture_27648 "
-                          f"({cb['seconds']:.1f} s, {cb['cpu']})",
+                          f"({cb['seconds']:.1f} s, {cb['threads']} threads of a {cb['cpu']} "
+                          f"host with {os.cpu_count()} logical CPUs, this job's share "
+                          f"{len(os.sched_getaffinity(0))})",
+                "reference_pair_loop_s_per_call": cb["pair_loop_seconds"] / cb["calls"],
+                "one_core": None if "error" in c1 else {
+                    "value": c1["pair_interactions_per_s"], "calls": c1["calls"],
+                    "seconds": c1["seconds"],
+                    "reference_pair_loop_s_per_call": c1["pair_loop_seconds"] / c1["calls"]},
             }
     print(json.dumps(line), flush=True)
     return 0

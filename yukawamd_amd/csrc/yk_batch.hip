@@ -65,6 +65,7 @@ struct HostReplica {
     // arena, fixed for the replica's lifetime
     int list_items = 0, segcap = 0;
     int64_t lists_off = 0, lists_len = 0;
+    double near_r = -1.0;   // near-field radius of the mixed form (fm), set at admission
 };
 
 constexpr int64_t LIST_UNIT = 1 << 20;   // list-arena allocation unit (entries)
@@ -255,8 +256,7 @@ RepConsts rep_consts(const HostReplica &h)
         const double rlf = rl * (1.0 + 5e-6) + 1e-6 * lm;   // as the single-system bound
         c.rl2f = (float)(rlf * rlf);
         c.skinf = (float)c.skin;
-        const double rn = near_radius(1, p.precision == YK_MIXED, p.rho, p.xmuc, p.rccut, c.skin);
-        c.rn2f = rn < 0.0 ? -1.0f : (float)(rn * rn);
+        c.rn2f = h.near_r < 0.0 ? -1.0f : (float)(h.near_r * h.near_r);
     }
     c.rc2f_lo = (float)(c.rc2 * (1.0 - 4e-6));
     c.rc2f_hi = (float)(c.rc2 * (1.0 + 4e-6));
@@ -839,7 +839,9 @@ int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *
     h.dt = dt;
     h.steps_left = steps;
     h.list_items = list_items_for(b, n);
-    if (h.list_items > 0) {
+    if (h.list_items > 0) {   // skin 5 % of the cutoff, as rep_consts
+        h.near_r = near_radius(1, params->precision == YK_MIXED, params->rho, params->xmuc,
+                               params->rccut, 0.05 * params->rccut);
         h.segcap = segcap_of(n);
         h.lists_len = list_entries_of(n);
         if ((rc = list_alloc(b, h.lists_len, &h.lists_off))) {

@@ -83,6 +83,7 @@ struct yk_system {
     bool use_list = true;
     int list_M = 64;                         // work items per target tile / cluster set
     double skin = 0.0;
+    double near_r = -1.0;                    // near-field radius of the mixed ion form (fm)
     unsigned *list = nullptr;                // [ntiles][n]
     int2 *counts = nullptr;                  // [ntiles] (fast, general)
     bool use_clist = false;                  // 16-slot cluster lists on top of the tile lists
@@ -278,10 +279,17 @@ ListConsts list_consts(const yk_system *s)
         l.rl2f = (float)(rlf * rlf);
     }
     l.skinf = (float)s->skin;
-    const double rn = near_radius(p.sim_type == YK_PURE_ION || p.sim_type == YK_ION_MIXTURE,
-                                  p.precision == YK_MIXED, p.rho, p.xmuc, p.rccut, s->skin);
-    l.rn2f = rn < 0.0 ? -1.0f : (float)(rn * rn);
+    l.rn2f = s->near_r < 0.0 ? -1.0f : (float)(s->near_r * s->near_r);
     return l;
+}
+
+// The near-field radius for the system's parameters and skin (computed once
+// per parameter change: near_radius integrates on the host).
+void set_near_radius(yk_system *s)
+{
+    const yk_params &p = s->p;
+    s->near_r = near_radius(p.sim_type == YK_PURE_ION || p.sim_type == YK_ION_MIXTURE,
+                            p.precision == YK_MIXED, p.rho, p.xmuc, p.rccut, s->skin);
 }
 
 unsigned long long bits_of(double v)
@@ -1029,6 +1037,7 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
     s->list_M = (params->precision == YK_MIXED) ? LIST_ITEMS_MIXED : LIST_ITEMS_FP64;
     s->skin = 0.05 * list_cutoff(*params);
     if (!(s->skin < 0.05 * params->xl[0])) s->skin = 0.0;   // huge cutoff: every pair listed
+    set_near_radius(s);
     s->part_doubles = 11 * nn * (size_t)s->nchunks;
     if (s->use_list) {
         s->part_doubles = std::max(s->part_doubles,
@@ -1100,6 +1109,7 @@ int yk_set_params(yk_system *s, const yk_params *params)
         s->skin = 0.05 * list_cutoff(*params);
     if ((rc = change_box(s, params->xl))) return rc;
     s->p = *params;
+    set_near_radius(s);
     if (params->resort_every > 0) s->resort_every = params->resort_every;
     s->have_accel = false;
     return invalidate_list(s);

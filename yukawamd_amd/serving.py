@@ -15,7 +15,13 @@ first step).  Between rounds it
     (arena slots + work-item partial sums, as the engine reports it) fits in
     the free budget and the arena holds a contiguous run of slots for them;
     a request that does not fit waits, and later, smaller arrivals may pass
-    it (backfill) unless it has waited longer than ``max_bypass_s``.
+    it (backfill) unless it has waited longer than ``max_bypass_s``.  The
+    budget counts the engine's per-request footprints; the engine's own growth
+    headroom comes on top (its list arena at least doubles when it grows and
+    holds both copies while it copies; partial sums are allocated 25 % ahead),
+    and an admission that finds the device short of memory waits for a
+    retirement like one that finds the arena full.  The next pending requests'
+    initial states are generated on host threads while the device runs.
 
 A run's arithmetic depends only on its own state: its result is the same
 whether it ran alone or among others, admitted early or late.
@@ -26,6 +32,7 @@ its last step ran (its result downloaded).
 from __future__ import annotations
 
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 
 import numpy as np
@@ -105,6 +112,11 @@ class Server:
         self.round_steps = int(round_steps)
         self.nind = int(nind)
         self.max_bypass_s = float(max_bypass_s)
+        # initial states of the next pending requests, generated on host threads
+        # while the device runs a round (the engine call releases the GIL)
+        self._gen = ThreadPoolExecutor(max_workers=2)
+        self._states: dict = {}
+        self.prefetch = 8
         self.pending: list[Request] = []
         self.live: dict[int, _Live] = {}
         self.used_bytes = 0
@@ -115,7 +127,17 @@ class Server:
         self.peak_live = 0
 
     def close(self):
+        self._gen.shutdown(wait=True)
         self.ens.close()
+
+    def _prefetch(self):
+        for req in self.pending[:self.prefetch]:
+            if req.id not in self._states:
+                self._states[req.id] = self._gen.submit(req.workload)
+
+    def _workload(self, req):
+        fut = self._states.pop(req.id, None)
+        return fut.result() if fut is not None else req.workload()
 
     def __enter__(self):
         return self
@@ -145,13 +167,17 @@ class Server:
             ok = self.used_bytes + need <= self.budget
             rid = None
             if ok:
-                w = req.workload()
+                w = self._workload(req)
                 try:
                     rid = self.ens.add_workload(w, steps=req.steps)
-                except Exception as e:    # arena fragmentation / replica table full
-                    if getattr(e, "code", None) != 7:
+                except Exception as e:
+                    # arena fragmentation / replica table full (YK_ERR_CAPACITY),
+                    # or device memory short of the engine's own growth headroom
+                    # (YK_ERR_NOMEM): the request waits for a retirement
+                    if getattr(e, "code", None) not in (3, 7):
                         raise
                     rid = None
+                    self._states[req.id] = self._gen.submit(lambda w=w: w)
             if rid is None:
                 if blocked_head is None:
                     blocked_head = req
@@ -194,6 +220,7 @@ class Server:
                 self.submit(reqs[k])
                 k += 1
             self._admit(now)
+            self._prefetch()
             if self.live:
                 self._round()
                 self._retire(clock() - t0)
