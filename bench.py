@@ -236,7 +236,10 @@ def run_serving(count, rate, round_steps):
     arriving in real time at `rate` per second, served to completion."""
     from yukawamd_amd.serving import Server, stream, summarize
     reqs = stream(count, rate)
-    with Server(hbm_budget_bytes=64 * 2 ** 30, round_steps=round_steps) as srv:
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    with Server(hbm_budget_bytes=64 * 2 ** 30, round_steps=round_steps, log=log) as srv:
         t0 = time.perf_counter()
         res = srv.serve(reqs)
         wall = time.perf_counter() - t0

@@ -203,3 +203,46 @@ def test_serving_window_served_equals_alone():
             np.testing.assert_allclose(res.v, v, rtol=1e-8, atol=1e-11 * np.abs(v).max())
             assert res.records[-1]["ev"] == pytest.approx(m["ev"], rel=1e-11)
             assert res.records[-1]["ek"] == pytest.approx(m["ek"], rel=1e-10)
+
+
+# --------------------------------------------------------------------------
+def energy_drift(w, steps=1000, nind=10, warm_groups=20, warm_steps=25):
+    """Relative drift max |E(t) - E(0)| / |E(0)| of E = ev + ek per particle
+    over `steps` NVE steps in each precision, from the state of a thermostatted
+    FP64 warm-up (warm_groups groups of warm_steps steps, each followed by
+    cancel_vcom + tnorm, as src/md.F:88-101)."""
+    from yukawamd_amd.system import from_workload
+    with from_workload(w, precision="fp64") as s:
+        s.accel(True)
+        for _ in range(warm_groups):
+            s.run(warm_steps)
+            s.cancel_vcom()
+        x, v, _ = s.get_state()
+    out = {}
+    for prec in ("fp64", "mixed"):
+        with from_workload(w, precision=prec) as s:
+            s.put_state(x, v)
+            e0 = s.accel(True)["ev"] + s.ttot()
+            recs = s.run(steps, nind)
+        e = np.array([r["ev"] + r["ek"] for r in recs])
+        out[prec] = float(np.abs(e - e0).max() / abs(e0))
+    return out
+
+
+def test_energy_drift_ion_mixture_27648():
+    """The headline workload over 1,000 steps: the mixed form's relative drift
+    is at most 1e-5 (the stated tolerance) and no larger than the FP64 form's
+    beyond 1e-7."""
+    d = energy_drift(W.ion_mixture())
+    print(f"ion_mixture_27648 drift per 1,000 steps: fp64 {d['fp64']:.3e} mixed {d['mixed']:.3e}")
+    assert d["fp64"] <= 1e-5
+    assert d["mixed"] <= 1e-5 and d["mixed"] <= d["fp64"] + 1e-7
+
+
+def test_energy_drift_nucleon_51200():
+    """Nucleons (dt = 2 fm/c against the HPP core's ~15 fm/c period): the
+    drift is the integrator's, the same in both precisions within 20 %."""
+    d = energy_drift(W.nucleon())
+    print(f"nucleon_51200 drift per 1,000 steps: fp64 {d['fp64']:.3e} mixed {d['mixed']:.3e}")
+    assert d["fp64"] <= 1e-3
+    assert d["mixed"] <= 1.2 * d["fp64"]

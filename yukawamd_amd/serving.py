@@ -96,7 +96,7 @@ class Server:
     def __init__(self, hbm_budget_bytes: float = 64 * 2 ** 30, capacity_slots: int | None = None,
                  max_runs: int = 1024, round_steps: int = 50, nind: int = 0,
                  max_records: int = 64, max_bypass_s: float = 30.0, tuning=None,
-                 ensemble=None):
+                 ensemble=None, log=None):
         if nind and round_steps % nind:
             raise ValueError("round_steps must be a multiple of nind")
         self.budget = int(hbm_budget_bytes)
@@ -109,6 +109,7 @@ class Server:
                 capacity_slots = max(1 << 16, min(self.budget // 1024, 1 << 25))
             self.ens = Ensemble(capacity=capacity_slots, max_replicas=max_runs,
                                 max_records=max_records, tuning=tuning)
+        self.log = log            # callable(str) told of every completion, or None
         self.round_steps = int(round_steps)
         self.nind = int(nind)
         self.max_bypass_s = float(max_bypass_s)
@@ -207,6 +208,12 @@ class Server:
                 self.used_bytes -= lv.bytes
                 self.results.append(Result(lv.request, lv.admitted_s, now, g["x"], g["v"],
                                            g["a"], g["init"], g["records"]))
+                if self.log:
+                    r = lv.request
+                    self.log(f"served request {r.id} (N={r.n}, {r.steps} steps, {r.precision}) "
+                             f"at {now:.1f} s, latency {now - r.arrival_s:.2f} s; "
+                             f"{len(self.results)} done, {len(self.live)} running, "
+                             f"{len(self.pending)} waiting")
 
     def serve(self, requests, clock=time.perf_counter, sleep=time.sleep):
         """Serve `requests` (arrival_s relative to the call) to completion."""
