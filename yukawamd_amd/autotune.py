@@ -44,11 +44,13 @@ MAX_ITEMS_PER_TILE = 64
 BUCKET_SLOTS = 1 << 20          # particles per tuning ensemble (at least one replica)
 
 
-def bucket_replicas(lo: int, hi: int, seed: int):
+def bucket_replicas(lo: int, hi: int, seed: int, slots: int = BUCKET_SLOTS):
+    """The tuning ensemble of the bucket (lo, hi]: replicas until `slots`
+    particles (at most 256 replicas)."""
     from . import workloads as W
     rng = np.random.Generator(np.random.PCG64(seed))
     reps, total = [], 0
-    while total < BUCKET_SLOTS or not reps:
+    while total < slots or not reps:
         n = int(round(math.exp(rng.uniform(math.log(lo + 1), math.log(hi)))))
         n = max(lo + 1, min(hi, n))
         w = W.replica(int(rng.integers(0, 2 ** 31 - 1)), n, bool(rng.random() < 0.5),

@@ -26,8 +26,10 @@
 
 #include "../../include/yk.h"
 #include "yk_batch_kernels.hpp"
+#include "yk_batch_plan.hpp"
 
 using namespace yk;
+using plan::LIST_UNIT;
 
 namespace {
 
@@ -68,8 +70,6 @@ struct HostReplica {
     double near_r = -1.0;   // near-field radius of the mixed form (fm), set at admission
 };
 
-constexpr int64_t LIST_UNIT = 1 << 20;   // list-arena allocation unit (entries)
-
 }  // namespace
 
 struct yk_batch {
@@ -109,7 +109,7 @@ struct yk_batch {
     bool dirty = true;       // items / constants must be rebuilt
     bool need_accel = true;  // accelerations of new replicas are missing
     int nitems = 0;
-    std::vector<int> tune_nupper, tune_chunk, tune_waves, tune_list;
+    plan::Tuning tune;       // work shape per N bucket
     // cluster-list arena (entries) with first-fit unit bookkeeping
     unsigned *clist = nullptr;
     int64_t clist_cap = 0;
@@ -146,47 +146,6 @@ void drop_graph(yk_batch *b)
     b->graph = nullptr;
     b->g_steps = 0;
     b->g_nind = -1;
-}
-
-int bucket_of(const yk_batch *b, int n)
-{
-    for (size_t k = 0; k < b->tune_nupper.size(); ++k)
-        if (n <= b->tune_nupper[k]) return (int)k;
-    return -1;
-}
-
-int chunk_tiles_for(const yk_batch *b, int n)
-{
-    const int nt = (n + TILE - 1) / TILE;
-    const int k = bucket_of(b, n);
-    if (k >= 0) return std::max(1, std::min(nt, b->tune_chunk[k]));
-    return std::max(1, (nt + 3) / 4);   // default: four source ranges per target tile
-}
-
-// Work items per tile of a list replica (0: tile items).
-int list_items_for(const yk_batch *b, int n)
-{
-    if (n > CLIST_MAX_N) return 0;
-    const int k = bucket_of(b, n);
-    if (k >= 0) return b->tune_list[k];
-    return 16;
-}
-
-int segcap_of(int n) { return ((n + TILE - 1) / TILE + NSEG - 1) / NSEG * TILE; }
-
-int64_t list_entries_of(int n)
-{
-    return (int64_t)((n + CL - 1) / CL) * NSEG * segcap_of(n);
-}
-
-// Work items of a replica of n particles and their device bytes.
-int64_t items_of(const yk_batch *b, int n)
-{
-    const int64_t nt = (n + TILE - 1) / TILE;
-    const int M = list_items_for(b, n);
-    if (M > 0) return nt * M;
-    const int64_t C = chunk_tiles_for(b, n);
-    return nt * ((nt + C - 1) / C);
 }
 
 // device bytes per arena slot (pos, vel/vold/acc, aii, den, orig, rep), per
@@ -269,23 +228,11 @@ RepConsts rep_consts(const HostReplica &h)
 int list_alloc(yk_batch *b, int64_t len, int64_t *off)
 {
     const int64_t units = (len + LIST_UNIT - 1) / LIST_UNIT;
-    int64_t start = -1;
-    for (int64_t u = 0, run = 0; u < (int64_t)b->list_used.size(); ++u) {
-        run = b->list_used[u] ? 0 : run + 1;
-        if (run == units) {
-            start = u - units + 1;
-            break;
-        }
-    }
+    int64_t start = plan::first_fit(b->list_used, units);
     if (start < 0) {
-        // grow: keep the used prefix, append enough free units
-        int64_t tail = 0;
-        while (tail < (int64_t)b->list_used.size() && !b->list_used[b->list_used.size() - 1 - tail])
-            ++tail;
-        const int64_t old_units = (int64_t)b->list_used.size();
-        const int64_t new_units = std::max(old_units + units - tail, 2 * old_units);
+        const plan::Growth g = plan::arena_growth(b->list_used, units);
         unsigned *nc = nullptr;
-        YB_HIP(hipMalloc(&nc, sizeof(unsigned) * (size_t)(new_units * LIST_UNIT)));
+        YB_HIP(hipMalloc(&nc, sizeof(unsigned) * (size_t)(g.new_units * LIST_UNIT)));
         if (b->clist) {
             YB_HIP(hipMemcpyAsync(nc, b->clist, sizeof(unsigned) * (size_t)b->clist_cap,
                                   hipMemcpyDeviceToDevice, b->stream));
@@ -293,10 +240,10 @@ int list_alloc(yk_batch *b, int64_t len, int64_t *off)
             YB_HIP(hipFree(b->clist));
         }
         b->clist = nc;
-        b->clist_cap = new_units * LIST_UNIT;
-        b->list_used.resize((size_t)new_units, 0);
+        b->clist_cap = g.new_units * LIST_UNIT;
+        b->list_used.resize((size_t)g.new_units, 0);
         b->dirty = true;   // kernels and the captured graph take the new pointer
-        start = old_units - tail;
+        start = g.start;
     }
     for (int64_t u = start; u < start + units; ++u) b->list_used[(size_t)u] = 1;
     *off = start * LIST_UNIT;
@@ -321,81 +268,32 @@ __global__ void k_fill_rep(int *rep, const int *tile_rep, const RepConsts *reps,
 // the current membership.
 int rebuild(yk_batch *b)
 {
+    static_assert(sizeof(plan::Item) == sizeof(BatchItem), "BatchItem is plan::Item");
+    static_assert(sizeof(int2) == 2 * sizeof(int), "tile_items are (first, count) pairs");
     drop_graph(b);
     const int ntiles_cap = b->cap / TILE;
     std::vector<RepConsts> rc(b->max_rep);
-    std::vector<int> tile_rep(ntiles_cap, -1);
-    std::vector<int2> tile_items(ntiles_cap, make_int2(0, 0));
-    std::vector<BatchItem> items;
-    std::vector<int> order;
+    std::vector<plan::Member> members(b->max_rep);
     for (int r = 0; r < b->max_rep; ++r) {
         const HostReplica &h = b->table[r];
-        if (h.active) {
-            rc[r] = rep_consts(h);
-            order.push_back(r);
-        } else {
-            rc[r] = RepConsts{};
-        }
+        rc[r] = h.active ? rep_consts(h) : RepConsts{};
+        members[r] = plan::Member{h.active, h.n, h.off, h.nblocks, h.list_items};
     }
-    // largest replicas first: their items start early and the queue's tail
-    // is made of small items
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int c) { return b->table[a].n > b->table[c].n; });
-    b->nlist = 0;
-    for (int r : order) {
-        const HostReplica &h = b->table[r];
-        const int t0 = h.off / TILE, nt = (h.n + TILE - 1) / TILE;
-        const int C = chunk_tiles_for(b, h.n);
-        const int M = h.list_items;
-        b->nlist += (M > 0) ? 1 : 0;
-        for (int t = t0; t < t0 + nt; ++t) {
-            tile_rep[t] = r;
-            tile_items[t].x = (int)items.size();
-            if (M > 0) {
-                for (int m = 0; m < M; ++m) items.push_back(BatchItem{r, t, m, -M});
-            } else {
-                for (int s = t0; s < t0 + nt; s += C)
-                    items.push_back(BatchItem{r, t, s, std::min(s + C, t0 + nt)});
-            }
-            tile_items[t].y = (int)items.size() - tile_items[t].x;
-        }
-    }
-    b->nitems = (int)items.size();
-    b->span = 0;
-    std::vector<int> ltiles;
-    for (int r : order) {
-        const HostReplica &h = b->table[r];
-        b->span = std::max(b->span, h.off + h.nblocks * BLOCK);
-        if (h.list_items > 0)
-            for (int t = h.off / TILE; t < h.off / TILE + (h.n + TILE - 1) / TILE; ++t)
-                ltiles.push_back(t);
-    }
-    b->n_list_tiles = (int)ltiles.size();
-    if (ltiles.size() > b->list_tiles_cap) {
+    const plan::Layout L = plan::layout(b->tune, members, b->cap);
+    b->nitems = (int)L.items.size();
+    b->nlist = L.nlist;
+    b->span = L.span;
+    b->waves = L.waves;
+    b->n_list_tiles = (int)L.list_tiles.size();
+    if (L.list_tiles.size() > b->list_tiles_cap) {
         if (b->list_tiles) YB_HIP(hipFree(b->list_tiles));
         b->list_tiles = nullptr;
-        b->list_tiles_cap = ltiles.size() + ltiles.size() / 4 + 64;
+        b->list_tiles_cap = L.list_tiles.size() + L.list_tiles.size() / 4 + 64;
         YB_HIP(hipMalloc(&b->list_tiles, sizeof(int) * b->list_tiles_cap));
     }
-    if (!ltiles.empty())
-        YB_HIP(hipMemcpyAsync(b->list_tiles, ltiles.data(), sizeof(int) * ltiles.size(),
+    if (!L.list_tiles.empty())
+        YB_HIP(hipMemcpyAsync(b->list_tiles, L.list_tiles.data(), sizeof(int) * L.list_tiles.size(),
                               hipMemcpyHostToDevice, b->stream));
-    // launch shape: the waves per workgroup of the bucket carrying the most
-    // pair work
-    {
-        std::vector<double> work(b->tune_nupper.size(), 0.0);
-        for (int r : order) {
-            const int k = bucket_of(b, b->table[r].n);
-            if (k >= 0) work[k] += (double)b->table[r].n * b->table[r].n;
-        }
-        b->waves = 4;
-        double best = 0.0;
-        for (size_t k = 0; k < work.size(); ++k)
-            if (work[k] > best) {
-                best = work[k];
-                b->waves = b->tune_waves[k];
-            }
-    }
     if ((size_t)b->nitems > b->part_items_cap) {
         if (b->part) YB_HIP(hipFree(b->part));
         if (b->items) YB_HIP(hipFree(b->items));
@@ -403,32 +301,23 @@ int rebuild(yk_batch *b)
         b->part = nullptr;
         b->items = nullptr;
         b->qorder = nullptr;
-        size_t want = std::max<size_t>((size_t)b->nitems + b->nitems / 4, 1024);
+        b->part_items_cap = 0;
+        const size_t want = plan::item_capacity(b->nitems);
         YB_HIP(hipMalloc(&b->part, sizeof(double) * 11 * TILE * want));
         YB_HIP(hipMalloc(&b->items, sizeof(BatchItem) * want));
         YB_HIP(hipMalloc(&b->qorder, sizeof(int) * want));
         b->part_items_cap = want;
     }
-    // queue order: a list item's share m leads with the list's back entries
-    // (FP64 in the mixed form), so shares go out by m -- every list's share 0
-    // first, tile items with them -- and the longest items start first;
-    // otherwise the build order (largest replica first)
-    std::vector<int> qorder(items.size());
-    std::iota(qorder.begin(), qorder.end(), 0);
-    std::stable_sort(qorder.begin(), qorder.end(), [&](int a, int c) {
-        const int ma = items[a].s1 < 0 ? items[a].s0 : 0, mc = items[c].s1 < 0 ? items[c].s0 : 0;
-        return ma < mc;
-    });
     YB_HIP(hipMemcpyAsync(b->reps, rc.data(), sizeof(RepConsts) * b->max_rep,
                           hipMemcpyHostToDevice, b->stream));
-    YB_HIP(hipMemcpyAsync(b->tile_rep, tile_rep.data(), sizeof(int) * ntiles_cap,
+    YB_HIP(hipMemcpyAsync(b->tile_rep, L.tile_rep.data(), sizeof(int) * ntiles_cap,
                           hipMemcpyHostToDevice, b->stream));
-    YB_HIP(hipMemcpyAsync(b->tile_items, tile_items.data(), sizeof(int2) * ntiles_cap,
+    YB_HIP(hipMemcpyAsync(b->tile_items, L.tile_items.data(), sizeof(int2) * ntiles_cap,
                           hipMemcpyHostToDevice, b->stream));
     if (b->nitems) {
-        YB_HIP(hipMemcpyAsync(b->items, items.data(), sizeof(BatchItem) * b->nitems,
+        YB_HIP(hipMemcpyAsync(b->items, L.items.data(), sizeof(BatchItem) * b->nitems,
                               hipMemcpyHostToDevice, b->stream));
-        YB_HIP(hipMemcpyAsync(b->qorder, qorder.data(), sizeof(int) * b->nitems,
+        YB_HIP(hipMemcpyAsync(b->qorder, L.qorder.data(), sizeof(int) * b->nitems,
                               hipMemcpyHostToDevice, b->stream));
     }
     hipLaunchKernelGGL(k_fill_rep, dim3((b->cap + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, b->stream,
@@ -439,14 +328,7 @@ int rebuild(yk_batch *b)
     return YK_OK;
 }
 
-int persistent_blocks(const yk_batch *b)
-{
-    // fill every CU with 32 waves (8 per SIMD), never more blocks than items
-    int per_cu = std::max(1, 32 / b->waves);
-    int blocks = b->cus * per_cu;
-    int need = (b->nitems + b->waves - 1) / b->waves;
-    return std::max(1, std::min(blocks, need));
-}
+int persistent_blocks(const yk_batch *b) { return plan::persistent_blocks(b->cus, b->waves, b->nitems); }
 
 // One step (MODE 1) or the accel pass (MODE 0), stream-ordered, graph-safe:
 // no memsets, the queue counter is reset by the per-replica kernel.
@@ -781,9 +663,9 @@ int yk_batch_footprint(const yk_batch *b, int32_t n, int64_t *slots_out, int64_t
     if (slots_out) *slots_out = slots;
     if (bytes_out)
         *bytes_out = slots * SLOT_BYTES + slots / TILE * TILE_BYTES + slots / BLOCK * BLOCK_BYTES +
-                     items_of(b, n) * ITEM_BYTES +
-                     (list_items_for(b, n) > 0
-                          ? (list_entries_of(n) + LIST_UNIT - 1) / LIST_UNIT * LIST_UNIT *
+                     plan::items_of(b->tune, n) * ITEM_BYTES +
+                     (b->tune.list_items_for(n) > 0
+                          ? (plan::list_entries_of(n) + LIST_UNIT - 1) / LIST_UNIT * LIST_UNIT *
                                 (int64_t)sizeof(unsigned)
                           : 0);
     return YK_OK;
@@ -807,14 +689,7 @@ int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *
         }
     if (id < 0) return bfail(YK_ERR_CAPACITY, "all %d replica slots are in use", b->max_rep);
     const int nblk = (n + BLOCK - 1) / BLOCK;
-    int start = -1;
-    for (int s = 0, run = 0; s < (int)b->used.size(); ++s) {
-        run = b->used[s] ? 0 : run + 1;
-        if (run == nblk) {
-            start = s - nblk + 1;
-            break;
-        }
-    }
+    const int start = (int)plan::first_fit(b->used, nblk);
     if (start < 0)
         return bfail(YK_ERR_CAPACITY, "no room for %d particles in the arena", n);
     for (size_t k = 0; k < (size_t)3 * n; ++k)
@@ -838,12 +713,12 @@ int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *
     h.nblocks = nblk;
     h.dt = dt;
     h.steps_left = steps;
-    h.list_items = list_items_for(b, n);
+    h.list_items = b->tune.list_items_for(n);
     if (h.list_items > 0) {   // skin 5 % of the cutoff, as rep_consts
         h.near_r = near_radius(1, params->precision == YK_MIXED, params->rho, params->xmuc,
                                params->rccut, 0.05 * params->rccut);
-        h.segcap = segcap_of(n);
-        h.lists_len = list_entries_of(n);
+        h.segcap = plan::segcap_of(n);
+        h.lists_len = plan::list_entries_of(n);
         if ((rc = list_alloc(b, h.lists_len, &h.lists_off))) {
             h = HostReplica{};
             return rc;
@@ -943,10 +818,10 @@ int yk_batch_set_tuning(yk_batch *b, int32_t nbuckets, const int32_t *n_upper,
                          "of 2/16/32/64 source tiles are disabled)",
                          k, w, li, ct);
     }
-    b->tune_nupper.assign(n_upper, n_upper + nbuckets);
-    b->tune_chunk.assign(chunk_tiles, chunk_tiles + nbuckets);
-    b->tune_waves.assign(waves_per_block, waves_per_block + nbuckets);
-    b->tune_list.assign(list_items, list_items + nbuckets);
+    b->tune.n_upper.assign(n_upper, n_upper + nbuckets);
+    b->tune.chunk_tiles.assign(chunk_tiles, chunk_tiles + nbuckets);
+    b->tune.waves.assign(waves_per_block, waves_per_block + nbuckets);
+    b->tune.list_items.assign(list_items, list_items + nbuckets);
     b->dirty = true;
     return YK_OK;
 }

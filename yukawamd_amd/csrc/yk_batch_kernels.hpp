@@ -17,6 +17,7 @@
 // so a replica's result does not depend on which wave ran which item nor on
 // the other replicas in the launch.
 #pragma once
+#include "yk_batch_plan.hpp"
 #include "yk_kernels.hpp"
 
 namespace yk {
@@ -198,8 +199,8 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
         sc[sg] = cvalid ? ccounts[(size_t)clg * NSEG + sg] : make_int2(0, 0);
         total += sc[sg].x + sc[sg].y;
     }
-    const int per = (total + M - 1) / M;
-    const int lo = min(m * per, total), hi = min(lo + per, total);
+    int lo = 0, hi = 0;
+    plan::list_share(total, M, m, lo, hi);
     int iters = hi - lo;
 #pragma unroll
     for (int off = CL; off < TILE; off <<= 1) {

@@ -29,15 +29,13 @@
 #include <hip/hip_runtime.h>
 #include <vector>
 
+#include "yk_layout.hpp"
 #include "yk_math.hpp"
 #include "yk_sfc.hpp"
 
 namespace yk {
 
-constexpr int BLOCK = 256;   // threads per workgroup (4 wave64)
-constexpr int TILE = 64;     // particles per tile (one wave)
 constexpr int STAGE = 256;   // sources staged in LDS at a time (4 tiles)
-constexpr int NQ = 14;       // reduced scalars: E, W, PP[6], KE[6]
 
 enum Kind { PURE = 0, MIX = 1, NN = 2 };
 
@@ -402,7 +400,6 @@ struct ListConsts {
     float rn2f;
 };
 
-constexpr unsigned SLOT_MASK = (1u << 26) - 1u;
 
 // Near field of the mixed ion form's cluster lists: sources within the near
 // radius (plus the skin, so those pairs stay near until the next build) of a
@@ -762,15 +759,7 @@ static __global__ __launch_bounds__(BLOCK) void k_tile_boxes(const double4 *__re
 // pairs.  Fast entries (one image for every target of the cluster while no
 // pair's separation has changed by more than the skin) fill the cluster's
 // region from the front, general ones from the back, both in slot order.
-constexpr int CL = 16;              // slots per cluster
-constexpr int CPW = TILE / CL;      // clusters per wave in the pair kernel
-constexpr int CLIST_MAX_N = 65536;  // above this the tile lists are used
-// A cluster's list is stored as NSEG segments: segment s holds the entries
-// whose source tile index is s mod NSEG (each built by its own workgroup, for
-// parallelism), each with its fast entries from the front and general ones
-// from the back of a region of segcap entries; the list is the segments in
-// order.
-constexpr int NSEG = 4;
+// Cluster size, segments and the size limit: yk_layout.hpp.
 
 // Candidate test of the cluster-list builds.  A workgroup stages every
 // candidate source once: its offset from the target tile's centre (the FP64
@@ -1107,48 +1096,8 @@ static __global__ __launch_bounds__(CORDER_THREADS) void k_corder(
 // list is the back entries of segments 0..3, then their fast entries, so that
 // the entries the mixed kernels evaluate in FP64 come first in every list.
 // entry(v): entry v and whether it is a back entry (0 at or past hi).
-struct SegList {
-    const unsigned *base;
-    int segcap, hi;
-    int nback;           // back entries of all segments
-    int g1, g2, g3;      // back entries of segments below 1, 2, 3
-    int f1, f2, f3;      // fast entries of segments below 1, 2, 3
-    __device__ __forceinline__ unsigned entry(int v, bool &back) const
-    {
-        back = false;
-        if (v >= hi) return 0u;
-        back = v < nback;
-        // arithmetic selection of the segment (selects of addresses would put
-        // the fields on the stack)
-        const int w = back ? v : v - nback;
-        const int c1 = back ? g1 : f1, c2 = back ? g2 : f2, c3 = back ? g3 : f3;
-        const int s1 = w >= c1, s2 = w >= c2, s3 = w >= c3;
-        const int sg = s1 + s2 + s3;
-        const int start = s1 * (c1 + s2 * (c2 - c1 + s3 * (c3 - c2)));
-        const int o = w - start;
-        const unsigned *r = base + (size_t)sg * segcap;
-        return back ? r[segcap - 1 - o] : r[o];
-    }
-};
-
-// A cluster's segment counts (fast, back) as a SegList limited to [0, hi).
-__device__ __forceinline__ SegList seg_list(const unsigned *base, int segcap, const int2 (&sc)[NSEG],
-                                            int hi)
-{
-    static_assert(NSEG == 4, "SegList holds four segments");
-    SegList s;
-    s.base = base;
-    s.segcap = segcap;
-    s.hi = hi;
-    s.g1 = sc[0].y;
-    s.g2 = s.g1 + sc[1].y;
-    s.g3 = s.g2 + sc[2].y;
-    s.nback = s.g3 + sc[3].y;
-    s.f1 = sc[0].x;
-    s.f2 = s.f1 + sc[1].x;
-    s.f3 = s.f2 + sc[2].x;
-    return s;
-}
+// SegList and seg_list (a cluster's list as its segments' entries, back
+// entries first): yk_layout.hpp.
 
 // The far pairs of the mixed ion form, FP32 throughout: target and source as
 // FP32 offsets from the cluster's centre (each the rounding of an FP64
