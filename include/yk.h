@@ -129,8 +129,9 @@ typedef struct yk_timing {
     int64_t force_launches; /* number of force-kernel launches timed       */
     double step_ms;         /* sum of whole-step durations (HIP events)    */
     int64_t steps;          /* number of steps timed                       */
-    int64_t pairs_swept;    /* directed pairs evaluated since the last     */
-                            /* yk_reset_timing (masked lanes included)     */
+    int64_t pairs_swept;    /* directed pairs evaluated by the timed force */
+                            /* launches since the last yk_reset_timing     */
+                            /* (masked lanes included)                     */
 } yk_timing;
 
 /* Library -------------------------------------------------------------- */

@@ -634,7 +634,7 @@ void batch_list_item(Engine &e, Checker &ck, const plan::Item &it, const Sizes &
             total += sc[sg].x + sc[sg].y;
             cnt[g][sg] = sc[sg];
         }
-        plan::list_share(total, M, m, lo[g], hi[g]);
+        list_share(total, M, m, lo[g], hi[g]);
         if (e.plant == 2 && m == M - 1 && total > 0) hi[g] += 1;
         iters = std::max(iters, hi[g] - lo[g]);
         sl[g] = seg_list(nullptr, h.segcap, sc, hi[g]);

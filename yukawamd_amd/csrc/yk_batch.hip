@@ -3,9 +3,9 @@
 // Many independent ion systems ("replicas": pure or mixture, FP64 or mixed,
 // each with its own N, box, screening length, cutoff, timestep and step
 // budget) live in one device arena and advance together: per step one drift
-// launch, one persistent pair launch pulling (replica, tile, source range)
-// items from a device queue, one finalize launch and one per-replica launch
-// (records, step countdown, queue reset).  The step sequence is captured
+// launch, one persistent pair launch pulling (replica, tile, source range or
+// list share) items from a device queue, one finalize launch and one
+// per-replica launch (records, step countdown, queue reset).  The step sequence is captured
 // once as a hipGraph and replayed.  Replicas can be added and removed
 // between runs (continuous batching); a replica's arithmetic does not depend
 // on what else is in the arena.  Only the vel_verlet_2 integrator without a
@@ -88,11 +88,11 @@ struct yk_batch {
     MeasOut *records = nullptr, *init_records = nullptr;
     BatchItem *items = nullptr;
     int *qorder = nullptr;    // queue position -> item (list shares 0 first)
+    unsigned *queue = nullptr;   // [QPARTS * QSTRIDE] queue counters of the pair launch
     int2 *tile_items = nullptr;
     double *part = nullptr;
     size_t part_items_cap = 0;
     double *blockpart = nullptr;
-    unsigned *queue = nullptr;
     // arena re-sort: keys, slot values, radix-sort scratch, gather targets
     unsigned long long *keys = nullptr, *keys2 = nullptr;
     int *vals = nullptr, *vals2 = nullptr, *due = nullptr;
@@ -331,7 +331,7 @@ int rebuild(yk_batch *b)
 int persistent_blocks(const yk_batch *b) { return plan::persistent_blocks(b->cus, b->waves, b->nitems); }
 
 // One step (MODE 1) or the accel pass (MODE 0), stream-ordered, graph-safe:
-// no memsets, the queue counter is reset by the per-replica kernel.
+// no memsets, the queue counters are reset by the per-replica kernel.
 template <bool MEAS, int MODE>
 int enqueue_step(yk_batch *b)
 {
@@ -363,8 +363,8 @@ int enqueue_step(yk_batch *b)
 // the last pass; the others keep theirs.
 int accel_pass(yk_batch *b)
 {
-    // the queue counter is zero here: it starts zeroed and every step's
-    // per-replica kernel resets it for the next launch
+    // the queue counters are zero here: they start zeroed and every step's
+    // per-replica kernel resets them for the next launch
     int rc = enqueue_step<true, 0>(b);
     if (rc) return rc;
     YB_HIP(hipStreamSynchronize(b->stream));
@@ -583,7 +583,7 @@ int yk_batch_create(int64_t capacity, int32_t max_replicas, int32_t max_records,
     YB_HIPB(hipMalloc(&b->state, sizeof(RepState) * (size_t)max_replicas));
     YB_HIPB(hipMalloc(&b->records, sizeof(MeasOut) * (size_t)max_replicas * b->rec_cap));
     YB_HIPB(hipMalloc(&b->init_records, sizeof(MeasOut) * (size_t)max_replicas));
-    YB_HIPB(hipMalloc(&b->queue, sizeof(unsigned)));
+    YB_HIPB(hipMalloc(&b->queue, sizeof(unsigned) * QPARTS * QSTRIDE));
     YB_HIPB(hipMalloc(&b->xref, sizeof(double4) * nn));
     YB_HIPB(hipMalloc(&b->ccounts, sizeof(int2) * (nn / CL) * NSEG));
     {
@@ -619,7 +619,7 @@ int yk_batch_create(int64_t capacity, int32_t max_replicas, int32_t max_records,
     YB_HIPB(hipMemsetAsync(b->vold, 0, sizeof(double) * 3 * nn, b->stream));
     YB_HIPB(hipMemsetAsync(b->acc, 0, sizeof(double) * 3 * nn, b->stream));
     YB_HIPB(hipMemsetAsync(b->state, 0, sizeof(RepState) * (size_t)max_replicas, b->stream));
-    YB_HIPB(hipMemsetAsync(b->queue, 0, sizeof(unsigned), b->stream));
+    YB_HIPB(hipMemsetAsync(b->queue, 0, sizeof(unsigned) * QPARTS * QSTRIDE, b->stream));
     std::vector<double> one(nn, 1.0);
     YB_HIPB(hipMemcpyAsync(b->aii, one.data(), sizeof(double) * nn, hipMemcpyHostToDevice,
                            b->stream));

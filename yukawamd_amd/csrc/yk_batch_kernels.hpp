@@ -9,10 +9,15 @@
 // arena capacity), aii, den, orig (caller index inside the replica).
 //
 // Pair forces: one persistent launch.  Work items (replica, target tile,
-// range of source tiles of the same replica) are pulled from a device queue
-// with an atomic counter; each wave owns one item at a time, stages source
-// tiles in its own LDS slice, culls them by AABB and takes the shift-only
-// minimum image when the two tiles allow it.  Every item writes its own
+// range of source tiles of the same replica, or a share of a tile's cluster
+// lists) are pulled from a device queue split into QPARTS counters: the
+// workgroups with blockIdx.x = p (mod QPARTS) take the queue positions
+// p, p + QPARTS, p + 2 QPARTS, ... in order, one atomic per item on their own
+// counter (an atomic on one address from every CU is served one at a time
+// beyond the XCDs' L2s, about 25 ns each; workgroups go to the 8 XCDs round
+// robin, so each counter serves about one XCD).  Each wave owns one item at a
+// time, stages source tiles in its own LDS slice, culls them by AABB and
+// takes the shift-only minimum image when the two tiles allow it.  Every item writes its own
 // partial sums, and the finalize kernel adds a tile's items in item order,
 // so a replica's result does not depend on which wave ran which item nor on
 // the other replicas in the launch.
@@ -48,6 +53,11 @@ struct RepState {            // device-updated per replica
     int fresh;               // 1 until the accel pass has evaluated the admitted state
     int pad0;
 };
+
+// The pair launch's work queue: QPARTS counters, QSTRIDE unsigneds apart (one
+// 256-byte line each).
+constexpr int QPARTS = 8;
+constexpr int QSTRIDE = 64;
 
 struct BatchItem {
     // tile item: target tile and source tiles [s0, s1), global indices;
@@ -200,7 +210,7 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
         total += sc[sg].x + sc[sg].y;
     }
     int lo = 0, hi = 0;
-    plan::list_share(total, M, m, lo, hi);
+    list_share(total, M, m, lo, hi);
     int iters = hi - lo;
 #pragma unroll
     for (int off = CL; off < TILE; off <<= 1) {
@@ -221,7 +231,8 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
     }
 }
 
-// Persistent: every wave loops pulling items until the queue is empty.
+// Persistent: every wave pulls queue positions from its workgroup's part of
+// the queue until that part is empty (qorder: position -> item).
 // accel_pass != 0 evaluates only replicas admitted since the last pass
 // (fresh); otherwise replicas whose step budget is used up are skipped.
 template <bool MEAS>
@@ -246,11 +257,17 @@ __global__ __launch_bounds__(512) void k_batch_pairs(const double4 *__restrict__
     load_exp_table(etab_s, etab_g);
     const double *etab = etab_s;
     constexpr int NC = MEAS ? 11 : 3;
+    // parts actually served: a launch of fewer than QPARTS workgroups uses
+    // as many parts as it has workgroups
+    const int parts = gridDim.x < (unsigned)QPARTS ? (int)gridDim.x : QPARTS;
+    const int part_id = (int)(blockIdx.x % (unsigned)parts);
+    unsigned *counter = queue + part_id * QSTRIDE;
     for (;;) {
-        unsigned k = 0;
-        if (lane == 0) k = atomicAdd(queue, 1u);
-        k = (unsigned)__builtin_amdgcn_readfirstlane((int)__shfl(k, 0, 64));
-        if (k >= (unsigned)nitems) break;   // every wave reaches this exit
+        unsigned j = 0;
+        if (lane == 0) j = atomicAdd(counter, 1u);
+        j = (unsigned)__builtin_amdgcn_readfirstlane((int)__shfl(j, 0, 64));
+        const long long k = (long long)part_id + (long long)parts * j;
+        if (k >= nitems) break;   // every wave reaches this exit
         const int q = qorder[k];             // a permutation of [0, nitems)
         const BatchItem it = items[q];
         double *out = part + (size_t)q * NC * TILE;
@@ -560,8 +577,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch_finalize(
 }
 
 // One workgroup per replica: optional measurement record (fixed-order sum of
-// the replica's finalize blocks), step countdown, and the queue reset for
-// the next step (block 0).
+// the replica's finalize blocks), step countdown, and the reset of the pair
+// launch's queue counters for the next step (block 0).
 // MODE 0 (accel pass of freshly admitted replicas) writes their record to
 // init_records[r] and clears fresh; MODE 1 appends to records[r][rec_count]
 // and counts a step.
@@ -576,7 +593,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_replica(const double *__restric
                                                          unsigned long long *__restrict__ maxdisp)
 {
     const int r = blockIdx.x;
-    if (r == 0 && threadIdx.x == 0) *queue = 0u;
+    if (r == 0 && threadIdx.x < QPARTS) queue[threadIdx.x * QSTRIDE] = 0u;
     if (r >= nrep) return;
     const RepConsts &rc = reps[r];
     // the list build of this step has consumed the displacement flag

@@ -201,14 +201,6 @@ inline int persistent_blocks(int cus, int waves, int nitems)
     return std::max(1, std::min(blocks, need));
 }
 
-// A list item's share m of M of one cluster's list of `total` entries:
-// [lo, hi).  constexpr, so the pair kernels call it too.
-constexpr void list_share(int total, int M, int m, int &lo, int &hi)
-{
-    const int per = (total + M - 1) / M;
-    lo = m * per < total ? m * per : total;
-    hi = lo + per < total ? lo + per : total;
-}
 
 }  // namespace plan
 }  // namespace yk

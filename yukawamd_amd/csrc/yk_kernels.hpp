@@ -611,8 +611,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_list(const double4 *__restrict_
                                                       ForceConsts c, int cap, int M,
                                                       double *__restrict__ part,
                                                       unsigned long long *__restrict__ maxdisp,
-                                                      double skin,
-                                                      unsigned long long *__restrict__ swept)
+                                                      double skin)
 {
     __shared__ double4 sj[BLOCK / TILE][TILE];
     const int n = c.n;
@@ -643,7 +642,6 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_list(const double4 *__restrict_
     const unsigned *mine = list + (size_t)t * cap;
     Acc<MEAS> acc;
     double4 *sw = sj[wv];
-    if (lane == 0 && hi > lo) atomicAdd(swept, (unsigned long long)(hi - lo) * TILE);
     // Software pipeline over batches of 64 entries: while batch k is computed
     // from LDS, the positions of batch k+1 and the entries of batch k+2 are
     // in flight in registers.
@@ -702,6 +700,32 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_list(const double4 *__restrict_
         o[5 * TILE] = acc.pxx; o[6 * TILE] = acc.pxy; o[7 * TILE] = acc.pxz;
         o[8 * TILE] = acc.pyy; o[9 * TILE] = acc.pyz; o[10 * TILE] = acc.pzz;
     }
+}
+
+// Directed pairs a k_pairs_list launch over the current lists evaluates: per
+// item 64 x its share of the tile's list; one thread per item, a block sum,
+// one atomic per block.
+static __global__ __launch_bounds__(BLOCK) void k_list_swept(const int2 *__restrict__ counts,
+                                                            int ntiles, int M,
+                                                            unsigned long long *__restrict__ swept)
+{
+    const int item = blockIdx.x * BLOCK + threadIdx.x;
+    unsigned long long v = 0;
+    if (item < ntiles * M) {
+        const int t = item / M, m = item % M;
+        const int2 cnt = counts[t];
+        int lo = 0, hi = 0;
+        list_share(cnt.x + cnt.y, M, m, lo, hi);
+        v = (unsigned long long)(hi - lo) * TILE;
+    }
+    __shared__ unsigned long long red[BLOCK];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0] > 0) atomicAdd(swept, red[0]);
 }
 
 static __global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
@@ -1305,7 +1329,11 @@ __device__ __forceinline__ void group_centre(const double4 &pi, bool valid, doub
 
 // Pair sums over the cluster lists.  A wave takes four clusters (order[4s ..
 // 4s+3]), 16 lanes each, one target per lane, and item m of M: an equal share
-// of every cluster's list (clist_share).
+// of every cluster's list (clist_share).  The kernel issues no atomics: on
+// this chip an atomic on one address from every CU is served one at a time
+// beyond the XCDs' own L2s (about 25 ns each), so a per-wave counter or work
+// queue costs as much as a short wave's pairs.  The pairs evaluated are
+// counted by k_clist_swept, from the same list counts, when asked for.
 template <int KIND, int PREC, bool MEAS>
 __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict__ pos,
                                                        const unsigned *__restrict__ clist,
@@ -1314,8 +1342,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
                                                        ForceConsts c, int segcap, int ncl, int M,
                                                        double *__restrict__ part,
                                                        unsigned long long *__restrict__ maxdisp,
-                                                       double skin,
-                                                       unsigned long long *__restrict__ swept)
+                                                       double skin)
 {
     __shared__ double4 sj[BLOCK / TILE][TILE];
     __shared__ float4 sfj[BLOCK / TILE][(PREC == 1 && KIND != NN) ? TILE : 1];
@@ -1358,8 +1385,8 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
         sc[sg] = (cl >= 0) ? ccounts[cl * NSEG + sg] : make_int2(0, 0);
         total += sc[sg].x + sc[sg].y;
     }
-    const int per = (total + M - 1) / M;
-    const int lo = min(m * per, total), hi = min(lo + per, total);
+    int lo = 0, hi = 0;
+    list_share(total, M, m, lo, hi);
     int iters = hi - lo;
 #pragma unroll
     for (int off = CL; off < TILE; off <<= 1) {
@@ -1371,7 +1398,6 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
     const SegList sl = seg_list(mine, segcap, sc, hi);
     Acc<MEAS> acc;
     clist_share<KIND, PREC, MEAS>(c, pi, cc, sl, lo, iters, pos, sj[wv], sfj[wv], lane, acc);
-    if (lane == 0 && iters > 0) atomicAdd(swept, (unsigned long long)iters * TILE);
     if (cl < 0) return;
     constexpr int NC = MEAS ? 11 : 3;
     double *o = part + ((size_t)cl * M + m) * NC * CL + gl;
@@ -1381,6 +1407,45 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict
         o[5 * CL] = acc.pxx; o[6 * CL] = acc.pxy; o[7 * CL] = acc.pxz;
         o[8 * CL] = acc.pyy; o[9 * CL] = acc.pyz; o[10 * CL] = acc.pzz;
     }
+}
+
+// Directed pairs a k_pairs_clist launch over the current lists evaluates:
+// per item 64 x its longest group share (the lanes a wave runs, masked or
+// not), one thread per item, a block sum, one atomic per block.
+static __global__ __launch_bounds__(BLOCK) void k_clist_swept(const int2 *__restrict__ ccounts,
+                                                             const int *__restrict__ order,
+                                                             int ncl, int M,
+                                                             unsigned long long *__restrict__ swept)
+{
+    const int nsets = (ncl + CPW - 1) / CPW;
+    const int item = blockIdx.x * BLOCK + threadIdx.x;
+    unsigned long long v = 0;
+    if (item < nsets * M) {
+        const int m = item / nsets, set = item % nsets;
+        int iters = 0;
+        for (int g = 0; g < CPW; ++g) {
+            const int ci = set * CPW + g;
+            if (ci >= ncl) continue;
+            const int cl = order[ci];
+            int total = 0;
+            for (int sg = 0; sg < NSEG; ++sg) {
+                const int2 q = ccounts[cl * NSEG + sg];
+                total += q.x + q.y;
+            }
+            int lo = 0, hi = 0;
+            list_share(total, M, m, lo, hi);
+            iters = hi - lo > iters ? hi - lo : iters;
+        }
+        v = (unsigned long long)iters * TILE;
+    }
+    __shared__ unsigned long long red[BLOCK];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0] > 0) atomicAdd(swept, red[0]);
 }
 
 // ---------------------------------------------------------------------------

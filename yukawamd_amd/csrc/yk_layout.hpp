@@ -26,6 +26,15 @@ constexpr int NSEG = 4;
 // bits 26-31.
 constexpr unsigned SLOT_MASK = (1u << 26) - 1u;
 
+// A work item's share m of M of one cluster's list of `total` entries:
+// [lo, hi).  constexpr, so device and host code share it.
+constexpr void list_share(int total, int M, int m, int &lo, int &hi)
+{
+    const int per = (total + M - 1) / M;
+    lo = m * per < total ? m * per : total;
+    hi = lo + per < total ? lo + per : total;
+}
+
 // A cluster's list as the pair kernels walk it: entry v of [0, hi), the back
 // (general) entries of the four segments first, then the fast ones.  Segment
 // sg occupies base[sg * segcap .. (sg + 1) * segcap): fast entries from the

@@ -64,6 +64,7 @@ int fail(int code, const char *fmt, ...)
 #endif
 constexpr int LIST_ITEMS_FP64 = YK_LIST_ITEMS_FP64;
 constexpr int LIST_ITEMS_MIXED = YK_LIST_ITEMS_MIXED;
+constexpr int CLIST_SMALL_WAVES = 8192;   // waves below which a cluster-list launch gets more items
 
 struct yk_system {
     yk_params p{};
@@ -420,7 +421,9 @@ int timing_collect(yk_system *s)
 }
 
 // Directed pairs evaluated by the pair kernels since the last reset
-// (lanes of a swept tile or list batch count, masked or not).
+// (lanes of a swept tile or list batch count, masked or not): the list
+// launches are counted under YK_FLAG_TIMING (k_clist_swept, k_list_swept),
+// the all-pairs kernel counts its own culled tiles.
 int read_swept(yk_system *s, unsigned long long *out)
 {
     YK_HIP(hipStreamSynchronize(s->stream));
@@ -438,11 +441,11 @@ void launch_pairs_k(yk_system *s, const ForceConsts &c, bool meas)
         if (meas)
             hipLaunchKernelGGL((k_pairs_clist<KIND, PREC, true>), grid, dim3(BLOCK), 0,
                                s->stream, s->pos, s->clist, s->ccounts, s->corder, c,
-                               s->segcap, s->ncl, s->list_M, s->part, s->maxdisp, lc.skin, s->swept);
+                               s->segcap, s->ncl, s->list_M, s->part, s->maxdisp, lc.skin);
         else
             hipLaunchKernelGGL((k_pairs_clist<KIND, PREC, false>), grid, dim3(BLOCK), 0,
                                s->stream, s->pos, s->clist, s->ccounts, s->corder, c,
-                               s->segcap, s->ncl, s->list_M, s->part, s->maxdisp, lc.skin, s->swept);
+                               s->segcap, s->ncl, s->list_M, s->part, s->maxdisp, lc.skin);
         return;
     }
     if (s->use_list) {
@@ -452,11 +455,11 @@ void launch_pairs_k(yk_system *s, const ForceConsts &c, bool meas)
         if (meas)
             hipLaunchKernelGGL((k_pairs_list<KIND, PREC, true>), grid, dim3(BLOCK), 0, s->stream,
                                s->pos, s->list, s->counts, c, s->n, s->list_M, s->part,
-                               s->maxdisp, lc.skin, s->swept);
+                               s->maxdisp, lc.skin);
         else
             hipLaunchKernelGGL((k_pairs_list<KIND, PREC, false>), grid, dim3(BLOCK), 0,
                                s->stream, s->pos, s->list, s->counts, c, s->n, s->list_M,
-                               s->part, s->maxdisp, lc.skin, s->swept);
+                               s->part, s->maxdisp, lc.skin);
         return;
     }
     dim3 grid(s->nblocks, s->nchunks);
@@ -520,7 +523,21 @@ int launch_pairs(yk_system *s, bool meas)
     default: return fail(YK_ERR_UNSUPPORTED, "sim_type %d", s->p.sim_type);
     }
     YK_HIP(hipGetLastError());
-    if (stop) YK_HIP(hipEventRecord(stop, s->stream));
+    if (stop) {
+        YK_HIP(hipEventRecord(stop, s->stream));
+        // the pairs the list launch evaluated, counted from its lists outside
+        // the timed launch (the all-pairs kernel counts its own culled tiles)
+        if (s->use_clist) {
+            const int items = (s->ncl + CPW - 1) / CPW * s->list_M;
+            hipLaunchKernelGGL(k_clist_swept, dim3((items + BLOCK - 1) / BLOCK), dim3(BLOCK), 0,
+                               s->stream, s->ccounts, s->corder, s->ncl, s->list_M, s->swept);
+        } else if (s->use_list) {
+            const int items = s->ntiles * s->list_M;
+            hipLaunchKernelGGL(k_list_swept, dim3((items + BLOCK - 1) / BLOCK), dim3(BLOCK), 0,
+                               s->stream, s->counts, s->ntiles, s->list_M, s->swept);
+        }
+        YK_HIP(hipGetLastError());
+    }
     if (s->ev_used >= 1024) return timing_collect(s);
     return YK_OK;
 }
@@ -1035,6 +1052,13 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
     s->use_list = !(params->flags & YK_FLAG_ALLPAIRS);
     // items per target tile (cluster set)
     s->list_M = (params->precision == YK_MIXED) ? LIST_ITEMS_MIXED : LIST_ITEMS_FP64;
+    if (s->use_list && !(params->flags & YK_FLAG_TILE_LISTS) && n <= CLIST_MAX_N) {
+        // small systems: more items per cluster set, up to 128, until the
+        // launch has 8,192 waves (N = 3,456: 128 items, 6,912 waves, 30 us
+        // per force launch against 33 us with 32 items)
+        const int nsets = ((n + CL - 1) / CL + CPW - 1) / CPW;
+        while (s->list_M < 128 && nsets * s->list_M < CLIST_SMALL_WAVES) s->list_M *= 2;
+    }
     s->skin = 0.05 * list_cutoff(*params);
     if (!(s->skin < 0.05 * params->xl[0])) s->skin = 0.0;   // huge cutoff: every pair listed
     set_near_radius(s);
