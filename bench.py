@@ -58,23 +58,25 @@ FM_C_SECONDS = 3.33564e-24
 # from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scale factors from a plain
 # 256 MiB copy); counters cannot be read inside a timed run, so the figure is
 # the profiled one, with its commands, in the file named here.
-TRAFFIC_BYTES_PER_LAUNCH = 1.031e8
+TRAFFIC_BYTES_PER_LAUNCH = 1.026e8
 TRAFFIC_SOURCE = "profiles/hbm_traffic_k_pairs_clist_ion_mixture_27648.md"
 # VALU issue of the pair kernels at N = 27,648 from rocprofv3 counters:
 # SQ_INSTS_VALU x 4 cycles (a wave64 VALU instruction on a 16-lane SIMD, FP64 or
 # unpacked FP32) / (GRBM_GUI_ACTIVE x 1,024 SIMDs); read from the profile named
-# here, since counters cannot be collected inside a timed run
+# here, since counters cannot be collected inside a timed run.  The mixed
+# kernel's figure leaves out its transcendental instructions, which run beside
+# the other VALU work (with them the count exceeds the budget: 1.04)
 VALU_ISSUE_FRAC_FP64 = 0.82          # of which FP64 arithmetic 0.62
 VALU_ISSUE_FRAC_FP64_ARITH = 0.62
-VALU_ISSUE_FRAC_MIXED = 0.83
+VALU_ISSUE_FRAC_MIXED = 0.97
 COUNTER_SOURCE = "profiles/valu_counters_k_pairs_clist_ion_mixture_27648.md"
 # bytes per particle and average durations of the O(N) step kernels at
 # N = 27,648 (rocprofv3 kernel trace of this benchmark), and their HBM rate
 INTEGRATORS = {
     "kernel": "k_finalize<MIX,force-only,vel_verlet_2,cluster partials,+next step's drift>",
-    "bytes_per_particle_per_step": 997, "avg_us": 12.1, "gb_per_s": 2276,
-    "hbm_frac": 2276 / 8000,
-    "drift_alone": {"bytes_per_particle": 180, "avg_us": 8.9, "gb_per_s": 561},
+    "bytes_per_particle_per_step": 997, "avg_us": 12.2, "gb_per_s": 2260,
+    "hbm_frac": 2260 / 8000,
+    "drift_alone": {"bytes_per_particle": 180, "avg_us": 9.1, "gb_per_s": 549},
     "source": TRAFFIC_SOURCE,
 }
 
