@@ -187,3 +187,59 @@ def test_driver_nucleus_run_matches_reference(tmp_path):
     dx = np.minimum(dx, L - dx)
     assert dx.max() <= 1e-9 * L.max()
     np.testing.assert_allclose(gfin["v"], rfin["v"], rtol=1e-7, atol=1e-10 * np.abs(rfin["v"]).max())
+
+
+FORMATTED_CASES = {
+    "xvfa": " nout=6, ftype='xvfa',",
+    "xvfb": " nout=6, ftype='xvfb', append=.true.,",
+    "xZAfb": " nout=12, ftype='xZAfb', append=.false.,",
+}
+
+
+def formatted_configs(path, ftype, n):
+    """Configurations of a formatted output file: (header numbers, rows)."""
+    lines = path.read_text().splitlines()
+    out = []
+    k = 0
+    while k < len(lines):
+        if ftype == "xvfa":   # the time (f15.3), then n rows
+            head = [float(lines[k])]
+            k += 1
+        else:                 # 17 '#' lines: type, code, date, sim_type, then numbers
+            block = lines[k:k + 17]
+            assert all(ln.startswith("#") for ln in block), block[:3]
+            head = [float(t) for ln in block[4:] for t in ln[1:].split()]
+            head.append(block[0].split()[1] == ftype)
+            k += 17
+        rows = np.array([[float(t) for t in ln.split()] for ln in lines[k:k + n]])
+        k += n
+        out.append((head, rows))
+    return out
+
+
+@pytest.mark.parametrize("case", sorted(FORMATTED_CASES))
+def test_driver_formatted_outputs_match_reference(tmp_path, case):
+    """The formatted save types (src/md_utils.f:83-104) go to the file named by
+    start -- here 'random' -- as the reference writes them: revision a
+    appended, revision b appended or overwritten; every configuration's
+    header and rows against the reference's."""
+    runmd = RUNMD.replace("iseed=2024,", "iseed=2024,\n" + FORMATTED_CASES[case])
+    run([str(MD_OMP)], tmp_path / "ref", runmd=runmd)
+    run([str(YKMD)], tmp_path / "gpu", runmd=runmd)
+    n = 1500
+    cr = formatted_configs(tmp_path / "ref" / "random", case, n)
+    cg = formatted_configs(tmp_path / "gpu" / "random", case, n)
+    assert len(cg) == len(cr) == {"xvfa": 4, "xvfb": 4, "xZAfb": 1}[case]
+    for (hg, rg), (hr, rr) in zip(cg, cr):
+        np.testing.assert_allclose(np.array(hg, float), np.array(hr, float), rtol=2e-8, atol=0)
+        assert rg.shape == rr.shape
+        # positions to rounding (across the periodic box where the header
+        # holds it: a particle within rounding of a face may sit on either
+        # side); then the rest
+        dx = np.abs(rg[:, :3] - rr[:, :3])
+        if case != "xvfa":
+            L = np.array(hr[4:7])   # xl0 (3), time, xl (3), ...
+            dx = np.minimum(dx, np.abs(L - dx))
+        assert dx.max() <= 1e-9 * np.abs(rr[:, :3]).max()
+        np.testing.assert_allclose(rg[:, 3:], rr[:, 3:], rtol=1e-7,
+                                   atol=1e-10 * np.abs(rr[:, 3:]).max())

@@ -57,7 +57,8 @@ YKMD = FORTRAN / "ykmd"
 
 def build_fortran() -> Path:
     """Compile the iso_c_binding module (yk_bind.mod / yk_bind.o) in-tree, and
-    the run driver yukawamd_amd/fortran/ykmd (yk_host.f90 + ykmd.f90, linked
+    the run driver yukawamd_amd/fortran/ykmd (yk_host.f90, yk_config.f90 and
+    ykmd.f90, linked
     against libyukawa_hip.so; unformatted files big-endian like the
     reference's build)."""
     out = FORTRAN / "_build"
@@ -65,9 +66,11 @@ def build_fortran() -> Path:
     flags = ["-O2", "-fPIC", "-fconvert=big-endian", "-module-dir", out]
     _run([FLANG, *flags, "-c", FORTRAN / "yk_bind.f90", "-o", out / "yk_bind.o"])
     _run([FLANG, *flags, "-c", FORTRAN / "yk_host.f90", "-o", out / "yk_host.o"])
+    _run([FLANG, *flags, "-c", FORTRAN / "yk_config.f90", "-o", out / "yk_config.o"])
     _run([FLANG, *flags, "-c", FORTRAN / "ykmd.f90", "-o", out / "ykmd.o"])
     tmp = YKMD.with_suffix(".tmp")
-    _run([FLANG, "-o", tmp, out / "ykmd.o", out / "yk_host.o", out / "yk_bind.o",
+    _run([FLANG, "-o", tmp, out / "ykmd.o", out / "yk_config.o", out / "yk_host.o",
+          out / "yk_bind.o",
           "-L", PKG, "-lyukawa_hip", "-Wl,-rpath,$ORIGIN/.."])
     os.replace(tmp, YKMD)
     return out / "yk_bind.o"

@@ -5,9 +5,12 @@
 ! Reads the reference's run-parameter file (runmd.in, namelist
 ! run_parameters with the names of src/input1.F:28-53, so the same file
 ! serves both programs), derives the run parameters as src/input1.F and
-! src/initconfig.f do, builds the initial state of a start='random' or
-! start='nuclear' run with the reference's generator (irnd = 0 or 4: MT19937;
-! yk_host.f90) or reads an 'xv8b' start file, and runs the schedule of src/md.F:66-205 on the GPU:
+! src/initconfig.f do, takes the charges and masses from the spec_list, a
+! species file ('ZAfa', 'ZAfb') or the start file, builds the initial state of
+! a start='random' or start='nuclear' run with the reference's generator
+! (irnd = 0 or 4: MT19937; yk_host.f90) or reads a start file of any type the
+! reference reads (yk_config.f90), generating Maxwell velocities when it has
+! none, and runs the schedule of src/md.F:66-205 on the GPU:
 ! initial accel and ttot, nwgroup warm-up groups (nwsteps steps, then
 ! cancel_vcom and tnorm), ngroup measurement groups of ntot*nind steps with
 ! the ncom / ntnorm / strain / nind / nptensor cadences (one yk_run_md call
@@ -19,7 +22,9 @@
 ! (md.<tstart>.xv8b, src/initconfig.f:385-390), the configurations saved
 ! every nout steps of the measurement phase (save_config, src/md_utils.f:61-114:
 ! the binary types of src/rw_xvb.f and src/rw_xva.f, one file per
-! configuration or appended to md.traj.<tend>.<ftype>) and the checkpoints
+! configuration or appended to md.traj.<tend>.<ftype>; the formatted types of
+! src/rw_xvfa.f and src/rw_xvfb.f into the file named by start, as the
+! reference writes them) and the checkpoints
 ! (md.<time>.xv8b, src/md_utils.f:19-38) every nckpt steps, after the warm-up
 ! and at the end of the run (src/md.F:93, :104, :165, :176).  A run that saves
 ! or checkpoints is split into device runs between those steps; their headers
@@ -37,6 +42,7 @@ program ykmd
   use iso_c_binding
   use yk_bind
   use yk_host
+  use yk_config
   implicit none
 
   ! run parameters (names and defaults of src/md_globals.f:36-135)
@@ -73,6 +79,9 @@ program ykmd
   real(dp) :: hev, hek, hpx, hpp(3, 3)   ! ev, ek, px, pp of src/md_globals.f now
   character(40) :: suffix
   character(20) :: sim_type_in
+  character(20) :: xtype
+  character(6) :: spec_type
+  logical :: spec_read, za_in_start, v_generated
   character(256) :: env, arg
   type(yk_params_t) :: p
   type(yk_meas_t) :: m
@@ -92,64 +101,101 @@ program ykmd
   sim_type_in = sim_type
   if (nout /= 0) then
     select case (trim(ftype))
-    case ('x4b', 'xv4b', 'x8b', 'xv8b', 'xva8b', 'md0', 'x4a', 'xv4a', 'x8a', 'xv8a', 'xva8a')
+    case ('onp', 'md1', 'xfa', 'xvfa', 'xvZAfa', 'xfb', 'xvfb', 'xZAfb', 'xvZAfb', &
+          'md0', 'x4a', 'xv4a', 'x8a', 'xv8a', 'xva8a', 'x4b', 'xv4b', 'x8b', 'xv8b', 'xva8b')
     case default
-      call fail('ftype not supported (binary revision-a and -b types only): '//trim(ftype))
+      call fail('ftype not supported: '//trim(ftype))
     end select
   end if
 
-  ! species from the internal list, in list order (src/initconfig.f:117-141)
-  if (trim(spec_file) /= '' .and. trim(spec_file) /= 'internal') &
-    call fail('only an internal spec_list is supported (spec_file)')
+  ! species (src/input1.F:128-157): a spec_list in runmd.in wins over a
+  ! spec_file; its species are counted here, in list order
   n = 0
   nspec = 0
-  do k = 1, MAXSPEC
-    if (spec_list(k)%n == 0 .and. spec_list(k)%z == 0.0d0 .and. spec_list(k)%a == 0.0d0) exit
-    n = n + spec_list(k)%n
-    nspec = k
-  end do
-  if (n <= 0) call fail('no particles in spec_list')
-  allocate(zii(0:n - 1), aii(0:n - 1), a(3, 0:n - 1))
-  a = 0.0d0
   ztot = 0.0d0
-  j = 0
-  do k = 1, nspec
-    ztot = ztot + spec_list(k)%n * spec_list(k)%z
-    do i = 1, spec_list(k)%n
-      zii(j) = spec_list(k)%z
-      aii(j) = spec_list(k)%a
-      j = j + 1
+  if (maxval(spec_list(:)%n) > 0) then
+    spec_file = 'internal'
+    do k = 1, MAXSPEC
+      if (spec_list(k)%n == 0 .and. spec_list(k)%z == 0.0d0 .and. spec_list(k)%a == 0.0d0) exit
+      if (spec_list(k)%n > 0) n = n + spec_list(k)%n
+      nspec = k
     end do
-  end do
+  end if
 
   ! random-number seed (src/input1.F:174-177): MT19937 only
   if (irnd /= 0 .and. irnd /= 4) &
     call fail('only irnd = 0 or 4 (Mersenne Twister) is supported')
   call mt_seed(iseed)
 
-  ! initial configuration (src/initconfig.f:202-226, :331-373)
+  ! charges and masses (src/initconfig.f:104-163): from the list, from a
+  ! species file, or later from a start file that holds them
+  spec_read = .false.
+  spec_type = ''
+  if (trim(spec_file) == 'internal') then
+    allocate(zii(0:n - 1), aii(0:n - 1))
+    j = 0
+    do k = 1, nspec
+      ztot = ztot + spec_list(k)%n * spec_list(k)%z
+      do i = 1, spec_list(k)%n
+        zii(j) = spec_list(k)%z
+        aii(j) = spec_list(k)%a
+        j = j + 1
+      end do
+    end do
+  else if (trim(spec_file) /= '') then
+    spec_type = file_type(spec_file)
+    select case (trim(spec_type))
+    case ('ZAfa', 'ZAfb')
+      call read_species(spec_file, spec_type, n, zii, aii, ztot, irtn)
+      if (irtn < 0) call fail('cannot read spec_file '//trim(spec_file))
+      spec_read = .true.
+    end select
+  end if
+
+  ! initial configuration (src/initconfig.f:172-345)
   select case (trim(start))
   case ('random', 'nuclear')
-    if (tstart < 0.0d0) call fail('tstart not specified')
+    xtype = trim(start)
+  case default
+    xtype = file_type(start)
+  end select
+  za_in_start = .false.
+  pp = 0.0d0
+  ev = 0.0d0
+  ek = 0.0d0
+  px = 0.0d0
+  time = 0.0d0
+  irtn = 0
+  select case (trim(xtype))
+  case ('random', 'nuclear')
+    if (n <= 0 .or. .not. allocated(zii)) call fail('no particles: give a spec_list or spec_file')
     call random_box(n, rho, aspect, xlf, xl0)
     xl = xl0
-    allocate(x(3, 0:n - 1), v(3, 0:n - 1))
-    if (trim(start) == 'random') then
+    if (tstart < 0.0d0) call fail('tstart not specified')
+    allocate(x(3, 0:n - 1))
+    if (trim(xtype) == 'random') then
       call random_positions(n, xl, x)
     else   ! a sphere about the box centre (src/initconfig.f:227-253)
       rmax = min(rmax, min(xl(1), xl(2), xl(3)) * 0.5)
       call sphere_positions(n, xl, rmax, x)
     end if
-    call maxwell_velocities(n, aii, kT, xmass, v)
-    call host_cancel_vcom(n, v, aii, xmass, kT)
-    time = tstart
-  case default
-    call read_xvb(trim(start), time, xl, k, x, v, irtn)
-    if (irtn /= 0) call fail('cannot read start file (xv8b expected): '//trim(start))
-    if (k /= n) call fail('start file holds another particle count than spec_list')
+    irtn = GOT_X
+  case ('onp', 'md1', 'xfa', 'xvfa', 'md0', 'x4a', 'xv4a', 'x8a', 'xv8a', 'xvZAfa')
+    ! revision a (no box in the file): the box follows n, rho and the aspect
+    call read_config(start, xtype, n, time, xl, xl0, ev, ek, px, pp, x, v, zii, aii, ztot, irtn)
+    if (irtn < 0) call fail('cannot read start file '//trim(start))
+    za_in_start = trim(xtype) == 'xvZAfa'
     if (tstart < 0.0d0) tstart = time
-    time = tstart
-    ! the density and the gage box follow the file's box (src/initconfig.f:299-308)
+    if (aspect(1) * aspect(2) * aspect(3) == 0.0d0) aspect = 1.0d0
+    xlf = (real(real(n), dp) / (aspect(1) * aspect(2) * aspect(3) * rho))**(1.0d0 / 3.0d0)
+    xl0 = aspect * xlf
+    xl = stretch * xl0
+  case ('x4b', 'x8b', 'xv4b', 'xv8b', 'xfb', 'xvfb', 'xZAfb', 'xvZAfb')
+    ! revision b: the box, time and n from the file; density and gage box follow
+    call read_config(start, xtype, n, time, xl, xl0, ev, ek, px, pp, x, v, zii, aii, ztot, irtn)
+    if (irtn < 0) call fail('cannot read start file '//trim(start))
+    za_in_start = trim(xtype) == 'xZAfb' .or. trim(xtype) == 'xvZAfb'
+    if (tstart < 0.0d0) tstart = time
     rho = real(real(n), dp) / (xl(1) * xl(2) * xl(3))
     if (aspect(1) * aspect(2) * aspect(3) == 0.0d0) then
       xlf = minval(xl)
@@ -160,12 +206,32 @@ program ykmd
       xl0 = aspect * xlf
     end if
     stretch = xl / xl0
+  case default
+    ! also 'xva8a', whose reference reader takes the particles from the time
+    ! record and stops there
+    call fail('input configuration type not supported: '//trim(start))
   end select
-  if (trim(start) == 'random' .or. trim(start) == 'nuclear') then
-    pp = 0.0d0
+  if (.not. allocated(zii) .or. .not. allocated(aii)) &
+    call fail('no charges and masses: give a spec_list, a spec_file or a start file with them')
+  if (size(zii) /= n) call fail('the start file holds another particle count than the species')
+  allocate(a(3, 0:n - 1))
+  a = 0.0d0
+  ! velocities from a Maxwell distribution when the start has none
+  ! (src/initconfig.f:357-373)
+  v_generated = iand(irtn, GOT_V) == 0
+  if (v_generated) then
+    if (allocated(v)) deallocate(v)
+    allocate(v(3, 0:n - 1))
+    call maxwell_velocities(n, aii, kT, xmass, v)
+    call host_cancel_vcom(n, v, aii, xmass, kT)
+  end if
+  time = tstart
+  ! the initial configuration, unless it came from an 'xv8b' file
+  ! (src/initconfig.f:385-390): the header of the globals at this point
+  if (trim(xtype) /= 'xv8b') then
     write(suffix, '(i5.5,i6.6)') int(time / 1000000.d0), int(mod(time, 1000000.d0))
-    call write_xvb('md.'//trim(suffix)//'.xv8b', 'xv8b', sim_type, time, xl, 0.0d0, &
-                   0.0d0, 0.0d0, pp, n, x, v, a)
+    call write_xvb('md.'//trim(suffix)//'.xv8b', 'xv8b', sim_type, time, xl, ev, &
+                   ek, px, pp, n, x, v, a)
   end if
 
   ! derived parameters (src/input1.F:184-196, :222, :255-311)
@@ -409,25 +475,32 @@ contains
     write(u, '(''  Number of processes:  '',i6)') 1
     write(u, '(a)') '  ----------------------------------------------------------'
     write(u, '(/,''  SIMULATION TYPE:  '',a /''  ------------------------'')') trim(sim_type_in)
-    write(u, '(a)') '  Species  #particles    Z        A'
-    do ks = 1, nspec
-      if (spec_list(ks)%n > 0) write(u, '(3x,i4,4x,i9,2x,f7.2,2x,f7.2)') ks, spec_list(ks)%n, &
-                                     spec_list(ks)%z, spec_list(ks)%a
-    end do
-    write(u, '(''   Total:  '',i9)') n
+    if (trim(spec_file) == 'internal') then
+      write(u, '(a)') '  Species  #particles    Z        A'
+      do ks = 1, nspec
+        if (spec_list(ks)%n > 0) write(u, '(3x,i4,4x,i9,2x,f7.2,2x,f7.2)') ks, spec_list(ks)%n, &
+                                       spec_list(ks)%z, spec_list(ks)%a
+      end do
+      write(u, '(''   Total:  '',i9)') n
+    end if
     if (irnd > 0) write(u, '(/''  irnd  = '',i9,2x,''('',a,'')''/''  iseed = '',i9)') &
       irnd, trim(rngname(irnd)), iseed
-    write(u, '(/,''  INITIAL CONFIGURATION'',/''  ---------------------'')')
-    if (trim(start) == 'random' .or. trim(start) == 'nuclear') then
-      if (trim(start) == 'random') then
-        write(u, '(a)') '  Generating random coordinates in box'
-      else
-        write(u, '(''  Generating random coordinates in sphere''/''  radius     = '',f8.4,'' fm'')') rmax
-      end if
-      write(u, '(a)') '  Generating velocities from Maxwell distribution'
-    else
-      write(u, '(''  file          = '',a)') trim(start)
+    if (spec_read) then   ! src/initconfig.f:143-161
+      write(u, '(''  spec file:  '',a)') trim(spec_file)
+      write(u, '(''     n = '',i9)') n
     end if
+    write(u, '(/,''  INITIAL CONFIGURATION'',/''  ---------------------'')')
+    select case (trim(xtype))
+    case ('random')
+      write(u, '(a)') '  Generating random coordinates in box'
+    case ('nuclear')
+      write(u, '(''  Generating random coordinates in sphere''/''  radius     = '',f8.4,'' fm'')') rmax
+    case default
+      write(u, '(''  file          = '',a)') trim(start)
+      if (za_in_start) write(u, '(''  This file also includes charges and masses.'', / &
+                                 &''     n          = '',i10)') n
+    end select
+    if (v_generated) write(u, '(a)') '  Generating velocities from Maxwell distribution' 
     write(u, '(''  start time    =  '',f14.2,'' fm/c''/''  time step     =        '',f8.2,'' fm/c'')') &
       tstart, dt
     write(u, '(/,''  rho               ='',es16.8,'' fm-3''/''  L0                ='',2x,f17.9, &
@@ -508,12 +581,21 @@ contains
       fname = 'md.'//trim(sfx)//'.'//trim(ftype)
     end if
     select case (trim(ftype))
-    case ('x4b', 'xv4b', 'x8b', 'xv8b', 'xva8b')
+    case ('x4b', 'xv4b', 'x8b', 'xv8b')
       call write_xvb(trim(fname), ftype, sim_type, time, xl, hev, hek, hpx, hpp, n, x, v, a, &
                      append)
-    case default
+    case ('md0', 'x4a', 'xv4a', 'x8a', 'xv8a', 'xva8a')
       call write_xva(trim(fname), ftype, time, n, x, v, a, append)
-    end select
+    ! the formatted types go to the file named by start, as the reference
+    ! has it (src/md_utils.f:83-104); the revision-a ones always appended
+    case ('onp', 'md1', 'xfa', 'xvfa')
+      call write_config_fa(start, ftype, .true., time, n, x, v, zii, aii)
+    case ('xvZAfa')
+      call write_config_fa(start, ftype, append, time, n, x, v, zii, aii)
+    case ('xfb', 'xvfb', 'xZAfb', 'xvZAfb')
+      call write_config_fb(start, ftype, append, sim_type, time, xl0, xl, hev, hek, hpx, hpp, &
+                           n, x, v, zii, aii)
+    end select   ! 'xva8b': nothing (no case of src/md_utils.f:82-105)
   end subroutine save_config
 
   subroutine check(rc, what)
