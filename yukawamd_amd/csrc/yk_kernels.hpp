@@ -1335,7 +1335,8 @@ __device__ __forceinline__ void group_centre(const double4 &pi, bool valid, doub
 // queue costs as much as a short wave's pairs.  The pairs evaluated are
 // counted by k_clist_swept, from the same list counts, when asked for.
 template <int KIND, int PREC, bool MEAS>
-__global__ __launch_bounds__(BLOCK) void k_pairs_clist(const double4 *__restrict__ pos,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PREC == 1 && !MEAS ? 6 : 1)))
+void k_pairs_clist(const double4 *__restrict__ pos,
                                                        const unsigned *__restrict__ clist,
                                                        const int2 *__restrict__ ccounts,
                                                        const int *__restrict__ order,
