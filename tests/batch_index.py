@@ -93,6 +93,9 @@ def load():
     lib.ykc_check.argtypes = [C.POINTER(EngineCfg), C.POINTER(Replica), C.c_int32,
                               C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.POINTER(Report)]
     lib.ykc_clear_cache.restype = None
+    lib.ykc_cursor_walk.restype = C.c_int64
+    lib.ykc_cursor_walk.argtypes = [C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int32,
+                                    C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
     _lib = lib
     return lib
 
@@ -150,6 +153,17 @@ def sweep_ops(nrep, warmup=3, steps=8):
     return [[ADD, k] for k in range(nrep)] + [[RUN, warmup], [RUN, steps]]
 
 
+def cursor_walk(counts, segcap, lo, hi, iters, stride=16, plant=0):
+    """Replays the pair kernels' cursor walk over one cluster's list
+    (ykc_cursor_walk): returns (mismatches, visits per entry of [0, hi))."""
+    lib = load()
+    c = (C.c_int32 * 8)(*[int(v) for v in np.asarray(counts).reshape(-1)])
+    visits = np.zeros(max(hi, 1), dtype=np.int32)
+    bad = lib.ykc_cursor_walk(c, segcap, lo, hi, iters, stride, plant,
+                              visits.ctypes.data_as(C.POINTER(C.c_int32)))
+    return int(bad), visits[:hi]
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--slots", type=int, default=1 << 20,
@@ -182,3 +196,4 @@ def main(argv=None):
 
 if __name__ == "__main__":
     raise SystemExit(main())
+

@@ -669,6 +669,16 @@ void batch_list_item(Engine &e, Checker &ck, const plan::Item &it, const Sizes &
         const int vend = std::min(sl[g].hi, lo[g] + iters + 2 * CL);
         if (e.exhaustive) {
             for (int v = lo[g]; v < vend; ++v) read(v);
+            // the kernel reaches these entries through a cursor per lane
+            for (int gl = 0; gl < CL; ++gl) {
+                SegList::Cursor cu = sl[g].cursor(lo[g] + gl, CL);
+                for (int v = lo[g] + gl; v < vend; v += CL) {
+                    bool back = false;
+                    const long long want = sl[g].offset(v, back);
+                    ck.in("list cursor offset", (long long)cu.off - want, 0, 1);
+                    sl[g].advance(cu, v + CL, CL);
+                }
+            }
         } else if (lo[g] < vend) {
             const SegList &q = sl[g];
             const int cuts[] = {q.g1, q.g2, q.g3, q.nback, q.nback + q.f1, q.nback + q.f2,
@@ -865,6 +875,53 @@ void resort_due(Engine &e, Checker &ck, std::vector<std::vector<int>> &order)
 }  // namespace
 
 extern "C" void ykc_clear_cache(void) { geometry_cache().clear(); }
+
+// The pair kernels' list walk (clist_share): lane gl of a 16-lane group reads
+// entries lo + gl, lo + gl + stride, ... through a SegList::Cursor, first at
+// lo + gl, then advanced by stride per batch, for as many batches as the
+// wave runs (iters rounded up, plus the one-batch prefetch).  counts: the
+// cluster's segments as (fast, back) pairs.  Every entry below hi is compared
+// with SegList::offset and checked to lie in its segment's written part;
+// visits: how often each v of [0, hi) is read (each v of [lo, hi) is read
+// once).  plant 1: the cursor never recomputes at a run's end; plant 2: it
+// steps fast runs the wrong way.  Returns the number of mismatches.
+extern "C" int64_t ykc_cursor_walk(const int32_t *counts, int32_t segcap, int32_t lo,
+                                   int32_t hi, int32_t iters, int32_t stride, int32_t plant,
+                                   int32_t *visits)
+{
+    Count2 sc[NSEG];
+    for (int k = 0; k < NSEG; ++k) sc[k] = Count2{counts[2 * k], counts[2 * k + 1]};
+    const SegList sl = seg_list<Count2>(nullptr, segcap, sc, hi);
+    int64_t bad = 0;
+    for (int gl = 0; gl < stride; ++gl) {
+        int v = lo + gl;
+        SegList::Cursor cu = sl.cursor(v, stride);
+        for (int b0 = -stride; b0 < iters; b0 += stride, v += stride) {
+            if (v < hi) {
+                bool back = false;
+                const long long want = sl.offset(v, back);
+                if ((long long)cu.off != want || (cu.step < 0) != back) ++bad;
+                const int sg = (int)(cu.off / (unsigned)segcap);
+                const int within = (int)(cu.off % (unsigned)segcap);
+                if (sg < 0 || sg >= NSEG) {
+                    ++bad;
+                } else if (cu.step < 0 ? within < segcap - sc[sg].y : within >= sc[sg].x) {
+                    ++bad;
+                }
+                ++visits[v];
+            }
+            const int vn = v + stride;
+            if (plant == 1) {
+                cu.off += (unsigned)cu.step;
+            } else if (plant == 2 && cu.step > 0 && vn < cu.vend) {
+                cu.off -= (unsigned)cu.step;
+            } else {
+                sl.advance(cu, vn, stride);
+            }
+        }
+    }
+    return bad;
+}
 
 extern "C" int ykc_check(const ykc_engine *cfg, const ykc_replica *reps, int32_t nrep,
                          const int32_t *ops, int32_t nops, int32_t exhaustive, ykc_report *out)

@@ -123,3 +123,52 @@ def test_planted_mistakes_are_caught(plant):
     r = case.run(tuning, ops, exhaustive=True, plant=plant)
     assert r["violations"] > 0, plant
     print(plant, r["first"])
+
+
+def _walk_cases():
+    """Segment layouts of one cluster's list: (fast, back) counts per segment,
+    empty segments and runs among them, shares cut anywhere."""
+    rng = np.random.default_rng(2027)
+    cases = [
+        ([[5, 3], [0, 0], [40, 17], [2, 0]], 64),
+        ([[0, 0], [0, 0], [0, 0], [0, 0]], 16),
+        ([[0, 33], [0, 0], [0, 1], [0, 0]], 40),
+        ([[64, 0], [1, 0], [0, 0], [31, 0]], 80),
+    ]
+    for _ in range(40):
+        segcap = int(rng.integers(16, 400))
+        counts = []
+        for _ in range(4):
+            f = int(rng.integers(0, segcap + 1)) if rng.random() > 0.2 else 0
+            b = int(rng.integers(0, segcap - f + 1)) if rng.random() > 0.3 else 0
+            counts.append([f, b])
+        cases.append((counts, segcap))
+    return cases
+
+
+@pytest.mark.parametrize("plant", [0, 1, 2])
+def test_list_cursor_walk(plant):
+    """The pair kernels walk a cluster's segmented list through a cursor that
+    steps +-16 within a segment's run and recomputes at a run's end.  Over
+    every share of every layout, each entry the walk reads equals
+    SegList::offset's (the direct formula), lies in its segment's written
+    part, and every entry of the share is read exactly once.  The two planted
+    mistakes (no recompute at a run's end; fast runs stepped backwards) must
+    be caught."""
+    rng = np.random.default_rng(7)
+    caught = 0
+    for counts, segcap in _walk_cases():
+        total = int(np.sum(counts))
+        for M in (1, 3, 8):
+            for m in range(M):
+                per = (total + M - 1) // M
+                lo = min(m * per, total)
+                hi = min(lo + per, total)
+                iters = hi - lo + int(rng.integers(0, 40))   # the wave's longest share
+                bad, visits = B.cursor_walk(counts, segcap, lo, hi, iters, plant=plant)
+                if plant == 0:
+                    assert bad == 0, (counts, segcap, lo, hi)
+                    assert np.all(visits[lo:hi] == 1) and np.all(visits[:lo] == 0)
+                caught += bad > 0
+    if plant:
+        assert caught > 0

@@ -217,8 +217,10 @@ RepConsts rep_consts(const HostReplica &h)
         c.skinf = (float)c.skin;
         c.rn2f = h.near_r < 0.0 ? -1.0f : (float)(h.near_r * h.near_r);
     }
-    c.rc2f_lo = (float)(c.rc2 * (1.0 - 4e-6));
-    c.rc2f_hi = (float)(c.rc2 * (1.0 + 4e-6));
+    c.fs = p.xmuc * 1.4426950408889634074;
+    c.fs2 = c.fs * c.fs;
+    c.rc2f_lo = (float)(c.rc2 * c.fs2 * (1.0 - 4e-6));
+    c.rc2f_hi = (float)(c.rc2 * c.fs2 * (1.0 + 4e-6));
     c.r2_floor_f = 1e-20f;
     return c;
 }

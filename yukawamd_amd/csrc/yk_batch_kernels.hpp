@@ -44,6 +44,7 @@ struct RepConsts {
     double rl2, skin;
     float Lf[3], invLf[3], halfLf_m[3], rl2f, skinf;
     float rn2f;                    // near field of the mixed form (ListConsts::rn2f)
+    double fs, fs2;                       // as ForceConsts
     float rc2f_lo, rc2f_hi, r2_floor_f;   // as ForceConsts
 };
 
@@ -168,8 +169,8 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
                                                 const unsigned *__restrict__ clist,
                                                 const int2 *__restrict__ ccounts,
                                                 const BatchItem &it, double4 *__restrict__ sw,
-                                                float4 *__restrict__ sf, int lane,
-                                                double *__restrict__ out)
+                                                float4 *__restrict__ sf, SegList *__restrict__ hdr,
+                                                int lane, double *__restrict__ out)
 {
     ForceConsts c{};
     c.etab = etab;
@@ -185,6 +186,8 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
     c.muf = rc.muf;
     c.mul2ef = rc.mul2ef;
     c.mul2e_lo = rc.mul2e_lo;
+    c.fs = rc.fs;
+    c.fs2 = rc.fs2;
     c.rc2f_lo = rc.rc2f_lo;
     c.rc2f_hi = rc.rc2f_hi;
     c.r2_floor_f = rc.r2_floor_f;
@@ -221,7 +224,7 @@ __device__ __forceinline__ void batch_list_item(const RepConsts &rc, const ExpCo
     const unsigned *mine = clist + rc.lists_off + (size_t)cll * NSEG * rc.segcap;
     const SegList sl = seg_list(mine, rc.segcap, sc, hi);
     Acc<MEAS> acc;
-    clist_share<KIND, PREC, MEAS>(c, pi, cc, sl, lo, iters, pos + rc.off, sw, sf, lane, acc);
+    clist_share<KIND, PREC, MEAS>(c, pi, cc, sl, lo, iters, pos + rc.off, sw, sf, hdr, lane, acc);
     double *o = out + lane;
     o[0] = acc.fx; o[TILE] = acc.fy; o[2 * TILE] = acc.fz;
     if constexpr (MEAS) {
@@ -251,7 +254,9 @@ __global__ __launch_bounds__(512) void k_batch_pairs(const double4 *__restrict__
     // mixed list items' FP32 offsets)
     extern __shared__ double4 sj_dyn[];
     __shared__ double etab_s[EXP_TN];   // FP64 replicas' exp table, one per workgroup
+    __shared__ SegList hdr_s[512 / TILE][CPW];   // list items' list headers, per wave
     const int lane = threadIdx.x & (TILE - 1);
+    SegList *hdr = hdr_s[threadIdx.x / TILE];
     double4 *sw = sj_dyn + (threadIdx.x / TILE) * TILE;
     float4 *sf = reinterpret_cast<float4 *>(sj_dyn + blockDim.x) + (threadIdx.x / TILE) * TILE;
     load_exp_table(etab_s, etab_g);
@@ -284,14 +289,14 @@ __global__ __launch_bounds__(512) void k_batch_pairs(const double4 *__restrict__
         if (it.s1 < 0) {
             if (kind == PURE) {
                 if (prec == 0)
-                    batch_list_item<PURE, 0, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, sf, lane, out);
+                    batch_list_item<PURE, 0, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, sf, hdr, lane, out);
                 else
-                    batch_list_item<PURE, 1, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, sf, lane, out);
+                    batch_list_item<PURE, 1, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, sf, hdr, lane, out);
             } else {
                 if (prec == 0)
-                    batch_list_item<MIX, 0, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, sf, lane, out);
+                    batch_list_item<MIX, 0, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, sf, hdr, lane, out);
                 else
-                    batch_list_item<MIX, 1, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, sf, lane, out);
+                    batch_list_item<MIX, 1, MEAS>(rc, ec, etab, pos, clist, ccounts, it, sw, sf, hdr, lane, out);
             }
             continue;
         }

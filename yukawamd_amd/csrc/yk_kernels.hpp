@@ -56,10 +56,14 @@ struct ForceConsts {
     // of mul2ef does not bias every pair at distance r by the same relative
     // amount (mu r x 6e-8 at most); one rounding of the exponent remains
     float mul2e_lo;
-    // the mixed ion form's far pairs (FP32 throughout): cutoff^2 widened and
-    // narrowed by 4e-6 relative (an FP32 r^2 within that band of rc^2 takes
-    // the FP64 test, so a pair is in or out of the cutoff exactly as in FP64),
-    // and the r^2 floor that keeps a self pair's 1/r^3 finite
+    // the mixed ion form's far pairs (FP32 throughout) work in lengths scaled
+    // by fs = mu log2 e, so that e^{-mu r} = 2^{-r'} and 1 + mu r = 1 + r' ln 2;
+    // fs2 = fs^2 converts their force sums back.  rc2f_lo, rc2f_hi: the scaled
+    // cutoff^2 narrowed and widened by 4e-6 relative (an FP32 r'^2 within that
+    // band takes the FP64 test, so a pair is in or out of the cutoff exactly as
+    // in FP64); r2_floor_f: the scaled r^2 floor that keeps a self pair's
+    // 1/r'^3 finite (1e-20, below the last bit of any physical r'^2 = fs^2 r^2)
+    double fs, fs2;
     float rc2f_lo, rc2f_hi, r2_floor_f;
     int n, ntiles, coul_on, nuc_on;
     ExpCoeffs ec;
@@ -1123,20 +1127,25 @@ static __global__ __launch_bounds__(CORDER_THREADS) void k_corder(
 // SegList and seg_list (a cluster's list as its segments' entries, back
 // entries first): yk_layout.hpp.
 
-// The far pairs of the mixed ion form, FP32 throughout: target and source as
-// FP32 offsets from the cluster's centre (each the rounding of an FP64
-// difference), displacement, r^2 (with a floor that keeps a self pair's 1/r^3
-// finite), cutoff test, 1/r, exp and the force factor; the force accumulates
-// in FP32 over one 16-source batch (the caller adds it to the FP64 sums), the
-// measured quantities in FP64.  s.w: the source's Z.  An FP32 r^2 within the
-// band [rc2f_lo, rc2f_hi] of the cutoff decides by the FP64 r^2 of the target
-// pi and the source's FP64 image sd.
+// The far pairs of the mixed ion form, FP32 throughout, in lengths scaled by
+// fs = mu log2 e: target and source as FP32 scaled offsets from the cluster's
+// centre (each the rounding of an FP64 product), displacement d', r'^2 (with a
+// floor that keeps a self pair's 1/r'^3 finite), cutoff test, 1/r', exp and
+// the force factor.  With r' = fs r, e^{-mu r} = 2^{-r'} and
+// (1 + mu r) / r'^3 = (1/r')^2 (1/r' + ln 2), so the factor
+// q = Z_j 2^{-r'} (1/r')^2 (1/r' + ln 2) takes no multiply by mu and q d' is
+// the force over fs^2.  The force accumulates in FP32 over one 16-source batch
+// (the caller adds fs^2 times it to the FP64 sums), the measured quantities in
+// FP64 in unscaled units.  s.w: the source's Z.  An FP32 r'^2 within the band
+// [rc2f_lo, rc2f_hi] of the scaled cutoff decides by the FP64 r^2 of the
+// target pi and the source's FP64 image sd.
 template <int KIND, bool MEAS>
 __device__ __forceinline__ void interact_far(const ForceConsts &c, float xi, float yi, float zi,
                                              const float4 &s, const double4 &pi,
                                              const double4 *__restrict__ sd, float &fx, float &fy,
                                              float &fz, Acc<MEAS> &acc)
 {
+    constexpr float LN2F = 0.693147180559945309f;
     const float dx = xi - s.x, dy = yi - s.y, dz = zi - s.z;
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, c.r2_floor_f)));
     bool in = r2 <= c.rc2f_hi;
@@ -1148,19 +1157,20 @@ __device__ __forceinline__ void interact_far(const ForceConsts &c, float xi, flo
     }
     if (in) {
         const float rinv = rsqrt32(r2);
-        const float r = r2 * rinv;
-        const float ex = exp2_32(__builtin_fmaf(-c.mul2ef, r, -c.mul2e_lo * r));
-        float q = __builtin_fmaf(c.muf, r, 1.0f) * ex;
+        const float ex = exp2_32(-(r2 * rinv));
+        float q = ((rinv * rinv) * (rinv + LN2F)) * ex;
         if constexpr (KIND == MIX) q *= s.w;
-        const float fc = q * ((rinv * rinv) * rinv);
-        fx = __builtin_fmaf(fc, dx, fx);
-        fy = __builtin_fmaf(fc, dy, fy);
-        fz = __builtin_fmaf(fc, dz, fz);
+        fx = __builtin_fmaf(q, dx, fx);
+        fy = __builtin_fmaf(q, dy, fy);
+        fz = __builtin_fmaf(q, dz, fz);
         if constexpr (MEAS) {
             if (r2 > 10.0f * c.r2_floor_f) {   // not the self pair
-                double phi = (double)(ex * rinv) - c.evcut;
+                // Z_j e^{-mu r} / r = fs Z_j 2^{-r'} / r'; with d = d' / fs, the
+                // virial and tensor terms F r^2 and d F d are fs q r'^2 and
+                // fs d' q d'
+                double phi = c.fs * (double)(ex * rinv) - c.evcut;
                 if constexpr (KIND == MIX) phi *= (double)s.w;
-                const double sd = (double)fc, ddx = dx, ddy = dy, ddz = dz;
+                const double sd = (double)q * c.fs, ddx = dx, ddy = dy, ddz = dz;
                 const double sx = sd * ddx, sy = sd * ddy, sz = sd * ddz;
                 acc.e += phi;
                 acc.w = __builtin_fma(sd, (double)r2, acc.w);
@@ -1183,29 +1193,60 @@ __device__ __forceinline__ void interact_far(const ForceConsts &c, float xi, flo
 // lanes past their share stage NaN sources, which every pair test rejects.
 // A batch holding a back entry (general, or near field) takes the per-pair
 // minimum image -- in FP64 also in the mixed ion form (PREC 1) --; the other
-// batches of the mixed ion form run interact_far on FP32 offsets from the
-// group's cluster centre cc.  Nucleons keep the double staging in both
-// precisions.
+// batches of the mixed ion form run interact_far on FP32 offsets, scaled by
+// fs, from the group's cluster centre cc.  Nucleons keep the double staging in
+// both precisions.
 template <int KIND, int PREC, bool MEAS>
 __device__ __forceinline__ void clist_share(const ForceConsts &c, const double4 &pi,
                                             const double cc[3], const SegList &sl, int lo,
                                             int iters, const double4 *__restrict__ pos,
                                             double4 *__restrict__ sw, float4 *__restrict__ sf,
-                                            int lane, Acc<MEAS> &acc)
+                                            SegList *__restrict__ hdr, int lane, Acc<MEAS> &acc)
 {
     constexpr bool FAR32 = PREC == 1 && KIND != NN;
     const int g = lane / CL, gl = lane % CL;
     const int hi = sl.hi;
+    const unsigned *const base = sl.base;
+    // each lane walks its entries lo + gl, lo + gl + CL, ... with a cursor:
+    // the offset steps by +-CL within a segment's run; where a run ends it is
+    // recomputed from the list's header, which waits in the group's LDS slot
+    // hdr[g] (every lane of the group writes the same value) and is read
+    // there behind a compiler barrier, so that the segment counts hold no
+    // registers across the loop
+    hdr[g] = sl;
+    __builtin_amdgcn_wave_barrier();
+    SegList::Cursor cu = sl.cursor(lo + gl, CL);
+    auto advance = [&](int vn) {
+        if (vn >= cu.vend) {
+            __asm__ volatile("" ::: "memory");
+            cu = hdr[g].cursor(vn, CL);
+        } else {
+            cu.off += (unsigned)cu.step;
+        }
+    };
+    auto fetch = [&](int vn, bool &back) -> unsigned {
+        back = false;
+        if (vn >= hi) return 0u;   // entry 0 (slot 0, a valid source) past the share
+        back = cu.step < 0;
+        return base[cu.off];
+    };
+    // the far batches' common origin: the cluster centre, scaled by fs and
+    // rounded to FP32 (any point common to the group's targets and sources
+    // serves; an FP32 one takes half the registers); offsets from it are
+    // formed by one FP64 FMA each and rounded to FP32
     float xi_f = 0.f, yi_f = 0.f, zi_f = 0.f;
+    float ccs[3] = {0.f, 0.f, 0.f};
     if constexpr (FAR32) {
-        xi_f = (float)(pi.x - cc[0]);
-        yi_f = (float)(pi.y - cc[1]);
-        zi_f = (float)(pi.z - cc[2]);
+        for (int k = 0; k < 3; ++k) ccs[k] = (float)(cc[k] * c.fs);
+        xi_f = (float)__builtin_fma(pi.x, c.fs, -(double)ccs[0]);
+        yi_f = (float)__builtin_fma(pi.y, c.fs, -(double)ccs[1]);
+        zi_f = (float)__builtin_fma(pi.z, c.fs, -(double)ccs[2]);
     }
     bool g_cur, g_nxt;
-    unsigned e_cur = sl.entry(lo + gl, g_cur);
+    unsigned e_cur = fetch(lo + gl, g_cur);
     double4 pj_cur = pos[e_cur & SLOT_MASK];
-    unsigned e_nxt = sl.entry(lo + CL + gl, g_nxt);
+    advance(lo + CL + gl);
+    unsigned e_nxt = fetch(lo + CL + gl, g_nxt);
     for (int b0 = 0; b0 < iters; b0 += CL) {
         const int v = lo + b0 + gl;
         const bool live = v < hi;
@@ -1221,8 +1262,14 @@ __device__ __forceinline__ void clist_share(const ForceConsts &c, const double4 
         const double q = __builtin_nan("");
         if (FAR32 && !gen) {
             const float qf = (float)q;
-            sf[lane] = live ? make_float4((float)(pj.x - cc[0]), (float)(pj.y - cc[1]),
-                                          (float)(pj.z - cc[2]), (float)pj.w)
+            // the origin's FP64 values are formed here, not held across the
+            // loop (the empty asm keeps the compiler from hoisting them)
+            float o0 = ccs[0], o1 = ccs[1], o2 = ccs[2];
+            __asm__("" : "+v"(o0), "+v"(o1), "+v"(o2));
+            sf[lane] = live ? make_float4((float)__builtin_fma(pj.x, c.fs, -(double)o0),
+                                          (float)__builtin_fma(pj.y, c.fs, -(double)o1),
+                                          (float)__builtin_fma(pj.z, c.fs, -(double)o2),
+                                          (float)pj.w)
                             : make_float4(qf, qf, qf, 0.f);
         }
         // the FP64 images: general batches, and the far batches' cutoff band
@@ -1232,7 +1279,8 @@ __device__ __forceinline__ void clist_share(const ForceConsts &c, const double4 
         e_cur = e_nxt;
         g_cur = g_nxt;
         pj_cur = pos[e_cur & SLOT_MASK];
-        e_nxt = sl.entry(lo + b0 + 2 * CL + gl, g_nxt);
+        advance(v + 2 * CL);
+        e_nxt = fetch(v + 2 * CL, g_nxt);
         const int kb = min(CL, iters - b0);
         // FP64 (the mixed ion form's too), minimum image per pair
         auto general_batch = [&]() {
@@ -1271,9 +1319,9 @@ __device__ __forceinline__ void clist_share(const ForceConsts &c, const double4 
                                                  acc);
                     }
                 }
-                acc.fx += (double)fx;
-                acc.fy += (double)fy;
-                acc.fz += (double)fz;
+                acc.fx = __builtin_fma(c.fs2, (double)fx, acc.fx);
+                acc.fy = __builtin_fma(c.fs2, (double)fy, acc.fy);
+                acc.fz = __builtin_fma(c.fs2, (double)fz, acc.fz);
             } else {
                 general_batch();
             }
@@ -1347,6 +1395,7 @@ void k_pairs_clist(const double4 *__restrict__ pos,
 {
     __shared__ double4 sj[BLOCK / TILE][TILE];
     __shared__ float4 sfj[BLOCK / TILE][(PREC == 1 && KIND != NN) ? TILE : 1];
+    __shared__ SegList hdr_s[BLOCK / TILE][CPW];
     constexpr bool TAB = PREC == 0 || KIND != NN;   // an FP64 pair path with the exp table
     const int n = c.n;
     if constexpr (TAB) pin_vgpr(c.ec.shifter);
@@ -1398,7 +1447,8 @@ void k_pairs_clist(const double4 *__restrict__ pos,
     const unsigned *mine = clist + (size_t)(cl >= 0 ? cl : 0) * NSEG * segcap;
     const SegList sl = seg_list(mine, segcap, sc, hi);
     Acc<MEAS> acc;
-    clist_share<KIND, PREC, MEAS>(c, pi, cc, sl, lo, iters, pos, sj[wv], sfj[wv], lane, acc);
+    clist_share<KIND, PREC, MEAS>(c, pi, cc, sl, lo, iters, pos, sj[wv], sfj[wv], hdr_s[wv], lane,
+                                  acc);
     if (cl < 0) return;
     constexpr int NC = MEAS ? 11 : 3;
     double *o = part + ((size_t)cl * M + m) * NC * CL + gl;

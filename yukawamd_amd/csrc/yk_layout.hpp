@@ -47,10 +47,19 @@ struct SegList {
     int g1, g2, g3;      // back entries of segments below 1, 2, 3
     int f1, f2, f3;      // fast entries of segments below 1, 2, 3
 
-    // The entry's offset from base (v < hi); back: whether it is a back entry.
-    [[gnu::always_inline]] constexpr long long offset(int v, bool &back) const
+    // A walk through the list with a fixed stride: the offset of the current
+    // entry v, the offset's step to entry v + stride while that stays in v's
+    // run (+stride in a segment's fast entries, -stride in its back entries),
+    // and the first v past the run (INT_MAX at or past hi, where nothing is
+    // read).  Offsets are below NSEG * segcap, which fits 32 bits.
+    struct Cursor {
+        unsigned off;
+        int step, vend;
+    };
+
+    [[gnu::always_inline]] constexpr Cursor cursor(int v, int stride) const
     {
-        back = v < nback;
+        const bool back = v < nback;
         // arithmetic selection of the segment (selects of addresses would put
         // the fields on the stack)
         const int w = back ? v : v - nback;
@@ -59,7 +68,37 @@ struct SegList {
         const int sg = s1 + s2 + s3;
         const int start = s1 * (c1 + s2 * (c2 - c1 + s3 * (c3 - c2)));
         const int o = w - start;
-        return (long long)sg * segcap + (back ? segcap - 1 - o : o);
+        // the run's end: the next segment's first entry of the same kind; the
+        // back entries of segment 3 end at nback, the fast ones of segment 3
+        // at the list's end
+        const int wend = s3 ? 0x7fffffff : s2 ? c3 : s1 ? c2 : c1;
+        Cursor cu{};
+        cu.off = (unsigned)(sg * segcap + (back ? segcap - 1 - o : o));
+        cu.step = back ? -stride : stride;
+        cu.vend = v >= hi ? 0x7fffffff : back ? (s3 ? nback : wend) : (s3 ? wend : nback + wend);
+        return cu;
+    }
+    // The cursor of v + stride from that of v.
+    [[gnu::always_inline]] constexpr void advance(Cursor &cu, int vnew, int stride) const
+    {
+        if (vnew >= cu.vend) cu = cursor(vnew, stride);
+        else cu.off += (unsigned)cu.step;
+    }
+    // The entry at the cursor of v, or 0 (slot 0, a valid source) for
+    // v >= hi; back: whether it is a back entry (false for v >= hi).
+    [[gnu::always_inline]] constexpr unsigned fetch(const Cursor &cu, int v, bool &back) const
+    {
+        back = false;
+        if (v >= hi) return 0u;
+        back = cu.step < 0;
+        return base[cu.off];
+    }
+
+    // The entry's offset from base (v < hi); back: whether it is a back entry.
+    [[gnu::always_inline]] constexpr long long offset(int v, bool &back) const
+    {
+        back = v < nback;
+        return (long long)cursor(v, 1).off;
     }
     // Entry v, or 0 (slot 0, a valid source) for v >= hi.
     [[gnu::always_inline]] constexpr unsigned entry(int v, bool &back) const

@@ -213,8 +213,10 @@ ForceConsts force_consts_of(const yk_params &p, int n, int ntiles, const double 
     c.bpcf = (float)c.bpc;
     c.bmcf = (float)c.bmc;
     c.vcf = (float)p.vc;
-    c.rc2f_lo = (float)(c.rc2 * (1.0 - 4e-6));
-    c.rc2f_hi = (float)(c.rc2 * (1.0 + 4e-6));
+    c.fs = p.xmuc * LOG2E;
+    c.fs2 = c.fs * c.fs;
+    c.rc2f_lo = (float)(c.rc2 * c.fs2 * (1.0 - 4e-6));
+    c.rc2f_hi = (float)(c.rc2 * c.fs2 * (1.0 + 4e-6));
     c.r2_floor_f = 1e-20f;
     c.n = n;
     c.ntiles = ntiles;
