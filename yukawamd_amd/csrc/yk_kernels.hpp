@@ -1585,6 +1585,45 @@ struct DriftArgs {
     unsigned long long *maxdisp;
 };
 
+// The per-slot sums of k_pairs_clist's M item partials (part[cl][m][NC][CL])
+// for lists of many items per cluster set (small systems, where k_finalize's
+// one thread per slot would wait on M dependent load rounds from a few
+// workgroups): one workgroup per cluster, T threads per slot, thread t
+// summing items t, t + T, t + 2T, ... in order, then the T sub-sums added in
+// order t = 0 .. T-1.  out[k][n] is read by k_finalize as one partial
+// (LAYOUT 0, nparts = 1); the order is fixed, so results are deterministic.
+template <int NC, int T>
+__global__ __launch_bounds__(CL *T) void k_part_reduce(const double *__restrict__ part, int M,
+                                                       int n, double *__restrict__ out)
+{
+    static_assert(T >= 4 && CL * T <= 1024, "T sub-sums per slot");
+    __shared__ double red[T][NC][CL];
+    const int cl = blockIdx.x;
+    const int gl = threadIdx.x % CL, t = threadIdx.x / CL;
+    double f[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) f[k] = 0.0;
+    const double *p0 = part + (size_t)cl * M * NC * CL + gl;
+    // 24 (force-only) or 22 (measuring) loads in flight
+    constexpr int UNROLL = NC == 3 ? 8 : 2;
+#pragma unroll UNROLL
+    for (int m = t; m < M; m += T) {
+        const double *p = p0 + (size_t)m * NC * CL;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) f[k] += p[k * CL];
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) red[t][k][gl] = f[k];
+    __syncthreads();
+    const int i = cl * CL + gl;
+    for (int k = t; k < NC; k += T) {
+        double s = red[0][k][gl];
+#pragma unroll
+        for (int u = 1; u < T; ++u) s += red[u][k][gl];
+        if (i < n) out[(size_t)k * n + i] = s;
+    }
+}
+
 template <int KIND, bool MEAS, int MODE, int LAYOUT, bool DRIFT = false>
 __global__ __launch_bounds__(BLOCK) void k_finalize(
     const double *__restrict__ part, int nparts, const double4 *__restrict__ pos,

@@ -100,6 +100,7 @@ struct yk_system {
     unsigned long long swept_base = 0;
     double *part = nullptr;        // [nchunks][11][n] or [ntiles][M][11][64]
     size_t part_doubles = 0;
+    double *psum = nullptr;        // [11][n] cluster-list partials summed (k_part_reduce)
     double *blockpart = nullptr;   // [nblocks][NQ]
     MeasOut *d_meas = nullptr;     // device scalar records
     double *d_vstate = nullptr;    // [4] vcom, fac of the velocity utilities
@@ -544,9 +545,42 @@ int launch_pairs(yk_system *s, bool meas)
     return YK_OK;
 }
 
+// Cluster-list launches of at least this many items per cluster set (small
+// systems) sum their partials with k_part_reduce before the finalize.
+constexpr int PART_REDUCE_MIN_M = 64;
+
+template <int NC>
+void launch_part_reduce(yk_system *s)
+{
+    if (s->list_M >= 128)
+        hipLaunchKernelGGL((k_part_reduce<NC, 16>), dim3(s->ncl), dim3(CL * 16), 0, s->stream,
+                           s->part, s->list_M, s->n, s->psum);
+    else
+        hipLaunchKernelGGL((k_part_reduce<NC, 8>), dim3(s->ncl), dim3(CL * 8), 0, s->stream,
+                           s->part, s->list_M, s->n, s->psum);
+}
+
 template <int KIND, bool MEAS, int MODE>
 void launch_fin_k(yk_system *s, const StepConsts &sc, bool drift)
 {
+    if (s->use_clist && s->list_M >= PART_REDUCE_MIN_M) {
+        // the partials summed by k_part_reduce, read as one LAYOUT 0 part
+        launch_part_reduce<MEAS ? 11 : 3>(s);
+        if constexpr (MODE == 1) {
+            if (drift) {   // with the next step's drift
+                const DriftArgs dr{s->pos, s->vold, s->ntiles, s->box,
+                                   s->use_list ? s->xref : nullptr, s->maxdisp};
+                hipLaunchKernelGGL((k_finalize<KIND, MEAS, 1, 0, true>), dim3(s->nblocks),
+                                   dim3(BLOCK), 0, s->stream, s->psum, 1, s->pos, s->den, s->aii,
+                                   s->orig, s->acc, s->vel, s->vold, sc, s->blockpart, dr);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((k_finalize<KIND, MEAS, MODE, 0>), dim3(s->nblocks), dim3(BLOCK), 0,
+                           s->stream, s->psum, 1, s->pos, s->den, s->aii, s->orig, s->acc,
+                           s->vel, s->vold, sc, s->blockpart);
+        return;
+    }
     if constexpr (MODE == 1) {
         if (drift && s->use_clist) {   // with the next step's drift
             const DriftArgs dr{s->pos, s->vold, s->ntiles, s->box,
@@ -804,6 +838,7 @@ void release(yk_system *s)
                       (void *)s->vold, (void *)s->vold2, (void *)s->acc, (void *)s->acc2,
                       (void *)s->aii, (void *)s->aii2, (void *)s->den, (void *)s->den2,
                       (void *)s->orig, (void *)s->orig2, (void *)s->box, (void *)s->part,
+                      (void *)s->psum,
                       (void *)s->blockpart, (void *)s->d_meas, (void *)s->d_vstate, (void *)s->etab_g,
                       (void *)s->keys,
                       (void *)s->keys2, (void *)s->vals, (void *)s->vals2, s->sort_tmp,
@@ -1080,6 +1115,8 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
                               sizeof(unsigned) * (size_t)s->ncl * NSEG * s->segcap));
             YK_HIPB(hipMalloc(&s->ccounts, sizeof(int2) * (size_t)s->ncl * NSEG));
             YK_HIPB(hipMalloc(&s->corder, sizeof(int) * (size_t)s->ncl));
+            if (s->list_M >= PART_REDUCE_MIN_M)
+                YK_HIPB(hipMalloc(&s->psum, sizeof(double) * 11 * nn));
         }
     }
     YK_HIPB(hipMalloc(&s->maxdisp, 2 * sizeof(unsigned long long)));
