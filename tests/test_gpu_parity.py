@@ -152,6 +152,42 @@ def test_run_is_repeated_newton_across_resorts(kind):
             assert np.array_equal(r1[k], r2[k]), k
 
 
+@pytest.mark.parametrize("n", [6000, 10000, 17000])
+def test_item_partial_sums_against_oracle(n):
+    """The cluster-list launch takes 128 items per cluster set below 8,192
+    particles, 64 up to 16,384 and 32 above; from 64 items the partials are
+    summed by k_part_reduce (16 and 8 threads per slot) before the finalize,
+    below that by the finalize itself.  Each against the oracle, measuring
+    and not, and a run loop (fused drift) equal to single newton steps."""
+    w = W.ion_mixture(n=n, seed=95)
+    p = w.params
+    for dm in (True, False):
+        a_o, m_o = oracle.accel(p, w.x, w.Z, w.A, dm, ref_f1=0)
+        a_g, m_g = gpu_accel(p, w.x, w.Z, w.A, dm)
+        rel, cond = force_errors(p, a_g, a_o, w.x, w.Z, w.A)
+        print(f"n={n} meas={dm}: max rel {rel:.2e} cond {cond:.2e}")
+        assert cond <= 1e-12 and rel <= 1e-10
+        if dm:
+            check_meas(m_g, m_o)
+    out = []
+    for loop in (False, True):
+        with YukawaSystem(p, Z=w.Z, A=w.A, dt=w.dt) as s:
+            s.put_state(w.x, w.v)
+            s.accel(True)
+            if loop:
+                recs = s.run(6, nind=3)
+            else:
+                recs = [s.newton(st % 3 == 0) for st in range(1, 7)]
+                recs = [r for st, r in enumerate(recs, 1) if st % 3 == 0]
+            out.append((s.get_state(), recs))
+    (sa, ra), (sb, rb) = out
+    for u, v in zip(sa, sb):
+        assert np.array_equal(u, v)
+    assert len(ra) == len(rb) == 2
+    for r1, r2 in zip(ra, rb):
+        assert r1 == r2
+
+
 def test_force_independent_of_measurement():
     """Unlike the reference's two branches (F2), the device force does not depend
     on do_measurements: both templates accumulate it with the same FMAs."""
