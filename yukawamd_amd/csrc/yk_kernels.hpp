@@ -397,6 +397,10 @@ struct ListConsts {
     // double test would: rl2f = (rl (1 + 5e-6) + 1e-6 max L)^2 (the float
     // offsets are within a few ulp of L); halfLf_m = halfL_m - 1e-5 L
     float Lf[3], invLf[3], halfLf_m[3], rl2f, skinf;
+    // the same for sources that interact only through the nucleon model's
+    // nuclear term (neutrons): (nuclear radius + skin)^2, widened alike
+    // (rl2f for ions)
+    float rl2f_n;
     // cluster lists of the mixed ion form: sources whose distance to the
     // cluster's AABB is at most sqrt(rn2f) (the near field) are stored with
     // the general entries, which the mixed pair kernels evaluate in FP64;
@@ -796,9 +800,12 @@ static __global__ __launch_bounds__(BLOCK) void k_tile_boxes(const double4 *__re
 // the test's bounds), and in .w the entry's
 // per-axis image code for the shift that reduction applied (bits 26-31; the
 // source is seen at x + (code - 1) L), bit 0 set when a shift lies beyond one
-// box (such a source gives general entries only).  An absent source (a slot
-// beyond the system) is staged FAR_AWAY on every axis with bit 0 set.
+// box (such a source gives general entries only), bit 1 set for a source of
+// charge 0 (a neutron of the nucleon model, listed only out to the nuclear
+// radius).  An absent source (a slot beyond the system) is staged FAR_AWAY
+// on every axis with bit 0 set.
 constexpr unsigned CODE_MASK = ~SLOT_MASK;
+constexpr unsigned STAGE_NEUTRAL = 2u;
 constexpr float FAR_AWAY = 3.0e38f;
 
 __device__ __forceinline__ float4 clist_stage(const double4 &p, bool present, const double tc[3],
@@ -816,6 +823,7 @@ __device__ __forceinline__ float4 clist_stage(const double4 &p, bool present, co
         w |= (unsigned)(ok ? 1 - (int)m : 1) << (26 + 2 * k);
         w |= ok ? 0u : 1u;
     }
+    w |= p.w == 0.0 ? STAGE_NEUTRAL : 0u;
     return make_float4(o[0], o[1], o[2], __uint_as_float(present ? w : 1u));
 }
 
@@ -833,15 +841,17 @@ __device__ __forceinline__ float4 clist_stage(const double4 &p, bool present, co
 //      general entries only.
 struct ClusterView {
     float cr[3], hf[3], lim[3];
+    float rl2n;   // list radius^2 of neutral sources (ListConsts::rl2f_n)
     int tier;
 };
 
 __device__ __forceinline__ ClusterView cluster_view(const double c[3], const double h[3],
                                                     bool live, const double tc[3],
                                                     const float Lf[3], const float halfLf_m[3],
-                                                    float skinf, float rl2f)
+                                                    float skinf, float rl2f, float rl2nf)
 {
     ClusterView v;
+    v.rl2n = rl2nf;
     const float rlf = __builtin_sqrtf(rl2f);
     bool t2 = live, t1 = live;
 #pragma unroll
@@ -861,7 +871,7 @@ __device__ __forceinline__ ClusterView cluster_view(const double c[3], const dou
 
 // The candidate test of the staged source s against the cluster box (FP32
 // with the widened bounds of ListConsts): cand, the source is within the list
-// radius; fast, its staged image serves every target of the cluster for the
+// radius (the nuclear one, v.rl2n, for a neutral source); fast, its staged image serves every target of the cluster for the
 // list's lifetime; and the same as this wave's lane masks mcand, mfast.  Each
 // mask is the ballot of a single compare (which compiles to the compare's own
 // result), combined with scalar ANDs.  Absent sources fail at tiers 1 and 2
@@ -898,7 +908,7 @@ __device__ __forceinline__ void clist_test(const ClusterView &v, const float4 &s
         gk = gk > 0.0f ? gk : 0.0f;
         d2 = __builtin_fmaf(gk, gk, d2);
     }
-    cand = d2 <= rl2f;
+    cand = d2 <= ((__float_as_uint(s.w) & STAGE_NEUTRAL) ? v.rl2n : rl2f);
     mcand = __builtin_amdgcn_ballot_w64(cand);
     // a near-field source is stored with the general entries (evaluated in
     // FP64 by the mixed kernels)
@@ -1042,7 +1052,8 @@ static __global__ __launch_bounds__(BLOCK) void k_clist_build(
         }
     }
     const bool live = cvalid && h[0] >= 0.0;   // the cluster holds a particle
-    const ClusterView cv = cluster_view(c, h, live, tc, lc.Lf, lc.halfLf_m, lc.skinf, lc.rl2f);
+    const ClusterView cv = cluster_view(c, h, live, tc, lc.Lf, lc.halfLf_m, lc.skinf, lc.rl2f,
+                                        lc.rl2f_n);
     // candidate source tiles of the whole tile, in ascending order
     int ncand = 0;
     {

@@ -196,7 +196,7 @@ ForceConsts force_consts_of(const yk_params &p, int n, int ntiles, const double 
     c.mu = p.xmuc;
     c.evcut = evcut_of(p);
     c.xp2i = (p.xpacket > 0) ? 1.0 / (2.0 * p.xpacket) : 0.0;
-    c.rn2 = exp_cutoff2(p.rncut * p.rncut, c.xp2i, false);
+    c.rn2 = gauss_cutoff2(exp_cutoff2(p.rncut * p.rncut, c.xp2i, false), c.xp2i);
     c.xpi = (p.xpacket > 0) ? 1.0 / p.xpacket : 0.0;
     c.aa = p.aa;
     c.aa2 = 2.0 * p.aa;
@@ -244,15 +244,31 @@ ForceConsts force_consts(const yk_system *s)
     return force_consts_of(s->p, s->n, s->ntiles, s->etab_g);
 }
 
+// The radius inside which the nucleon kernels evaluate the nuclear term:
+// rncut, or less where the Gaussian is negligible (gauss_cutoff2).
+double nuclear_cutoff(const yk_params &p)
+{
+    const double xp2i = (p.xpacket > 0) ? 1.0 / (2.0 * p.xpacket) : 0.0;
+    return std::sqrt(gauss_cutoff2(p.rncut * p.rncut, xp2i));
+}
+
 double list_cutoff(const yk_params &p)
 {
     double cut = p.rccut;
     if (p.sim_type == YK_NUCLEON) {
         cut = 0.0;
         if (p.coulomb_on) cut = p.rccut;
-        if (p.nuclear_on) cut = std::max(cut, p.rncut);
+        if (p.nuclear_on) cut = std::max(cut, nuclear_cutoff(p));
     }
     return cut;
+}
+
+// The list radius of sources that interact only through the nuclear term
+// (neutrons of the nucleon model; every source of an ion system).
+double list_cutoff_neutral(const yk_params &p)
+{
+    if (p.sim_type != YK_NUCLEON) return list_cutoff(p);
+    return p.nuclear_on ? nuclear_cutoff(p) : 0.0;
 }
 
 ListConsts list_consts(const yk_system *s)
@@ -281,6 +297,9 @@ ListConsts list_consts(const yk_system *s)
         const double lm = std::max(p.xl[0], std::max(p.xl[1], p.xl[2]));
         const double rlf = rl * (1.0 + 5e-6) + 1e-6 * lm;
         l.rl2f = (float)(rlf * rlf);
+        const double rln = list_cutoff_neutral(p) + s->skin + margin;
+        const double rlnf = rln * (1.0 + 5e-6) + 1e-6 * lm;
+        l.rl2f_n = (float)(rlnf * rlnf);
     }
     l.skinf = (float)s->skin;
     l.rn2f = s->near_r < 0.0 ? -1.0f : (float)(s->near_r * s->near_r);

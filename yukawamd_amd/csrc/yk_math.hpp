@@ -80,6 +80,23 @@ inline double exp_cutoff2(double cut2, double rate, bool of_r)
     return cut2 < lim2 ? cut2 : lim2;
 }
 
+// The nucleon model's Gaussian factor g = e^{-r^2 xp2i} is below 2^-80 beyond
+// r^2 = 80 ln2 / xp2i.  A pair there adds at most xpi (2 |aa| g + |b +- c|) g r
+// to a force and (|aa| g + |b +- c|) g to the energy: for the pasta parameters
+// (aa = 110, b +- c = -2 / -50 MeV, xpacket = 1.25 fm^2, cutoff 20 fm) below
+// 7e-22 MeV/fm and 4e-23 MeV per pair, 9e-19 MeV/fm over the 1,300 such
+// neighbours of a nucleon.  The kernels evaluate the nuclear term only inside
+// that radius, and the cluster lists hold sources that interact only through
+// it (neutrons) only out to it.
+constexpr double GAUSS_NEGLIGIBLE_LOG = 80.0 * 0.69314718055994530942;
+
+inline double gauss_cutoff2(double cut2, double xp2i)
+{
+    if (!(xp2i > 0.0)) return cut2;
+    const double lim2 = GAUSS_NEGLIGIBLE_LOG / xp2i;
+    return cut2 < lim2 ? cut2 : lim2;
+}
+
 // The 2^(j/N) table, j = 0 .. N-1, rounded to double from long double.
 inline void exp_table(double *t)
 {
