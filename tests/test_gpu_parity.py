@@ -302,6 +302,30 @@ def test_pairlist_matches_allpairs_over_a_trajectory():
         assert r1["ek"] == pytest.approx(r2["ek"], rel=1e-10)
 
 
+def test_nucleon_lists_valid_along_a_trajectory():
+    """Nucleon cluster lists (neutrons and protons in separate Hilbert runs,
+    neutral sources and all sources of a proton-free cluster listed only out
+    to the nuclear radius) stay complete while they are reused: every five
+    steps the run's forces match the all-pairs kernel on the same positions,
+    and at the end the oracle, which evaluates the Gaussian out to rncut."""
+    w = W.nucleon(n=8000, seed=104)
+    p = w.params
+    with YukawaSystem(p, Z=w.Z, A=w.A, dt=w.dt) as s:
+        s.put_state(w.x, w.v)
+        s.accel(True)
+        for k in range(4):
+            s.run(5, nind=5)
+            x, v, a = s.get_state()
+            a_ap, _ = gpu_accel(p, x, w.Z, w.A, False, "allpairs")
+            rel, cond = force_errors(p, a, a_ap, x, w.Z, w.A)
+            print(f"after {5 * (k + 1)} steps: lists vs all-pairs max rel {rel:.2e} cond {cond:.2e}")
+            assert cond <= 1e-12
+    a_o, _ = oracle.accel(p, x, w.Z, w.A, False, ref_f1=0)
+    rel, cond = force_errors(p, a, a_o, x, w.Z, w.A)
+    print(f"against the oracle: max rel {rel:.2e} cond {cond:.2e}")
+    assert cond <= 1e-12
+
+
 @pytest.mark.parametrize("path", ["clist", "tilelist"])
 def test_pairlist_valid_with_one_fast_ion(path):
     """One fast ion (0.45 skin per step) among nearly cold ones: the lists are

@@ -416,7 +416,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_clist_build(
     }
     const bool live = cvalid && h[0] >= 0.0;
     const ClusterView cv = cluster_view(c, h, live, tc, rc.Lf, rc.halfLf_m, rc.skinf, rc.rl2f,
-                                        rc.rl2f);   // ions: every source to the one radius
+                                        rc.rl2f, true);   // ions: every source to the one radius
     int ncand = 0;
     {
         const double rl = __builtin_sqrt(rc.rl2);

@@ -841,16 +841,22 @@ __device__ __forceinline__ float4 clist_stage(const double4 &p, bool present, co
 //      general entries only.
 struct ClusterView {
     float cr[3], hf[3], lim[3];
-    float rl2n;   // list radius^2 of neutral sources (ListConsts::rl2f_n)
+    // list radius^2 of charged and of neutral sources: ListConsts::rl2f and
+    // rl2f_n, or rl2f_n for both when the cluster holds no charged particle
+    // (in the nucleon model only proton pairs interact beyond the nuclear
+    // radius)
+    float rl2c, rl2n;
     int tier;
 };
 
 __device__ __forceinline__ ClusterView cluster_view(const double c[3], const double h[3],
                                                     bool live, const double tc[3],
                                                     const float Lf[3], const float halfLf_m[3],
-                                                    float skinf, float rl2f, float rl2nf)
+                                                    float skinf, float rl2f, float rl2nf,
+                                                    bool charged)
 {
     ClusterView v;
+    v.rl2c = charged ? rl2f : rl2nf;
     v.rl2n = rl2nf;
     const float rlf = __builtin_sqrtf(rl2f);
     bool t2 = live, t1 = live;
@@ -871,7 +877,7 @@ __device__ __forceinline__ ClusterView cluster_view(const double c[3], const dou
 
 // The candidate test of the staged source s against the cluster box (FP32
 // with the widened bounds of ListConsts): cand, the source is within the list
-// radius (the nuclear one, v.rl2n, for a neutral source); fast, its staged image serves every target of the cluster for the
+// radius (v.rl2c or, for a neutral source, v.rl2n); fast, its staged image serves every target of the cluster for the
 // list's lifetime; and the same as this wave's lane masks mcand, mfast.  Each
 // mask is the ballot of a single compare (which compiles to the compare's own
 // result), combined with scalar ANDs.  Absent sources fail at tiers 1 and 2
@@ -908,7 +914,7 @@ __device__ __forceinline__ void clist_test(const ClusterView &v, const float4 &s
         gk = gk > 0.0f ? gk : 0.0f;
         d2 = __builtin_fmaf(gk, gk, d2);
     }
-    cand = d2 <= ((__float_as_uint(s.w) & STAGE_NEUTRAL) ? v.rl2n : rl2f);
+    cand = d2 <= ((__float_as_uint(s.w) & STAGE_NEUTRAL) ? v.rl2n : v.rl2c);
     mcand = __builtin_amdgcn_ballot_w64(cand);
     // a near-field source is stored with the general entries (evaluated in
     // FP64 by the mixed kernels)
@@ -1035,13 +1041,16 @@ static __global__ __launch_bounds__(BLOCK) void k_clist_build(
     double tc[3], th[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) { tc[k] = box[t].c[k]; th[k] = box[t].h[k]; }
-    // cluster AABB (lanes 0..15 hold its slots)
+    // cluster AABB (lanes 0..15 hold its slots) and whether it holds a
+    // charged particle
     double c[3], h[3];
+    bool charged;
     {
         const int i = cl * CL + lane;
         const bool v = lane < CL && i < lc.n;
         const double4 p = v ? pos[i] : make_double4(0, 0, 0, 0);
         if (v && seg == 0) xref[i] = p;
+        charged = __ballot(v && p.w != 0.0) != 0ull;
         const double xi[3] = {p.x, p.y, p.z};
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -1053,7 +1062,7 @@ static __global__ __launch_bounds__(BLOCK) void k_clist_build(
     }
     const bool live = cvalid && h[0] >= 0.0;   // the cluster holds a particle
     const ClusterView cv = cluster_view(c, h, live, tc, lc.Lf, lc.halfLf_m, lc.skinf, lc.rl2f,
-                                        lc.rl2f_n);
+                                        lc.rl2f_n, charged);
     // candidate source tiles of the whole tile, in ascending order
     int ncand = 0;
     {
@@ -1512,11 +1521,15 @@ static __global__ __launch_bounds__(BLOCK) void k_clist_swept(const int2 *__rest
 
 // ---------------------------------------------------------------------------
 // Hilbert keys of the current slots (cell grid 2^g per axis) and the gather
-// that applies a sort permutation to every per-particle array.
+// that applies a sort permutation to every per-particle array.  by_charge:
+// charged particles take bit 3g of the key, so the sort puts the neutral
+// ones (neutrons of the nucleon model) first and the charged ones after,
+// each in Hilbert order, and a 16-slot cluster holds one kind except where
+// the two runs meet.
 static __global__ __launch_bounds__(BLOCK) void k_hilbert(double4 *__restrict__ pos, int n,
                                                    double gx, double gy, double gz, int g,
                                                    double Lx, double Ly, double Lz,
-                                                   unsigned *__restrict__ key,
+                                                   int by_charge, unsigned *__restrict__ key,
                                                    int *__restrict__ val)
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
@@ -1531,7 +1544,8 @@ static __global__ __launch_bounds__(BLOCK) void k_hilbert(double4 *__restrict__ 
     cx = cx < 0 ? 0 : (cx > gmax ? gmax : cx);
     cy = cy < 0 ? 0 : (cy > gmax ? gmax : cy);
     cz = cz < 0 ? 0 : (cz > gmax ? gmax : cz);
-    key[i] = hilbert_key3((unsigned)cx, (unsigned)cy, (unsigned)cz, g);
+    const unsigned charged = (by_charge && p.w != 0.0) ? 1u << (3 * g) : 0u;
+    key[i] = hilbert_key3((unsigned)cx, (unsigned)cy, (unsigned)cz, g) | charged;
     val[i] = i;
 }
 

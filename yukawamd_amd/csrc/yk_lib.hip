@@ -701,13 +701,17 @@ int resort(yk_system *s)
     const int n = s->n;
     const int g = s->grid_bits;
     const double G = (double)(1 << g);
+    // nucleons: neutrons and protons in separate Hilbert runs, so that a
+    // neutron cluster's list needs sources only within the nuclear radius
+    // (list_cutoff_neutral) and only proton clusters reach to the Coulomb one
+    const int by_charge = s->p.sim_type == YK_NUCLEON ? 1 : 0;
     hipLaunchKernelGGL(k_hilbert, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos, n,
                        G / s->p.xl[0], G / s->p.xl[1], G / s->p.xl[2], g,
-                       s->p.xl[0], s->p.xl[1], s->p.xl[2], s->keys, s->vals);
+                       s->p.xl[0], s->p.xl[1], s->p.xl[2], by_charge, s->keys, s->vals);
     YK_HIP(hipGetLastError());
     size_t bytes = s->sort_tmp_bytes;
     YK_HIP(hipcub::DeviceRadixSort::SortPairs(s->sort_tmp, bytes, s->keys, s->keys2, s->vals,
-                                              s->vals2, n, 0, 3 * g, s->stream));
+                                              s->vals2, n, 0, 3 * g + by_charge, s->stream));
     PermArrays a{s->pos, s->vel, s->vold, s->acc, s->aii, s->den, s->orig,
                  s->pos2, s->vel2, s->vold2, s->acc2, s->aii2, s->den2, s->orig2};
     hipLaunchKernelGGL(k_permute, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->vals2, n, a);
@@ -1155,8 +1159,9 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
     YK_HIPB(hipMalloc(&s->keys2, sizeof(unsigned) * nn));
     YK_HIPB(hipMalloc(&s->vals, sizeof(int) * nn));
     YK_HIPB(hipMalloc(&s->vals2, sizeof(int) * nn));
+    // sized for the keys' largest width (3g bits and the charge bit)
     YK_HIPB(hipcub::DeviceRadixSort::SortPairs(nullptr, s->sort_tmp_bytes, s->keys, s->keys2,
-                                               s->vals, s->vals2, n, 0, 3 * s->grid_bits,
+                                               s->vals, s->vals2, n, 0, 3 * s->grid_bits + 1,
                                                s->stream));
     YK_HIPB(hipMalloc(&s->sort_tmp, s->sort_tmp_bytes > 0 ? s->sort_tmp_bytes : 16));
     YK_HIPB(hipHostMalloc(&s->h_stage, sizeof(double) * 4 * nn, hipHostMallocDefault));
