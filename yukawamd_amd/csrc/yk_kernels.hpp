@@ -389,6 +389,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const double4 *__restrict__ pos
 struct ListConsts {
     double L[3], invL[3], halfL_m[3];
     double rl2;          // (cutoff + skin)^2
+    double rl2n;         // the same for a pair not both charged (nuclear radius)
     double skin;
     int n, ntiles, cap;  // cap = entries per tile region (= n)
     int pad;
@@ -782,6 +783,20 @@ static __global__ __launch_bounds__(BLOCK) void k_tile_boxes(const double4 *__re
     if (tile < ntiles) tile_box_store(x, valid, tile, box);
 }
 
+// Whether each 64-slot tile holds a particle of nonzero charge (one wave per
+// tile; slots keep their particles between re-sorts, so this is computed
+// after each one).
+static __global__ __launch_bounds__(BLOCK) void k_tile_charge(const double4 *__restrict__ pos,
+                                                              int n, int ntiles,
+                                                              int *__restrict__ tchg)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int tile = i / TILE;
+    const bool q = i < n && pos[i].w != 0.0;
+    const unsigned long long m = __ballot(q);
+    if (tile < ntiles && (threadIdx.x & (TILE - 1)) == 0) tchg[tile] = m != 0ull ? 1 : 0;
+}
+
 // ---------------------------------------------------------------------------
 // Cluster lists.  A cluster is 16 consecutive sorted slots; its list holds
 // the source slots whose minimum-image distance to the cluster's AABB is
@@ -1024,10 +1039,13 @@ __device__ __forceinline__ void clist_scan(const double4 *__restrict__ pos, int 
 // own cluster against every staged slot, so one read of a source tile serves
 // the tile's four clusters (staged once per workgroup as clist_stage
 // offsets).  Also records the positions the list was built from (xref).
+// tchg[t]: whether tile t holds a charged particle (k_tile_charge); a tile
+// pair not charged on both sides is culled at the nuclear radius (lc.rl2n).
+// nullptr: every tile counts as charged.
 static __global__ __launch_bounds__(BLOCK) void k_clist_build(
     const double4 *__restrict__ pos, const TileBox *__restrict__ box, ListConsts lc,
     int segcap, const unsigned long long *__restrict__ maxdisp, unsigned *__restrict__ clist,
-    int2 *__restrict__ ccounts, double4 *__restrict__ xref)
+    int2 *__restrict__ ccounts, double4 *__restrict__ xref, const int *__restrict__ tchg)
 {
     if (!lists_stale(maxdisp, lc.skin)) return;   // lists still valid (uniform)
     constexpr int W = BLOCK / TILE;
@@ -1066,7 +1084,8 @@ static __global__ __launch_bounds__(BLOCK) void k_clist_build(
     // candidate source tiles of the whole tile, in ascending order
     int ncand = 0;
     {
-        const double rl = __builtin_sqrt(lc.rl2);
+        const double rl = __builtin_sqrt(lc.rl2), rln = __builtin_sqrt(lc.rl2n);
+        const bool tch = tchg ? tchg[t] != 0 : true;
         const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
         for (int tb = 0; tb < lc.ntiles; tb += TILE * NSEG) {
             const int st = tb + lane * NSEG + seg;
@@ -1081,7 +1100,8 @@ static __global__ __launch_bounds__(BLOCK) void k_clist_build(
                     gk = gk > 0.0 ? gk : 0.0;
                     gap2 = __builtin_fma(gk, gk, gap2);
                 }
-                tcand = gap2 <= rl * rl;
+                const bool both = tch && (tchg ? tchg[st] != 0 : true);
+                tcand = gap2 <= (both ? rl * rl : rln * rln);
             }
             const unsigned long long tm = __ballot(tcand);
             if (wv == 0 && tcand) ctiles[ncand + __popcll(tm & lt)] = st;

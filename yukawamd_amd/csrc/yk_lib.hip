@@ -92,6 +92,7 @@ struct yk_system {
     unsigned *clist = nullptr;               // [ncl][n]
     int2 *ccounts = nullptr;                 // [ncl] (fast, general)
     int *corder = nullptr;                   // [ncl] clusters by list length
+    int *tchg = nullptr;                     // [ntiles] tile holds a charged particle
     int segcap = 0;                          // entries per cluster-list segment
     bool corder_stale = true;                // slots moved: order to be recomputed
     double4 *xref = nullptr;                 // positions at the last build
@@ -284,6 +285,8 @@ ListConsts list_consts(const yk_system *s)
     }
     double rl = list_cutoff(p) + s->skin + margin;
     l.rl2 = rl * rl;
+    const double rln0 = list_cutoff_neutral(p) + s->skin + margin;
+    l.rl2n = rln0 * rln0;
     l.skin = s->skin;
     l.n = s->n;
     l.ntiles = s->ntiles;
@@ -503,7 +506,7 @@ int launch_list_build(yk_system *s)
     if (s->use_clist) {
         hipLaunchKernelGGL(k_clist_build, dim3(s->ntiles, NSEG), dim3(BLOCK), 0, s->stream,
                            s->pos, s->box, lc, s->segcap, s->maxdisp, s->clist, s->ccounts,
-                           s->xref);
+                           s->xref, s->tchg);
         // the cluster order is refreshed with the build that follows a re-sort
         // (or a new state); between re-sorts clusters keep their particles and
         // their list lengths change slowly, so the order stays balanced
@@ -725,6 +728,9 @@ int resort(yk_system *s)
     std::swap(s->orig, s->orig2);
     hipLaunchKernelGGL(k_tile_boxes, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos, n,
                        s->ntiles, s->box);
+    if (s->tchg)
+        hipLaunchKernelGGL(k_tile_charge, dim3(s->nblocks), dim3(BLOCK), 0, s->stream, s->pos,
+                           n, s->ntiles, s->tchg);
     YK_HIP(hipGetLastError());
     return invalidate_list(s);   // slots moved: the lists refer to old slots
 }
@@ -866,7 +872,7 @@ void release(yk_system *s)
                       (void *)s->keys,
                       (void *)s->keys2, (void *)s->vals, (void *)s->vals2, s->sort_tmp,
                       (void *)s->list, (void *)s->counts, (void *)s->xref,
-                      (void *)s->clist, (void *)s->ccounts, (void *)s->corder,
+                      (void *)s->clist, (void *)s->ccounts, (void *)s->corder, (void *)s->tchg,
                       (void *)s->maxdisp, (void *)s->swept})
         if (ptr) (void)hipFree(ptr);
     if (s->h_stage) (void)hipHostFree(s->h_stage);
@@ -1138,6 +1144,10 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
                               sizeof(unsigned) * (size_t)s->ncl * NSEG * s->segcap));
             YK_HIPB(hipMalloc(&s->ccounts, sizeof(int2) * (size_t)s->ncl * NSEG));
             YK_HIPB(hipMalloc(&s->corder, sizeof(int) * (size_t)s->ncl));
+            YK_HIPB(hipMalloc(&s->tchg, sizeof(int) * (size_t)s->ntiles));
+            // nonzero (every tile charged, the conservative cull) until the
+            // first re-sort computes the flags
+            YK_HIPB(hipMemsetAsync(s->tchg, 1, sizeof(int) * (size_t)s->ntiles, s->stream));
             if (s->list_M >= PART_REDUCE_MIN_M)
                 YK_HIPB(hipMalloc(&s->psum, sizeof(double) * 11 * nn));
         }
