@@ -99,6 +99,9 @@ def parse():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the pure-ion (N=3,456) and nucleon (N=51,200) lines")
     ap.add_argument("--config-steps", type=int, default=100)
+    ap.add_argument("--config-only", default=None, choices=["pure_ion_3456", "nucleon_51200"],
+                    help="only run that config's line (both forms) and print it as the JSON "
+                         "line (for kernel traces of one workload)")
     ap.add_argument("--no-drift", action="store_true",
                     help="skip the energy drift over 1,000 steps (mixture and nucleons)")
     ap.add_argument("--no-serve", action="store_true")
@@ -282,6 +285,28 @@ def copy_calibration(mib):
     return 0
 
 
+def config_line(wc, args, steps):
+    fl = FLOPS_PER_PAIR_PURE if wc.params.sim_type == "pure-ion" else None
+    flm = FLOPS_PER_PAIR_PURE_MEAS if fl else None
+    out = {"n": wc.n, "steps": steps, "dt_fm_per_c": wc.dt,
+           "in_cutoff_fraction": in_cutoff_fraction(wc.params)}
+    for prec in args.forms.split(","):
+        out[prec] = form_figures(wc, prec, steps, args.warmup, args.nind, fl, flm)
+        if fl:
+            out[prec]["roofline_frac_fp64"] = out[prec]["force_kernel_tflops"] / PEAK_FP64_TFLOPS
+    return out
+
+
+def config_only(args):
+    from yukawamd_amd import workloads as W
+    if args.config_only == "pure_ion_3456":
+        wc, steps = W.pure_ion(), args.config_steps
+    else:
+        wc, steps = W.nucleon(), max(args.config_steps // 2, 10)
+    print(json.dumps({"config_only": wc.name, wc.name: config_line(wc, args, steps)}))
+    return 0
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -293,6 +318,8 @@ def main():
               file=sys.stderr)
         return 2
     from yukawamd_amd import workloads as W
+    if args.config_only:
+        return config_only(args)
     w = W.ion_mixture()
     n = w.n
     pairs = n * (n - 1) // 2
@@ -326,16 +353,9 @@ def main():
                                    / forms["fp64"]["pair_interactions_per_s"])
     if not args.no_configs:
         cfg = {}
-        for wc, fl, flm, steps in ((W.pure_ion(), FLOPS_PER_PAIR_PURE, FLOPS_PER_PAIR_PURE_MEAS,
-                                    args.config_steps),
-                                   (W.nucleon(), None, None, max(args.config_steps // 2, 10))):
-            cfg[wc.name] = {"n": wc.n, "steps": steps, "dt_fm_per_c": wc.dt,
-                            "in_cutoff_fraction": in_cutoff_fraction(wc.params)}
-            for prec in args.forms.split(","):
-                cfg[wc.name][prec] = form_figures(wc, prec, steps, args.warmup, args.nind, fl, flm)
-                if fl:
-                    cfg[wc.name][prec]["roofline_frac_fp64"] = (
-                        cfg[wc.name][prec]["force_kernel_tflops"] / PEAK_FP64_TFLOPS)
+        for wc, steps in ((W.pure_ion(), args.config_steps),
+                          (W.nucleon(), max(args.config_steps // 2, 10))):
+            cfg[wc.name] = config_line(wc, args, steps)
         line["configs"] = cfg
     if not args.no_drift:
         line["energy_drift_1000_steps"] = {
