@@ -892,8 +892,9 @@ __device__ __forceinline__ ClusterView cluster_view(const double c[3], const dou
 
 // The candidate test of the staged source s against the cluster box (FP32
 // with the widened bounds of ListConsts): cand, the source is within the list
-// radius (v.rl2c or, for a neutral source, v.rl2n); fast, its staged image serves every target of the cluster for the
-// list's lifetime; and the same as this wave's lane masks mcand, mfast.  Each
+// radius (v.rl2c or, for a neutral source, v.rl2n); fast, its staged image
+// serves every target of the cluster for the list's lifetime; and the same as
+// this wave's lane masks mcand, mfast.  Each
 // mask is the ballot of a single compare (which compiles to the compare's own
 // result), combined with scalar ANDs.  Absent sources fail at tiers 1 and 2
 // (FAR_AWAY); tier 0 callers test presence.
@@ -1638,7 +1639,7 @@ struct DriftArgs {
 // order t = 0 .. T-1.  out[k][n] is read by k_finalize as one partial
 // (LAYOUT 0, nparts = 1); the order is fixed, so results are deterministic.
 template <int NC, int T>
-__global__ __launch_bounds__(CL *T) void k_part_reduce(const double *__restrict__ part, int M,
+__global__ __launch_bounds__(CL * T) void k_part_reduce(const double *__restrict__ part, int M,
                                                        int n, double *__restrict__ out)
 {
     static_assert(T >= 4 && CL * T <= 1024, "T sub-sums per slot");
