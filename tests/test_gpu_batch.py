@@ -50,6 +50,28 @@ def test_batch_fp64_accel_matches_oracle(mode):
 
 
 @pytest.mark.parametrize("mode", list(MODES))
+def test_unscreened_mixed_replica(mode):
+    """A mixed-precision replica with mu = 0 (plain Coulomb): the list items'
+    far pairs work in lengths scaled by mu log2 e, so a list replica runs in
+    FP64 (FP64 bar); tile items keep the mixed pair (mixed bar)."""
+    w = W.replica(14, 600, True, 170.0, 1.0, "mixed")
+    with Ensemble(capacity=4096, max_replicas=2, tuning=MODES[mode]) as e:
+        rid = e.add_workload(w, steps=0, xmuc=0.0)
+        e.run(0)
+        g = e.get(rid)
+    p = w.params.replace(xmuc=0.0, precision="fp64")
+    a_o, m_o = oracle.accel(p, w.x, w.Z, w.A, True, ref_f1=0)
+    err = np.abs(g["a"] - a_o).max() / np.abs(a_o).max()
+    print(mode, "mu = 0 mixed replica: max |da|/max|a|", err)
+    if mode == "lists":
+        assert err <= 1e-12
+        assert g["init"]["ev"] == pytest.approx(m_o["ev"], rel=1e-13)
+    else:
+        assert err <= 1e-5
+        assert g["init"]["ev"] == pytest.approx(m_o["ev"], rel=1e-6)
+
+
+@pytest.mark.parametrize("mode", list(MODES))
 def test_replica_alone_equals_in_mix(mode):
     target = W.replica(21, 1500, True, 180.0, 0.9, "mixed")
     others = [W.replica(22 + k, n, k % 2 == 0, 170.0, 1.3, "mixed")

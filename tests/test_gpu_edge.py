@@ -12,6 +12,7 @@
 * A cutoff of 1e8 fm, past the kernels' 1e6/mu limit, and screening so strong
   that mu r exceeds that limit inside the cutoff (exact zeros).
 * The 'B-field' sim type: accelerations exactly zero (src/int_control.f:71-72).
+* Unscreened Coulomb (mu = 0), FP64 and mixed.
 
 Every case runs the cluster lists, the 64-slot tile lists and the culled
 all-pairs kernel, FP64, measuring and force-only, with the bars of the
@@ -111,6 +112,27 @@ def test_exponent_beyond_range_contributes_zero(path):
     assert np.array_equal(a_o, np.zeros_like(a_o)) and np.array_equal(a_g, a_o)
     assert m_o["ev"] == 0.0 and m_g["ev"] == 0.0
     assert m_g["px"] == pytest.approx(p.rho * p.kT, rel=1e-15)
+
+
+@pytest.mark.parametrize("path", ["clist", "tilelist", "allpairs"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_unscreened_coulomb(path, precision):
+    """mu = 0 (plain Coulomb with the energy shift 1/rc): the cluster-list ion
+    forms work in lengths scaled by mu, so they take the unscaled FP64 form
+    there, also when mixed precision is asked for (FP64 bars); the mixed form
+    of the tile lists and the all-pairs kernel keeps the mixed bars."""
+    p, x, Z, A = mixture(900, (180.0, 180.0, 180.0), 70.0, seed=12)
+    p = p.replace(xmuc=0.0, precision=precision)
+    a_o, m_o = oracle.accel(p.replace(precision="fp64"), x, Z, A, True, ref_f1=0)
+    a_g, m_g = gpu_accel(p, x, Z, A, True, path)
+    rel, cond = force_errors(p, a_g, a_o, x, Z, A)
+    print(f"mu = 0 {path} {precision}: max rel {rel:.2e} cond {cond:.2e}")
+    if precision == "fp64" or path == "clist":
+        assert cond <= 1e-12 and rel <= 1e-10
+        check_meas(m_g, m_o)
+    else:
+        assert cond <= 1e-6 and rel <= 1e-4
+        assert m_g["ev"] == pytest.approx(m_o["ev"], rel=1e-6)
 
 
 def test_bfield_sim_type_has_no_pair_force():

@@ -188,7 +188,9 @@ RepConsts rep_consts(const HostReplica &h)
     c.three_n = 3.0f * (float)h.n;
     c.n_f = (float)h.n;
     c.kind = (p.sim_type == YK_PURE_ION) ? PURE : MIX;
-    c.prec = p.precision == YK_MIXED ? 1 : 0;
+    // the mixed list items work in lengths scaled by mu log2 e: an unscreened
+    // (mu = 0) list replica is computed in FP64
+    c.prec = (p.precision == YK_MIXED && (p.xmuc > 0.0 || h.list_items == 0)) ? 1 : 0;
     c.off = h.off;
     c.n = h.n;
     c.tile0 = h.off / TILE;

@@ -459,6 +459,11 @@ int read_swept(yk_system *s, unsigned long long *out)
 template <int KIND, int PREC>
 void launch_pairs_k(yk_system *s, const ForceConsts &c, bool meas)
 {
+    if constexpr (PREC == 1 && KIND != NN) {
+        // the mixed ion form's far pairs on the cluster lists work in lengths
+        // scaled by mu log2 e: unscreened Coulomb (mu = 0) is computed in FP64
+        if (s->use_clist && !(c.mu > 0.0)) return launch_pairs_k<KIND, 0>(s, c, meas);
+    }
     if (s->use_clist) {
         const ListConsts lc = list_consts(s);
         const int items = (s->ncl + CPW - 1) / CPW * s->list_M;
