@@ -50,7 +50,7 @@ def abs_pair_force_sum_np(p, x, Z):
 
 
 def gpu_accel(p, x, Z, A, do_meas=True, path="clist"):
-    """path: 'clist' (cluster lists, the default for N <= 65,536), 'tilelist'
+    """path: 'clist' (cluster lists, the default for N <= 262,144), 'tilelist'
     (64-particle tile lists only) or 'allpairs' (culled tile pairs)."""
     p = p.replace(allpairs=path == "allpairs", tile_lists=path == "tilelist")
     with YukawaSystem(p, Z=Z, A=A) as s:

@@ -188,6 +188,65 @@ def test_item_partial_sums_against_oracle(n):
         assert r1 == r2
 
 
+def test_cluster_lists_above_65536():
+    """A single system of 100,000 particles runs on the cluster lists (6,250
+    clusters: after a re-sort the cluster order comes from a device radix sort
+    instead of k_corder's one-workgroup bitonic sort).  A run with re-sorts
+    every 5 steps equals single newton steps bit for bit, and the forces at
+    its end match the oracle (OpenMP restatement) and the tile lists."""
+    w = W.ion_mixture(n=100_000, seed=96)
+    p = w.params.replace(resort_every=5)
+    out = []
+    for loop in (False, True):
+        with YukawaSystem(p, Z=w.Z, A=w.A, dt=w.dt) as s:
+            s.put_state(w.x, w.v)
+            s.accel(True)
+            if loop:
+                recs = s.run(12, nind=6)
+            else:
+                recs = [s.newton(st % 6 == 0) for st in range(1, 13)]
+                recs = [r for st, r in enumerate(recs, 1) if st % 6 == 0]
+            out.append((s.get_state(), recs))
+    (sa, ra), (sb, rb) = out
+    for u, v in zip(sa, sb):
+        assert np.array_equal(u, v)
+    assert len(ra) == len(rb) == 2 and ra == rb
+    x, _, a = sa
+    a_o, _ = oracle.accel_omp(p, x, w.Z, w.A)
+    # accel_omp keeps the reference's last-particle behaviour (F1): the last
+    # particle's force lacks its own factor Z
+    a_o[-1] *= w.Z[-1]
+    cond_sum = oracle.pair_abs_sum(p, x, w.Z)[0]
+    a_t, _ = gpu_accel(p, x, w.Z, w.A, False, "tilelist")
+    for name, a_ref in (("oracle", a_o), ("tile lists", a_t)):
+        rel, cond = force_errors(p, a, a_ref, x, w.Z, w.A, cond_sum)
+        print(f"n=100000 cluster lists vs {name}: max rel {rel:.2e} cond {cond:.2e}")
+        assert cond <= 1e-12 and rel <= 1e-10
+
+
+def test_cluster_lists_at_the_size_limit():
+    """262,144 particles, the largest single system on cluster lists: 2^32
+    list entries (16 GiB), so segment offsets pass 2^31 entries (a regression
+    test for their widening in the build kernel).  Forces and measurements
+    against the tile lists; the total force vanishes (Newton's third law)."""
+    w = W.ion_mixture(n=262_144, seed=97)
+    p = w.params
+    a_c, m_c = gpu_accel(p, w.x, w.Z, w.A, True, "clist")
+    a_t, m_t = gpu_accel(p, w.x, w.Z, w.A, True, "tilelist")
+    m = masses(p, w.A)[:, None]
+    dF = np.linalg.norm((a_c - a_t) * m, axis=1)
+    F = np.linalg.norm(a_t * m, axis=1)
+    rel = (dF / F).max()
+    print(f"n=262144 cluster lists vs tile lists: max |dF|/|F| {rel:.2e}")
+    assert rel <= 1e-10
+    assert m_c["ev"] == pytest.approx(m_t["ev"], rel=1e-12)
+    assert m_c["px"] == pytest.approx(m_t["px"], rel=1e-12)
+    diag = max(abs(m_t["pp"][k]) for k in (0, 3, 5))
+    assert np.abs(np.array(m_c["pp"]) - np.array(m_t["pp"])).max() <= 1e-12 * diag
+    Fc = a_c * m
+    assert np.abs(Fc.sum(0)).max() <= 1e-12 * np.abs(Fc).sum()
+
+
 def test_force_independent_of_measurement():
     """Unlike the reference's two branches (F2), the device force does not depend
     on do_measurements: both templates accumulate it with the same FMAs."""

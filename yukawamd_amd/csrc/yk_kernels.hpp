@@ -1002,9 +1002,12 @@ __device__ __forceinline__ void clist_scan(const double4 *__restrict__ pos, int 
     };
     auto load = [&](int j) -> double4 { return (j < n) ? pos[j] : make_double4(0, 0, 0, 0); };
     const int nrounds = (ncand + W - 1) / W;
-    // wave-uniform segment offset (scalar registers)
-    seg_off = ((size_t)__builtin_amdgcn_readfirstlane((unsigned)(seg_off >> 32)) << 32) |
-              __builtin_amdgcn_readfirstlane((unsigned)seg_off);
+    // wave-uniform segment offset (scalar registers); readfirstlane returns
+    // int, so each half is widened as unsigned (a low half of 2^31 or more
+    // sign-extended would put the segment 16 EiB away)
+    const unsigned off_hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(seg_off >> 32));
+    const unsigned off_lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)seg_off);
+    seg_off = ((size_t)off_hi << 32) | (size_t)off_lo;
     int ja = slot_of(0), jb = slot_of(1);
     double4 pa = load(ja), pb = load(jb);
     unsigned uf = 0, ug = 0;
@@ -1051,7 +1054,7 @@ static __global__ __launch_bounds__(BLOCK) void k_clist_build(
     if (!lists_stale(maxdisp, lc.skin)) return;   // lists still valid (uniform)
     constexpr int W = BLOCK / TILE;
     __shared__ float4 stage[2][W][TILE];
-    __shared__ int ctiles[CLIST_MAX_N / TILE / NSEG + 1];
+    __shared__ int ctiles[CLIST_MAX_N_SYSTEM / TILE / NSEG + 1];
     const int t = blockIdx.x;
     const int seg = blockIdx.y;   // source tiles st with st % NSEG == seg
     const int lane = threadIdx.x & (TILE - 1), wv = threadIdx.x / TILE;
@@ -1119,7 +1122,8 @@ static __global__ __launch_bounds__(BLOCK) void k_clist_build(
 
 // Cluster order for the pair kernel: by list length, longest first (ties by
 // index), so the four clusters a wave takes have nearly equal lists.  One
-// workgroup of 1024 threads, bitonic sort of up to 4096 keys in LDS.
+// workgroup of 1024 threads, bitonic sort of up to 4096 keys in LDS; larger
+// systems sort k_corder_keys' keys with a device-wide radix sort.
 constexpr int CORDER_THREADS = 1024;
 constexpr int CORDER_MAX = 4096;
 static __global__ __launch_bounds__(CORDER_THREADS) void k_corder(
@@ -1158,6 +1162,26 @@ static __global__ __launch_bounds__(CORDER_THREADS) void k_corder(
     }
     for (int k = threadIdx.x; k < ncl; k += CORDER_THREADS) order[k] = (int)(key[k] & 0xfffu);
 }
+
+// The same order for more than CORDER_MAX clusters: key 2^18 - 1 - list
+// length (clamped) and value the cluster index, for a stable radix sort over
+// 18 bits (ties keep index order, as in k_corder).
+static __global__ __launch_bounds__(BLOCK) void k_corder_keys(const int2 *__restrict__ ccounts,
+                                                              int ncl,
+                                                              unsigned *__restrict__ key,
+                                                              int *__restrict__ val)
+{
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= ncl) return;
+    unsigned len = 0;
+    for (int sg = 0; sg < NSEG; ++sg) {
+        const int2 cc = ccounts[k * NSEG + sg];
+        len += (unsigned)(cc.x + cc.y);
+    }
+    key[k] = 0x3ffffu - (len > 0x3ffffu ? 0x3ffffu : len);
+    val[k] = k;
+}
+constexpr int CORDER_KEY_BITS = 18;
 
 // A cluster's list as its NSEG segments, read back entries first: each
 // segment's region holds its fast entries from the front and the others

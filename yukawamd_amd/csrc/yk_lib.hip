@@ -516,8 +516,23 @@ int launch_list_build(yk_system *s)
         // (or a new state); between re-sorts clusters keep their particles and
         // their list lengths change slowly, so the order stays balanced
         if (s->corder_stale) {
-            hipLaunchKernelGGL(k_corder, dim3(1), dim3(CORDER_THREADS), 0, s->stream,
-                               s->ccounts, s->ncl, lc, s->maxdisp, s->corder);
+            if (s->ncl <= CORDER_MAX) {
+                hipLaunchKernelGGL(k_corder, dim3(1), dim3(CORDER_THREADS), 0, s->stream,
+                                   s->ccounts, s->ncl, lc, s->maxdisp, s->corder);
+            } else {
+                // the order is stale only after invalidate_list, which also
+                // forces this build, so the counts are fresh here; the Hilbert
+                // sort's buffers are free between re-sorts
+                hipLaunchKernelGGL(k_corder_keys, dim3((s->ncl + BLOCK - 1) / BLOCK),
+                                   dim3(BLOCK), 0, s->stream, s->ccounts, s->ncl, s->keys,
+                                   s->vals);
+                YK_HIP(hipGetLastError());
+                size_t bytes = s->sort_tmp_bytes;
+                YK_HIP(hipcub::DeviceRadixSort::SortPairs(s->sort_tmp, bytes, s->keys,
+                                                          s->keys2, s->vals, s->corder,
+                                                          s->ncl, 0, CORDER_KEY_BITS,
+                                                          s->stream));
+            }
             s->corder_stale = false;
         }
     } else {
@@ -1123,7 +1138,7 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
     s->use_list = !(params->flags & YK_FLAG_ALLPAIRS);
     // items per target tile (cluster set)
     s->list_M = (params->precision == YK_MIXED) ? LIST_ITEMS_MIXED : LIST_ITEMS_FP64;
-    if (s->use_list && !(params->flags & YK_FLAG_TILE_LISTS) && n <= CLIST_MAX_N) {
+    if (s->use_list && !(params->flags & YK_FLAG_TILE_LISTS) && n <= CLIST_MAX_N_SYSTEM) {
         // small systems: more items per cluster set, up to 128, until the
         // launch has 8,192 waves (N = 3,456: 128 items, 6,912 waves, 30 us
         // per force launch against 33 us with 32 items)
@@ -1138,7 +1153,7 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
         s->part_doubles = std::max(s->part_doubles,
                                    (size_t)s->ntiles * s->list_M * 11 * TILE);
         YK_HIPB(hipMalloc(&s->xref, sizeof(double4) * nn));
-        s->use_clist = !(params->flags & YK_FLAG_TILE_LISTS) && n <= CLIST_MAX_N;
+        s->use_clist = !(params->flags & YK_FLAG_TILE_LISTS) && n <= CLIST_MAX_N_SYSTEM;
         if (!s->use_clist) {
             YK_HIPB(hipMalloc(&s->list, sizeof(unsigned) * (size_t)s->ntiles * nn));
             YK_HIPB(hipMalloc(&s->counts, sizeof(int2) * (size_t)s->ntiles));
@@ -1178,6 +1193,13 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
     YK_HIPB(hipcub::DeviceRadixSort::SortPairs(nullptr, s->sort_tmp_bytes, s->keys, s->keys2,
                                                s->vals, s->vals2, n, 0, 3 * s->grid_bits + 1,
                                                s->stream));
+    if (s->use_clist && s->ncl > CORDER_MAX) {   // and the cluster order's sort
+        size_t cbytes = 0;
+        YK_HIPB(hipcub::DeviceRadixSort::SortPairs(nullptr, cbytes, s->keys, s->keys2, s->vals,
+                                                   s->corder, s->ncl, 0, CORDER_KEY_BITS,
+                                                   s->stream));
+        s->sort_tmp_bytes = std::max(s->sort_tmp_bytes, cbytes);
+    }
     YK_HIPB(hipMalloc(&s->sort_tmp, s->sort_tmp_bytes > 0 ? s->sort_tmp_bytes : 16));
     YK_HIPB(hipHostMalloc(&s->h_stage, sizeof(double) * 4 * nn, hipHostMallocDefault));
     YK_HIPB(hipHostMalloc(&s->h_orig, sizeof(int) * nn, hipHostMallocDefault));
