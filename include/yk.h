@@ -308,10 +308,9 @@ int yk_batch_remove(yk_batch *b, int32_t id);
  * the persistent launch takes the waves of the bucket that holds the largest
  * share of the pair work (sum of n^2) among the active replicas.  A replica
  * keeps the list / tile choice it was admitted with.  Without a table:
- * lists with 16 items per tile up to 65,536 particles.  Returns
- * YK_ERR_UNSUPPORTED for shapes that are not offered: waves_per_block 1,
- * list_items 2 or 4, and tile items (list_items 0) of 2, 16, 32 or 64 source
- * tiles. */
+ * lists with 16 items per tile up to 65,536 particles.  Returns YK_ERR_ARG
+ * for chunk_tiles < 1, list_items outside [0, 1024], another waves_per_block
+ * or n_upper not increasing. */
 int yk_batch_set_tuning(yk_batch *b, int32_t nbuckets, const int32_t *n_upper,
                         const int32_t *chunk_tiles, const int32_t *waves_per_block,
                         const int32_t *list_items);

@@ -28,9 +28,11 @@ DEPS = [SRC, CSRC / "yk_batch_plan.hpp", CSRC / "yk_layout.hpp"]
 
 ADD, REMOVE, RUN = 1, 2, 3
 
-# The candidate grid of the sweep that ended in the GPU memory fault: waves
-# 1, 2, 4; tile items of 2-128 source tiles (at most 64 items per target
-# tile); 2, 4, 8 or 16 list items per tile up to 65,536 particles.
+# The candidate grid of the sweep that ended in a GPU memory fault (the list
+# build's segment offsets sign-extended past 2^31 list entries, a device-code
+# conversion outside what this check restates): waves 1, 2, 4; tile items of
+# 2-128 source tiles (at most 64 items per target tile); 2, 4, 8 or 16 list
+# items per tile up to 65,536 particles.
 FAULT_SWEEP_WAVES = [1, 2, 4]
 FAULT_SWEEP_CHUNKS = [2, 4, 8, 16, 32, 64, 128]
 FAULT_SWEEP_LIST_ITEMS = [2, 4, 8, 16]

@@ -69,7 +69,7 @@ def test_current_candidates_in_bounds():
 
 def test_committed_table_on_the_ensemble_draw():
     """The committed tuning table on 24 replicas of the ensemble draw plus a
-    replica above 65,536 particles (tile items of 128 source tiles), 25 steps
+    replica above 65,536 particles (tile items), 25 steps
     with re-sorts every 10, measuring every 10th."""
     table = json.loads(TUNING_FILE.read_text())["buckets"]
     reps = W.ensemble(24) + [W.replica(77, 70000, True, 180.0, 1.0, "mixed")]

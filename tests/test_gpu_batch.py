@@ -5,7 +5,10 @@
 * a replica gives bitwise the same trajectory and records alone and inside a
   mix of other replicas;
 * the batch matches the single-system path to summation-order rounding;
-* step budgets, removal/re-admission and the capacity limit behave.
+* step budgets, removal/re-admission and the capacity limit behave;
+* work shapes outside the committed table reproduce the oracle;
+* a list arena past 2^31 entries gives the replica above that line the same
+  trajectory as alone.
 """
 import numpy as np
 import pytest
@@ -186,6 +189,28 @@ def test_staggered_admission_with_resort_is_bitwise(mode):
         assert np.array_equal(mixed_in[key], alone[key]), key
 
 
+def test_list_arena_beyond_2_31_entries():
+    """Nine replicas of 65,536 particles on cluster lists hold 9 x 2^28 list
+    entries: the ninth replica's lists start at entry 2^31, where the build
+    kernel's segment offsets need their full 64-bit widening (a sign-extended
+    low half addressed 16 EiB away and faulted).  The ninth replica matches
+    itself alone, whose lists start at entry 0, bit for bit."""
+    reps = [W.replica(91 + k, 65536, k % 2 == 0, 175.0, 1.0, "mixed") for k in range(9)]
+    tuning = [{"n_upper": 1 << 20, "chunk_tiles": 8, "waves_per_block": 4, "list_items": 8}]
+    with Ensemble(capacity=9 * 65536, max_replicas=9, max_records=2, tuning=tuning) as e:
+        ids = [e.add_workload(w, steps=4) for w in reps]
+        e.run(4, nind=2)
+        last = e.get(ids[-1])
+        assert all(e.status(r)[0] == 0 for r in ids)
+    with Ensemble(capacity=65536, max_replicas=1, max_records=2, tuning=tuning) as e:
+        rid = e.add_workload(reps[-1], steps=4)
+        e.run(4, nind=2)
+        alone = e.get(rid)
+    for key in ("x", "v", "a"):
+        assert np.array_equal(last[key], alone[key]), key
+    assert last["records"] == alone["records"] and len(last["records"]) == 2
+
+
 @pytest.mark.parametrize("shape", [
     {"list_items": 8, "chunk_tiles": 8, "waves_per_block": 1},
     {"list_items": 2, "chunk_tiles": 8, "waves_per_block": 4},
@@ -194,16 +219,23 @@ def test_staggered_admission_with_resort_is_bitwise(mode):
     {"list_items": 0, "chunk_tiles": 16, "waves_per_block": 4},
     {"list_items": 0, "chunk_tiles": 32, "waves_per_block": 2},
     {"list_items": 0, "chunk_tiles": 64, "waves_per_block": 4},
+    {"list_items": 32, "chunk_tiles": 8, "waves_per_block": 8},
 ])
-def test_refused_tuning_shapes(shape):
-    """Shapes only an earlier tuning sweep ran (one of whose ensembles ended in
-    a GPU memory fault of undetermined cause) are refused with
-    YK_ERR_UNSUPPORTED; the table in force is left unchanged."""
-    with Ensemble(capacity=1024, max_replicas=1, tuning=MODES["tiles"]) as e:
-        with pytest.raises(YkError) as err:
-            e.set_tuning([dict(shape, n_upper=4096)])
-        assert err.value.code == 6
-        w = W.replica(81, 500, True, 175.0, 1.0, "fp64")
-        rid = e.add_workload(w, steps=2)
-        e.run(2)
-        assert e.status(rid)[0] == 0
+def test_tuning_shapes_against_oracle(shape):
+    """Shapes outside the committed table: 1- and 8-wave workgroups, 2, 4 and
+    32 list items per tile, tile items of 2 to 64 source tiles.  FP64
+    replicas against the oracle (condition-free 1e-12 of the largest
+    acceleration) and a step budget run out."""
+    reps = small_set()
+    with Ensemble(capacity=4096, max_replicas=4, tuning=[dict(shape, n_upper=1 << 20)]) as e:
+        ids = [e.add_workload(w, steps=3) for w in reps]
+        e.run(0)
+        for rid, w in zip(ids, reps):
+            g = e.get(rid)
+            a_o, m_o = oracle.accel(w.params, w.x, w.Z, w.A, True, ref_f1=0)
+            err = np.abs(g["a"] - a_o).max() / np.abs(a_o).max()
+            print(shape, w.name, w.n, "max |da|/max|a|", err)
+            assert err <= 1e-12
+            assert g["init"]["ev"] == pytest.approx(m_o["ev"], rel=1e-13)
+        e.run(3)
+        assert all(e.status(r)[0] == 0 for r in ids)

@@ -1,35 +1,38 @@
 """The batched engine's tuning table and the autotuner's candidate grid hold
-no shape that yk_batch_set_tuning refuses (CPU only)."""
+only shapes that yk_batch_set_tuning accepts (CPU only)."""
 import json
 
 from yukawamd_amd import autotune
 from yukawamd_amd.batch import TUNING_FILE
 
 
-def test_committed_table_has_only_offered_shapes():
+def accepted(b):
+    """yk_batch_set_tuning's argument rules."""
+    return (b["chunk_tiles"] >= 1 and 0 <= b["list_items"] <= 1024
+            and b["waves_per_block"] in (1, 2, 4, 8))
+
+
+def test_committed_table_has_only_accepted_shapes():
     table = json.loads(TUNING_FILE.read_text())["buckets"]
     assert table
+    uppers = [b["n_upper"] for b in table]
+    assert uppers == sorted(set(uppers))
     for b in table:
-        assert not autotune.disabled_shape(b["list_items"], b["chunk_tiles"],
-                                           b["waves_per_block"]), b
+        assert accepted(b), b
+        if b["n_upper"] > autotune.CLIST_MAX_N:
+            assert b["list_items"] == 0, b
 
 
-def test_candidates_exclude_refused_shapes():
+def test_candidates():
     for hi in autotune.DEFAULT_BUCKETS:
         cands = autotune.candidates(hi)
         assert cands, hi
+        nt = (hi + 63) // 64
         for c in cands:
-            assert not autotune.disabled_shape(c["list_items"], c["chunk_tiles"],
-                                               c["waves_per_block"]), (hi, c)
+            assert accepted(c), (hi, c)
+            if c["list_items"] == 0:
+                assert -(-nt // c["chunk_tiles"]) <= autotune.MAX_ITEMS_PER_TILE, (hi, c)
             if hi > autotune.CLIST_MAX_N:
                 assert c["list_items"] == 0
-
-
-def test_disabled_shape_rule():
-    assert autotune.disabled_shape(8, 8, 1)
-    assert autotune.disabled_shape(2, 8, 4) and autotune.disabled_shape(4, 8, 4)
-    for ct in (2, 16, 32, 64):
-        assert autotune.disabled_shape(0, ct, 4)
-    assert not autotune.disabled_shape(8, 16, 4)    # chunk_tiles unused by list items
-    assert not autotune.disabled_shape(0, 128, 4)
-    assert not autotune.disabled_shape(16, 8, 2)
+        keys = [tuple(sorted(c.items())) for c in cands]
+        assert len(keys) == len(set(keys)), hi

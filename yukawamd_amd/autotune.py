@@ -37,8 +37,8 @@ import numpy as np
 
 DEFAULT_BUCKETS = [2048, 4096, 8192, 16384, 65536, 262144]
 WAVES = [2, 4, 8]
-CHUNKS = [4, 8, 128]
-LIST_ITEMS = [8, 16, 32]
+CHUNKS = [4, 8, 16, 32, 64, 128]
+LIST_ITEMS = [4, 8, 16, 32]
 CLIST_MAX_N = 65536
 MAX_ITEMS_PER_TILE = 64
 BUCKET_SLOTS = 1 << 20          # particles per tuning ensemble (at least one replica)
@@ -62,15 +62,6 @@ def bucket_replicas(lo: int, hi: int, seed: int, slots: int = BUCKET_SLOTS):
     return reps
 
 
-def disabled_shape(list_items: int, chunk_tiles: int, waves_per_block: int) -> bool:
-    """Shapes yk_batch_set_tuning refuses with YK_ERR_UNSUPPORTED: 1-wave
-    workgroups, 2 or 4 list items per tile, tile items of 2/16/32/64 source
-    tiles (the shapes only an earlier sweep ran, one of whose ensembles ended
-    in a GPU memory fault of undetermined cause)."""
-    return (waves_per_block == 1 or list_items in (2, 4)
-            or (list_items == 0 and chunk_tiles in (2, 16, 32, 64)))
-
-
 def candidates(n_max: int):
     nt = (n_max + 63) // 64
     out = []
@@ -85,8 +76,7 @@ def candidates(n_max: int):
         for mm in LIST_ITEMS:
             for w in WAVES:
                 out.append({"list_items": mm, "chunk_tiles": 8, "waves_per_block": w})
-    return [c for c in out
-            if not disabled_shape(c["list_items"], c["chunk_tiles"], c["waves_per_block"])]
+    return out
 
 
 def tune_bucket(lo, hi, steps, warmup, seed, log):
@@ -111,22 +101,7 @@ def tune_bucket(lo, hi, steps, warmup, seed, log):
     return best, results, len(reps), pairs
 
 
-SWEEP_DISABLED = (
-    "the candidate sweep is disabled: a sweep over the current kernels ended in a GPU memory "
-    "fault in the accel pass of one of its candidate ensembles, and the faulting shape and its "
-    "cause have not been found (the shapes not exercised elsewhere are 1-wave workgroups, 2 or "
-    "4 list items per tile and tile items of 2, 16, 32 or 64 source tiles).  "
-    "yukawamd_amd/tuning/batch_gfx950.json keeps the table of an earlier sweep; its shapes "
-    "run in the benchmark and the tests.")
-
-
 def main(argv=None):
-    print("autotune: " + SWEEP_DISABLED, file=sys.stderr)
-    return 2
-
-
-def sweep(argv=None):
-    """The candidate sweep (disabled: main() does not call it)."""
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=None)
     ap.add_argument("--steps", type=int, default=8)

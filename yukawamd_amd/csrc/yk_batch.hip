@@ -810,17 +810,6 @@ int yk_batch_set_tuning(yk_batch *b, int32_t nbuckets, const int32_t *n_upper,
             return bfail(YK_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
         if (k > 0 && n_upper[k] <= n_upper[k - 1])
             return bfail(YK_ERR_ARG, "n_upper must increase");
-        // Shapes that only a tuning sweep ran, one of whose ensembles ended in
-        // a GPU memory fault of undetermined cause: refused until that cause
-        // is known.
-        const int ct = chunk_tiles[k], li = list_items[k];
-        if (w == 1 || li == 2 || li == 4 ||
-            (li == 0 && (ct == 2 || ct == 16 || ct == 32 || ct == 64)))
-            return bfail(YK_ERR_UNSUPPORTED,
-                         "bucket %d: waves_per_block=%d list_items=%d chunk_tiles=%d is "
-                         "not offered (1-wave workgroups, 2 or 4 list items and tile items "
-                         "of 2/16/32/64 source tiles are disabled)",
-                         k, w, li, ct);
     }
     b->tune.n_upper.assign(n_upper, n_upper + nbuckets);
     b->tune.chunk_tiles.assign(chunk_tiles, chunk_tiles + nbuckets);
