@@ -108,6 +108,8 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-rate", type=float, default=1.5, help="Poisson arrivals per s")
     ap.add_argument("--serve-round", type=int, default=50, help="steps per serving round")
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0,
+                    help="HBM the serving loop admits requests against")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (reference "
                          "kernel for scaling HBM byte counters) and exit")
@@ -236,7 +238,7 @@ def run_batch(ws, steps, warmup, nind):
     }
 
 
-def run_serving(count, rate, round_steps):
+def run_serving(count, rate, round_steps, budget_gib=64.0):
     """The first `count` requests of the seeded serving stream (seed 1004),
     arriving in real time at `rate` per second, served to completion."""
     from yukawamd_amd.serving import Server, stream, summarize
@@ -244,7 +246,7 @@ def run_serving(count, rate, round_steps):
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    with Server(hbm_budget_bytes=64 * 2 ** 30, round_steps=round_steps, log=log) as srv:
+    with Server(hbm_budget_bytes=budget_gib * 2 ** 30, round_steps=round_steps, log=log) as srv:
         t0 = time.perf_counter()
         res = srv.serve(reqs)
         wall = time.perf_counter() - t0
@@ -257,7 +259,7 @@ def run_serving(count, rate, round_steps):
                        "mixture, N log-uniform [2048, 262144], steps U[100, 2000], Gamma "
                        f"U[150,220], kappa U[0.5,2], 80% mixed / 20% FP64; Poisson arrivals "
                        f"at {rate}/s in real time; rounds of {round_steps} steps; HBM budget "
-                       "64 GiB")
+                       f"{budget_gib:g} GiB")
     return out
 
 
@@ -380,7 +382,8 @@ def main():
         b["tuning"] = str(TUNING_FILE.relative_to(ROOT)) if TUNING_FILE.exists() else "default"
         line["batched_ensemble"] = b
     if not args.no_serve:
-        line["serving"] = run_serving(args.serve_requests, args.serve_rate, args.serve_round)
+        line["serving"] = run_serving(args.serve_requests, args.serve_rate, args.serve_round,
+                                      args.serve_budget_gib)
     incut = in_cutoff_fraction(w.params) * n * (n - 1)    # directed pairs inside rc
     swept = f64["directed_pairs_evaluated_per_launch"]
     line["roofline"] = {

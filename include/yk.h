@@ -302,7 +302,7 @@ int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *
 int yk_batch_remove(yk_batch *b, int32_t id);
 
 /* Tuning per N bucket (first k with n <= n_upper[k]): list_items > 0 gives
- * the bucket's replicas (up to 65,536 particles) 16-slot cluster pair lists
+ * the bucket's replicas (up to 262,144 particles) 16-slot cluster pair lists
  * with that many work items per 64-slot tile; list_items = 0 gives them tile
  * items of chunk_tiles source tiles each.  waves_per_block (1, 2, 4 or 8):
  * the persistent launch takes the waves of the bucket that holds the largest

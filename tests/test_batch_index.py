@@ -48,14 +48,15 @@ def test_current_candidates_are_a_subset_of_the_checked_grid():
     for hi in DEFAULT_BUCKETS:
         checked = {tuple(sorted(c.items())) for c in B.fault_sweep_candidates(hi)}
         for c in candidates(hi):
-            if c["waves_per_block"] == 8 or c["list_items"] == 32:
+            if c["waves_per_block"] == 8 or c["list_items"] == 32 or \
+                    (c["list_items"] > 0 and hi > 65536):
                 continue   # shapes the current grid adds: covered by the next test
             assert tuple(sorted(c.items())) in checked, (hi, c)
 
 
 def test_current_candidates_in_bounds():
-    """The current sweep grid (8-wave workgroups and 32 list items added) on
-    every bucket, reduced ensembles."""
+    """The current sweep grid (8-wave workgroups, 32 list items and lists up
+    to 262,144 particles added) on every bucket, reduced ensembles."""
     lo = 1023
     for k, hi in enumerate(DEFAULT_BUCKETS):
         reps = bucket_replicas(lo, hi, 2005 + k, slots=4096)

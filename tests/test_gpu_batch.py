@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import oracle
+from _forces import masses
 from yukawamd_amd import workloads as W
 from yukawamd_amd._lib import YkError
 from yukawamd_amd.batch import Ensemble
@@ -187,6 +188,41 @@ def test_staggered_admission_with_resort_is_bitwise(mode):
         alone = e.get(rid)
     for key in ("x", "v", "a"):
         assert np.array_equal(mixed_in[key], alone[key]), key
+
+
+def test_list_replica_above_65536():
+    """A 100,000-particle FP64 replica on cluster lists (16 items per tile)
+    beside a small one: its initial accelerations match the single-system
+    path's cluster lists to summation-order rounding, and after 4 steps its
+    state still does."""
+    big = W.replica(17, 100_000, True, 180.0, 1.0, "fp64")
+    small = W.replica(18, 3000, False, 170.0, 1.5, "fp64")
+    tuning = [{"n_upper": 1 << 20, "chunk_tiles": 8, "waves_per_block": 4, "list_items": 16}]
+    with Ensemble(capacity=104_192, max_replicas=2, max_records=2, tuning=tuning) as e:
+        e.add_workload(small, steps=4)
+        rid = e.add_workload(big, steps=4)
+        e.run(0)
+        g0 = e.get(rid)
+        e.run(4, nind=2)
+        g = e.get(rid)
+    with YukawaSystem(big.params, Z=big.Z, A=big.A, dt=big.dt) as s:
+        s.put_state(big.x, big.v)
+        m0 = s.accel(True)
+        a0 = s.a
+        recs = s.run(4, nind=2)
+        x, v, a = s.get_state()
+    m = masses(big.params, big.A)[:, None]
+    rel = (np.linalg.norm((g0["a"] - a0) * m, axis=1) / np.linalg.norm(a0 * m, axis=1)).max()
+    print("n=100000 batched list replica vs single system: max |dF|/|F|", rel)
+    assert rel <= 1e-10
+    assert g0["init"]["ev"] == pytest.approx(m0["ev"], rel=1e-12)
+    L = big.params.xl[0]
+    dx = np.abs(x - g["x"])
+    dx = np.minimum(dx, L - dx)
+    assert dx.max() <= 1e-10 * L
+    for r1, r2 in zip(g["records"], recs):
+        assert r1["ev"] == pytest.approx(r2["ev"], rel=1e-11)
+        assert r1["ek"] == pytest.approx(r2["ek"], rel=1e-10)
 
 
 def test_list_arena_beyond_2_31_entries():

@@ -11,7 +11,7 @@ shape and stepped, and the shortest device time per step wins.  The knobs:
   list_items       > 0: the replicas get 16-slot cluster pair lists and this
                    many work items per 64-slot target tile (item = an equal
                    share of each of the tile's four cluster lists); 0: tile
-                   items (replicas above 65,536 particles always use them);
+                   items (replicas above 262,144 particles always use them);
   chunk_tiles      tile items: source tiles per item, the item grain (load
                    balance against per-item overhead: partial-sum traffic and
                    finalize work grow with the items per tile);
@@ -39,7 +39,7 @@ DEFAULT_BUCKETS = [2048, 4096, 8192, 16384, 65536, 262144]
 WAVES = [2, 4, 8]
 CHUNKS = [4, 8, 16, 32, 64, 128]
 LIST_ITEMS = [4, 8, 16, 32]
-CLIST_MAX_N = 65536
+CLIST_MAX_N = 262144
 MAX_ITEMS_PER_TILE = 64
 BUCKET_SLOTS = 1 << 20          # particles per tuning ensemble (at least one replica)
 

@@ -35,13 +35,14 @@ struct Tuning {
         if (k >= 0) return std::max(1, std::min(nt, chunk_tiles[k]));
         return std::max(1, (nt + 3) / 4);   // default: four source ranges per target tile
     }
-    // work items per tile of a list replica (0: tile items)
+    // work items per tile of a list replica (0: tile items); without a
+    // bucket: 16 up to 65,536 particles (1 GiB of lists at most)
     int list_items_for(int n) const
     {
         if (n > CLIST_MAX_N) return 0;
         const int k = bucket_of(n);
         if (k >= 0) return list_items[k];
-        return 16;
+        return n <= 65536 ? 16 : 0;
     }
 };
 

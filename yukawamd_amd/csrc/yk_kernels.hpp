@@ -1054,7 +1054,7 @@ static __global__ __launch_bounds__(BLOCK) void k_clist_build(
     if (!lists_stale(maxdisp, lc.skin)) return;   // lists still valid (uniform)
     constexpr int W = BLOCK / TILE;
     __shared__ float4 stage[2][W][TILE];
-    __shared__ int ctiles[CLIST_MAX_N_SYSTEM / TILE / NSEG + 1];
+    __shared__ int ctiles[CLIST_MAX_N / TILE / NSEG + 1];
     const int t = blockIdx.x;
     const int seg = blockIdx.y;   // source tiles st with st % NSEG == seg
     const int lane = threadIdx.x & (TILE - 1), wv = threadIdx.x / TILE;

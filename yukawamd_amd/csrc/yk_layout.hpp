@@ -14,12 +14,11 @@ constexpr int NQ = 14;       // reduced scalars: E, W, PP[6], KE[6]
 // kernel takes four.
 constexpr int CL = 16;              // slots per cluster
 constexpr int CPW = TILE / CL;      // clusters per wave in the pair kernel
-// Cluster lists hold up to n entries per cluster (n^2/4 bytes in all).  The
-// batched engine's replicas use them up to CLIST_MAX_N particles (tile items
-// above), a single system up to CLIST_MAX_N_SYSTEM (17 GB of lists at the
-// top; the 64-slot tile lists above, n^2/16 bytes).
-constexpr int CLIST_MAX_N = 65536;
-constexpr int CLIST_MAX_N_SYSTEM = 262144;
+// Cluster lists hold up to n entries per cluster (n^2/4 bytes in all): used
+// up to CLIST_MAX_N particles (16 GiB of lists at the top); larger single
+// systems take the 64-slot tile lists (n^2/16 bytes), larger batched replicas
+// tile items.
+constexpr int CLIST_MAX_N = 262144;
 // A cluster's list is stored as NSEG segments: segment s holds the entries
 // whose source tile index is s mod NSEG (each built by its own workgroup, for
 // parallelism), each with its fast entries from the front and general ones

@@ -1138,7 +1138,7 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
     s->use_list = !(params->flags & YK_FLAG_ALLPAIRS);
     // items per target tile (cluster set)
     s->list_M = (params->precision == YK_MIXED) ? LIST_ITEMS_MIXED : LIST_ITEMS_FP64;
-    if (s->use_list && !(params->flags & YK_FLAG_TILE_LISTS) && n <= CLIST_MAX_N_SYSTEM) {
+    if (s->use_list && !(params->flags & YK_FLAG_TILE_LISTS) && n <= CLIST_MAX_N) {
         // small systems: more items per cluster set, up to 128, until the
         // launch has 8,192 waves (N = 3,456: 128 items, 6,912 waves, 30 us
         // per force launch against 33 us with 32 items)
@@ -1153,7 +1153,7 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
         s->part_doubles = std::max(s->part_doubles,
                                    (size_t)s->ntiles * s->list_M * 11 * TILE);
         YK_HIPB(hipMalloc(&s->xref, sizeof(double4) * nn));
-        s->use_clist = !(params->flags & YK_FLAG_TILE_LISTS) && n <= CLIST_MAX_N_SYSTEM;
+        s->use_clist = !(params->flags & YK_FLAG_TILE_LISTS) && n <= CLIST_MAX_N;
         if (!s->use_clist) {
             YK_HIPB(hipMalloc(&s->list, sizeof(unsigned) * (size_t)s->ntiles * nn));
             YK_HIPB(hipMalloc(&s->counts, sizeof(int2) * (size_t)s->ntiles));
