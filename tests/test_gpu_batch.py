@@ -225,6 +225,27 @@ def test_list_replica_above_65536():
         assert r1["ek"] == pytest.approx(r2["ek"], rel=1e-10)
 
 
+def test_out_of_memory_at_admission():
+    """Replicas of 262,144 particles on cluster lists take 16 GiB of list
+    arena each; the arena doubles as it grows (16, 32, 64, 128 GiB), so the
+    ninth admission needs a 256 GiB arena beside the 128 GiB one and is
+    refused with YK_ERR_NOMEM.  The ensemble keeps running (the failed
+    allocation leaves no error behind for the next launch check), and after a
+    removal the same replica is admitted."""
+    tuning = [{"n_upper": 1 << 20, "chunk_tiles": 8, "waves_per_block": 4, "list_items": 4}]
+    reps = [W.replica(131 + k, 262144, False, 175.0, 1.0, "mixed") for k in range(9)]
+    with Ensemble(capacity=9 * 262144, max_replicas=9, max_records=1, tuning=tuning) as e:
+        ids = [e.add_workload(w, steps=2) for w in reps[:8]]
+        with pytest.raises(YkError) as err:
+            e.add_workload(reps[8], steps=2)
+        assert err.value.code == 3
+        e.run(1)
+        e.remove(ids[0])
+        ids[0] = e.add_workload(reps[8], steps=1)
+        e.run(1)
+        assert all(e.status(r)[0] == 0 for r in ids)
+
+
 def test_list_arena_beyond_2_31_entries():
     """Nine replicas of 65,536 particles on cluster lists hold 9 x 2^28 list
     entries: the ninth replica's lists start at entry 2^31, where the build

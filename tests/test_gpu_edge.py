@@ -24,6 +24,8 @@ import pytest
 
 import oracle
 from _forces import check_meas, force_errors, gpu_accel
+from yukawamd_amd import workloads as W
+from yukawamd_amd._lib import YkError
 from yukawamd_amd.params import SimParams
 from yukawamd_amd.system import YukawaSystem
 
@@ -147,3 +149,18 @@ def test_bfield_sim_type_has_no_pair_force():
         s.accel(True)
         a_g = s.a
     assert np.array_equal(a_o, np.zeros((n, 3))) and np.array_equal(a_g, a_o)
+
+
+def test_out_of_memory_at_create():
+    """A system whose tile lists would take 25 TB is refused with
+    YK_ERR_NOMEM, and the failed allocation leaves no error behind: a system
+    created next computes its forces."""
+    p = W.pure_ion(n=1000).params.replace(tile_lists=True)
+    with pytest.raises(YkError) as err:
+        YukawaSystem(p, n=20_000_000)
+    assert err.value.code == 3
+    w = W.pure_ion(n=1000, seed=5)
+    with YukawaSystem(w.params, Z=w.Z, A=w.A) as s:
+        s.put_state(w.x)
+        s.accel(False)
+        assert np.isfinite(s.a).all() and np.abs(s.a).max() > 0

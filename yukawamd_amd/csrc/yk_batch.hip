@@ -46,13 +46,18 @@ int bfail(int code, const char *fmt, ...)
     return code;
 }
 
+// A failed allocation is returned as YK_ERR_NOMEM, which callers may handle
+// (the serving loop waits for a retirement); the runtime's last-error state
+// is cleared of it, so that the next launch check does not report it again.
 #define YB_HIP(call)                                                                   \
     do {                                                                               \
         hipError_t e_ = (call);                                                        \
-        if (e_ != hipSuccess)                                                          \
+        if (e_ != hipSuccess) {                                                        \
+            if (e_ == hipErrorOutOfMemory) (void)hipGetLastError();                    \
             return bfail(e_ == hipErrorOutOfMemory ? YK_ERR_NOMEM : YK_ERR_HIP,         \
                          "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
                          __LINE__);                                                    \
+        }                                                                              \
     } while (0)
 
 struct HostReplica {
@@ -270,6 +275,30 @@ __global__ void k_fill_rep(int *rep, const int *tile_rep, const RepConsts *reps,
 
 // Items, tile item ranges, per-slot replica index and replica constants for
 // the current membership.
+// Room for nitems work items: partial sums, item records and queue order,
+// allocated 25 % ahead.  A reallocation drops the captured graph (it holds
+// the old pointers) and marks the layout for a rebuild; a failed one leaves
+// no buffers (capacity 0) and returns YK_ERR_NOMEM.
+int ensure_items(yk_batch *b, size_t nitems)
+{
+    if (nitems <= b->part_items_cap) return YK_OK;
+    drop_graph(b);
+    b->dirty = true;
+    if (b->part) YB_HIP(hipFree(b->part));
+    if (b->items) YB_HIP(hipFree(b->items));
+    if (b->qorder) YB_HIP(hipFree(b->qorder));
+    b->part = nullptr;
+    b->items = nullptr;
+    b->qorder = nullptr;
+    b->part_items_cap = 0;
+    const size_t want = plan::item_capacity((int)nitems);
+    YB_HIP(hipMalloc(&b->part, sizeof(double) * 11 * TILE * want));
+    YB_HIP(hipMalloc(&b->items, sizeof(BatchItem) * want));
+    YB_HIP(hipMalloc(&b->qorder, sizeof(int) * want));
+    b->part_items_cap = want;
+    return YK_OK;
+}
+
 int rebuild(yk_batch *b)
 {
     static_assert(sizeof(plan::Item) == sizeof(BatchItem), "BatchItem is plan::Item");
@@ -298,20 +327,7 @@ int rebuild(yk_batch *b)
     if (!L.list_tiles.empty())
         YB_HIP(hipMemcpyAsync(b->list_tiles, L.list_tiles.data(), sizeof(int) * L.list_tiles.size(),
                               hipMemcpyHostToDevice, b->stream));
-    if ((size_t)b->nitems > b->part_items_cap) {
-        if (b->part) YB_HIP(hipFree(b->part));
-        if (b->items) YB_HIP(hipFree(b->items));
-        if (b->qorder) YB_HIP(hipFree(b->qorder));
-        b->part = nullptr;
-        b->items = nullptr;
-        b->qorder = nullptr;
-        b->part_items_cap = 0;
-        const size_t want = plan::item_capacity(b->nitems);
-        YB_HIP(hipMalloc(&b->part, sizeof(double) * 11 * TILE * want));
-        YB_HIP(hipMalloc(&b->items, sizeof(BatchItem) * want));
-        YB_HIP(hipMalloc(&b->qorder, sizeof(int) * want));
-        b->part_items_cap = want;
-    }
+    if (const int e = ensure_items(b, (size_t)b->nitems)) return e;
     YB_HIP(hipMemcpyAsync(b->reps, rc.data(), sizeof(RepConsts) * b->max_rep,
                           hipMemcpyHostToDevice, b->stream));
     YB_HIP(hipMemcpyAsync(b->tile_rep, L.tile_rep.data(), sizeof(int) * ntiles_cap,
@@ -562,9 +578,11 @@ int yk_batch_create(int64_t capacity, int32_t max_replicas, int32_t max_records,
 #define YB_HIPB(call)                                                                  \
     do {                                                                               \
         hipError_t e_ = (call);                                                        \
-        if (e_ != hipSuccess)                                                          \
+        if (e_ != hipSuccess) {                                                        \
+            if (e_ == hipErrorOutOfMemory) (void)hipGetLastError();                    \
             return bail(bfail(e_ == hipErrorOutOfMemory ? YK_ERR_NOMEM : YK_ERR_HIP,    \
                               "%s failed: %s", #call, hipGetErrorString(e_)));         \
+        }                                                                              \
     } while (0)
     YB_HIPB(hipGetDevice(&b->device));
     hipDeviceProp_t prop;
@@ -724,6 +742,22 @@ int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *
         h.segcap = plan::segcap_of(n);
         h.lists_len = plan::list_entries_of(n);
         if ((rc = list_alloc(b, h.lists_len, &h.lists_off))) {
+            h = HostReplica{};
+            return rc;
+        }
+    }
+    {   // the membership's work items with this replica: device memory for
+        // them is taken now, so that a shortage is reported here, not by a run
+        int64_t total = plan::member_items(b->tune, plan::Member{true, n, h.off, nblk,
+                                                                 h.list_items});
+        for (int r = 0; r < b->max_rep; ++r) {
+            const HostReplica &o = b->table[r];
+            if (o.active)
+                total += plan::member_items(b->tune, plan::Member{true, o.n, o.off, o.nblocks,
+                                                                  o.list_items});
+        }
+        if ((rc = ensure_items(b, (size_t)total))) {
+            if (h.list_items > 0) list_free(b, h.lists_off, h.lists_len);
             h = HostReplica{};
             return rc;
         }

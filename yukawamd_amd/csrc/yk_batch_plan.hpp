@@ -97,6 +97,16 @@ struct Member {
     int list_items = 0;                // > 0: cluster lists, this many items per tile
 };
 
+// Work items of a member as layout() makes them: per target tile, M list
+// shares or one item per chunk of source tiles.
+inline int64_t member_items(const Tuning &t, const Member &h)
+{
+    const int64_t nt = (h.n + TILE - 1) / TILE;
+    if (h.list_items > 0) return nt * h.list_items;
+    const int64_t C = t.chunk_tiles_for(h.n);
+    return nt * ((nt + C - 1) / C);
+}
+
 // A work item as the pair kernel reads it (BatchItem's layout): tile item =
 // target tile and source tiles [s0, s1), global indices; list item (s1 < 0) =
 // the target tile's four clusters, share s0 of -s1.

@@ -43,9 +43,12 @@ int fail(int code, const char *fmt, ...)
 #define YK_HIP(call)                                                                   \
     do {                                                                               \
         hipError_t e_ = (call);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            return fail(YK_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                           \
+        if (e_ != hipSuccess) {                                                        \
+            if (e_ == hipErrorOutOfMemory) (void)hipGetLastError();                    \
+            return fail(e_ == hipErrorOutOfMemory ? YK_ERR_NOMEM : YK_ERR_HIP,          \
+                        "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                     \
+        }                                                                              \
     } while (0)
 
 }  // namespace
@@ -1096,12 +1099,17 @@ int yk_create(const yk_params *params, int32_t n, const double *zii, const doubl
         release(s);
         return code;
     };
+    // a failed allocation leaves the runtime's last-error state set: cleared,
+    // so that a later launch check (of this or another system) does not
+    // report it
 #define YK_HIPB(call)                                                                  \
     do {                                                                               \
         hipError_t e_ = (call);                                                        \
-        if (e_ != hipSuccess)                                                          \
+        if (e_ != hipSuccess) {                                                        \
+            if (e_ == hipErrorOutOfMemory) (void)hipGetLastError();                    \
             return bail(fail(e_ == hipErrorOutOfMemory ? YK_ERR_NOMEM : YK_ERR_HIP,    \
                              "%s failed: %s", #call, hipGetErrorString(e_)));          \
+        }                                                                              \
     } while (0)
     const size_t nn = (size_t)n;
     // per-particle Z, A and the acceleration denominator, caller order
