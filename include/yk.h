@@ -71,7 +71,7 @@ enum yk_flags {
      * excludes YK_FLAG_KDK. */
     YK_FLAG_SW = 1u << 4,
     /* Pair lists per 64-particle tile only, without the 16-particle cluster
-     * lists built from them (which systems up to 65,536 particles use by
+     * lists built from them (which systems up to 262,144 particles use by
      * default); fixed at yk_create. */
     YK_FLAG_TILE_LISTS = 1u << 5
 };

@@ -1,12 +1,14 @@
 """Autotuner of the batched engine's work shape, per N bucket.
 
-    python -m yukawamd_amd.autotune [--out PATH] [--steps K] [--buckets 2048,4096,...]
+    python -m yukawamd_amd.autotune [--out PATH] [--steps K] [--nind M] [--buckets 2048,...]
 
 For every N bucket an ensemble of replicas drawn from that bucket alone
 (N log-uniform inside the bucket, pure Z=8 or the three-species mixture with
 probability 1/2, Gamma U[150, 220], kappa U[0.5, 2], mixed precision: the
 batched-ensemble parameter ranges) is admitted afresh under every candidate
-shape and stepped, and the shortest device time per step wins.  The knobs:
+shape and stepped (K = 10 timed steps measuring every M = 10th, as records
+are taken in production runs: the measuring step's kernels weigh in), and the
+shortest device time per step wins.  The knobs:
 
   list_items       > 0: the replicas get 16-slot cluster pair lists and this
                    many work items per 64-slot target tile (item = an equal
@@ -79,7 +81,7 @@ def candidates(n_max: int):
     return out
 
 
-def tune_bucket(lo, hi, steps, warmup, seed, log):
+def tune_bucket(lo, hi, steps, warmup, seed, log, nind=0):
     from .batch import Ensemble
     reps = bucket_replicas(lo, hi, seed)
     cap = sum(((w.n + 255) // 256) * 256 for w in reps)
@@ -88,11 +90,13 @@ def tune_bucket(lo, hi, steps, warmup, seed, log):
     for cand in candidates(hi):
         tuning = [dict(cand, n_upper=hi)]
         # the list / tile choice is made at admission: a fresh ensemble each
-        with Ensemble(capacity=cap, max_replicas=len(reps), max_records=1, tuning=tuning) as e:
+        nrec = max(1, (steps + warmup) // nind) if nind else 1
+        with Ensemble(capacity=cap, max_replicas=len(reps), max_records=nrec,
+                      tuning=tuning) as e:
             for w in reps:
                 e.add_workload(w, steps=steps + warmup)
             e.run(warmup)
-            e.run(steps)
+            e.run(steps, nind)
             ms = e.last_run()["ms"] / steps
         r = dict(cand, ms_per_step=ms, pairs_per_s=pairs / (ms * 1e-3))
         results.append(r)
@@ -104,7 +108,8 @@ def tune_bucket(lo, hi, steps, warmup, seed, log):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=None)
-    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--nind", type=int, default=10, help="measure every nind-th step")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--buckets", default=",".join(map(str, DEFAULT_BUCKETS)))
     ap.add_argument("--seed", type=int, default=1005)
@@ -122,14 +127,14 @@ def main(argv=None):
     for k, hi in enumerate(uppers):
         log(f"bucket ({lo}, {hi}]")
         best, results, nrep, pairs = tune_bucket(lo, hi, args.steps, args.warmup,
-                                                  args.seed + k, log)
+                                                  args.seed + k, log, args.nind)
         table.append({"n_upper": hi, "list_items": best["list_items"],
                       "chunk_tiles": best["chunk_tiles"],
                       "waves_per_block": best["waves_per_block"],
                       "ms_per_step": best["ms_per_step"], "pairs_per_s": best["pairs_per_s"]})
         detail[str(hi)] = {"replicas": nrep, "pairs_per_step": pairs, "results": results}
         lo = hi
-    out = {"device": dev, "precision": "mixed", "steps": args.steps,
+    out = {"device": dev, "precision": "mixed", "steps": args.steps, "nind": args.nind,
            "seconds": time.perf_counter() - t0, "buckets": table, "candidates": detail}
     text = json.dumps(out, indent=1)
     if args.out:
