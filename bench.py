@@ -99,7 +99,8 @@ def parse():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the pure-ion (N=3,456) and nucleon (N=51,200) lines")
     ap.add_argument("--config-steps", type=int, default=100)
-    ap.add_argument("--config-only", default=None, choices=["pure_ion_3456", "nucleon_51200"],
+    ap.add_argument("--config-only", default=None,
+                    choices=["pure_ion_3456", "nucleon_51200", "ion_mixture_262144"],
                     help="only run that config's line (both forms) and print it as the JSON "
                          "line (for kernel traces of one workload)")
     ap.add_argument("--no-drift", action="store_true",
@@ -303,6 +304,8 @@ def config_only(args):
     from yukawamd_amd import workloads as W
     if args.config_only == "pure_ion_3456":
         wc, steps = W.pure_ion(), args.config_steps
+    elif args.config_only == "ion_mixture_262144":   # the largest system on cluster lists
+        wc, steps = W.ion_mixture(n=262144, seed=1006), max(args.config_steps // 5, 10)
     else:
         wc, steps = W.nucleon(), max(args.config_steps // 2, 10)
     print(json.dumps({"config_only": wc.name, wc.name: config_line(wc, args, steps)}))
