@@ -224,6 +224,37 @@ def test_cluster_lists_above_65536():
         assert cond <= 1e-12 and rel <= 1e-10
 
 
+def test_large_systems_mixed_and_nucleon():
+    """Above 65,536 particles on cluster lists (radix-sorted cluster order):
+    the mixed ion form at N = 100,000 against the FP64 tile lists (the mixed
+    bar: max |dF|/|F| <= 1e-6), and nucleons at N = 100,000 (charge-sorted
+    Hilbert runs, neutral sources listed to the nuclear radius only) against
+    the tile lists in FP64 (|dF_i| <= 1e-12 of the largest force: the two
+    differ only in summation order), with the total force vanishing."""
+    w = W.ion_mixture(n=100_000, seed=98)
+    p = w.params
+    a_t, m_t = gpu_accel(p, w.x, w.Z, w.A, True, "tilelist")
+    a_m, m_m = gpu_accel(p.replace(precision="mixed"), w.x, w.Z, w.A, True, "clist")
+    m = masses(p, w.A)[:, None]
+    rel = (np.linalg.norm((a_m - a_t) * m, axis=1) / np.linalg.norm(a_t * m, axis=1)).max()
+    print(f"n=100000 mixed cluster lists vs FP64 tile lists: max |dF|/|F| {rel:.2e}")
+    assert rel <= 1e-6
+    assert m_m["ev"] == pytest.approx(m_t["ev"], rel=1e-6)
+    w = W.nucleon(n=100_000, seed=99)
+    p = w.params
+    a_t, m_t = gpu_accel(p, w.x, w.Z, w.A, True, "tilelist")
+    a_c, m_c = gpu_accel(p, w.x, w.Z, w.A, True, "clist")
+    m = masses(p, w.A)[:, None]
+    dF = np.linalg.norm((a_c - a_t) * m, axis=1)
+    F = np.linalg.norm(a_t * m, axis=1)
+    print(f"n=100000 nucleons cluster lists vs tile lists: max |dF|/|F| "
+          f"{(dF / F).max():.2e}, max |dF| / max |F| {dF.max() / F.max():.2e}")
+    assert dF.max() <= 1e-12 * F.max()
+    assert m_c["ev"] == pytest.approx(m_t["ev"], rel=1e-12)
+    Fc = a_c * m
+    assert np.abs(Fc.sum(0)).max() <= 1e-12 * np.abs(Fc).sum()
+
+
 def test_cluster_lists_at_the_size_limit():
     """262,144 particles, the largest single system on cluster lists: 2^32
     list entries (16 GiB), so segment offsets pass 2^31 entries (a regression
