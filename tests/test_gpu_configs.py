@@ -165,8 +165,8 @@ def test_ensemble_sample_alone_equals_in_ensemble():
 
 
 def test_serving_window_served_equals_alone():
-    """Requests 0-9 of the seed-1004 stream (N from 3,282 to 243,423: the tile
-    items above 65,536 particles; FP64 and mixed), steps
+    """Requests 0-9 of the seed-1004 stream (N from 3,282 to 243,423, on the
+    committed table's cluster lists and tile items; FP64 and mixed), steps
     capped at 20, arriving at 10 per second: each result is bitwise the same as
     the request alone; the FP64 request 0 (N = 3,851) follows the oracle."""
     reqs = stream(10, rate_per_s=10.0)
