@@ -288,14 +288,19 @@ int yk_batch_create(int64_t capacity, int32_t max_replicas, int32_t max_records,
 void yk_batch_destroy(yk_batch *b);
 
 /* Footprint of a replica of n particles under b's current tuning: the arena
- * slots it occupies and the device bytes it holds (its slots plus the
- * per-item partial sums of its work items). */
+ * slots it occupies and the device bytes it holds (its slots, the per-item
+ * partial sums of its work items and, for a list replica, its list region of
+ * n^2/4 bytes at most). */
 int yk_batch_footprint(const yk_batch *b, int32_t n, int64_t *slots, int64_t *bytes);
 
 /* Admit a replica: positions/velocities in (3,n) layout (v may be NULL),
  * zii/aii as in yk_create (NULL allowed for pure-ion), its timestep and its
  * step budget.  Accelerations of the initial positions are computed at the
- * next yk_batch_run (the initial record, yk_batch_get's init). */
+ * next yk_batch_run (the initial record, yk_batch_get's init).  The device
+ * memory the replica needs (its list region, the membership's work-item
+ * buffers) is taken here: YK_ERR_NOMEM when the device is short of it,
+ * YK_ERR_CAPACITY when the arena or the replica table is full; either leaves
+ * the ensemble as it was. */
 int yk_batch_add(yk_batch *b, const yk_params *params, int32_t n, const double *zii,
                  const double *aii, const double *x, const double *v, double dt,
                  int32_t steps, int32_t *id);
