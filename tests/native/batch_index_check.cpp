@@ -267,6 +267,20 @@ int add(Engine &e, int k, int &id_out)
         for (int64_t u = s0; u < s0 + units; ++u) e.list_used[(size_t)u] = 1;
         h.lists_off = s0 * plan::LIST_UNIT;
     }
+    {   // the engine takes the membership's work-item buffers at admission
+        int64_t total = plan::member_items(e.tune, plan::Member{true, n, h.off, nblk,
+                                                               h.list_items});
+        for (int r = 0; r < e.max_rep; ++r) {
+            const Rep &o = e.table[r];
+            if (o.active)
+                total += plan::member_items(e.tune, plan::Member{true, o.n, o.off, o.nblocks,
+                                                                 o.list_items});
+        }
+        if ((size_t)total > e.part_items_cap) {
+            e.part_items_cap = plan::item_capacity((int)total);
+            e.part_by.assign(e.part_items_cap, -1);   // a new allocation: nothing written yet
+        }
+    }
     for (int s = start; s < start + nblk; ++s) e.used[(size_t)s] = 1;
     h.active = true;
     h.fresh = true;
@@ -301,6 +315,12 @@ void rebuild(Engine &e, Checker &ck)
     }
     e.L = plan::layout(e.tune, members, e.cap);
     const int nitems = (int)e.L.items.size();
+    {   // the items reserved at admission (member_items) are the layout's
+        int64_t reserved = 0;
+        for (const plan::Member &m : members)
+            if (m.active) reserved += plan::member_items(e.tune, m);
+        ck.in("work items of the layout against member_items", nitems, reserved, reserved + 1);
+    }
     e.rpt->items_max = std::max<int64_t>(e.rpt->items_max, nitems);
     if (e.L.list_tiles.size() > e.list_tiles_cap)
         e.list_tiles_cap = e.L.list_tiles.size() + e.L.list_tiles.size() / 4 + 64;
