@@ -107,7 +107,7 @@ def parse():
                     help="skip the energy drift over 1,000 steps (mixture and nucleons)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=40)
-    ap.add_argument("--serve-rate", type=float, default=1.5, help="Poisson arrivals per s")
+    ap.add_argument("--serve-rate", type=float, default=3.0, help="Poisson arrivals per s")
     ap.add_argument("--serve-round", type=int, default=50, help="steps per serving round")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0,
                     help="HBM the serving loop admits requests against")
