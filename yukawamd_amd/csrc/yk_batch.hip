@@ -282,6 +282,8 @@ __global__ void k_fill_rep(int *rep, const int *tile_rep, const RepConsts *reps,
 int ensure_items(yk_batch *b, size_t nitems)
 {
     if (nitems <= b->part_items_cap) return YK_OK;
+    if (nitems > (size_t)(INT32_MAX / 2))   // item indices are 32-bit
+        return bfail(YK_ERR_NOMEM, "%zu work items exceed the engine's limit", nitems);
     drop_graph(b);
     b->dirty = true;
     if (b->part) YB_HIP(hipFree(b->part));
