@@ -9,11 +9,15 @@
 * batched ensemble: 32 replicas of the 256-replica draw (seed 1003; every N
   bucket of the tuning table) under the committed tuning table, each replica
   bitwise the same alone and in the ensemble, FP64 copies against the oracle;
+  one replica per N bucket over 1,000 steps of the mixed batched form: energy
+  drift within the single systems' bar;
 * serving stream: the first 10 requests of the seed-1004 stream (two above
   65,536 particles, four in FP64), steps capped at 20: every served result
   bitwise the same as the request run alone, and an FP64 request's
   trajectory against the oracle's velocity-Verlet steps.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -237,6 +241,38 @@ def test_energy_drift_ion_mixture_27648():
     print(f"ion_mixture_27648 drift per 1,000 steps: fp64 {d['fp64']:.3e} mixed {d['mixed']:.3e}")
     assert d["fp64"] <= 1e-5
     assert d["mixed"] <= 1e-5 and d["mixed"] <= d["fp64"] + 1e-7
+
+
+def test_energy_drift_batched_replicas():
+    """Replicas of the 256-replica ensemble draw, one per N bucket up to 16,384,
+    in the batched engine's mixed form: from an FP64-warmed state (20 groups of
+    25 steps, each followed by cancel_vcom), 1,000 steps measuring every 10th;
+    each replica's relative drift is at most 1e-5, the single systems' bar."""
+    ens = W.ensemble(256)
+    picks = [next(w for w in ens if lo < w.n <= hi)
+             for lo, hi in ((1023, 2048), (2048, 4096), (4096, 8192), (8192, 16384))]
+    warmed = []
+    for w in picks:
+        with YukawaSystem(w.params.replace(precision="fp64"), Z=w.Z, A=w.A, dt=w.dt) as s:
+            s.put_state(w.x, w.v)
+            s.accel(True)
+            for _ in range(20):
+                s.run(25)
+                s.cancel_vcom()
+            x, v, _ = s.get_state()
+            s.put_state(x, v)
+            e0 = s.accel(True)["ev"] + s.ttot()
+        warmed.append((dataclasses.replace(w, x=x, v=v), e0))
+    cap = sum(((w.n + 255) // 256) * 256 for w, _ in warmed)
+    with Ensemble(capacity=cap, max_replicas=4, max_records=100) as e:
+        ids = [e.add_workload(w, steps=1000) for w, _ in warmed]
+        e.run(1000, nind=10)
+        for rid, (w, e0) in zip(ids, warmed):
+            recs = e.get(rid)["records"]
+            assert len(recs) == 100
+            drift = max(abs(r["ev"] + r["ek"] - e0) for r in recs) / abs(e0)
+            print(f"{w.name} (N = {w.n}, mixed, batched) drift per 1,000 steps: {drift:.3e}")
+            assert drift <= 1e-5
 
 
 def test_energy_drift_nucleon_51200():
