@@ -6,6 +6,7 @@
   mix of other replicas;
 * the batch matches the single-system path to summation-order rounding;
 * step budgets, removal/re-admission and the capacity limit behave;
+* mixed replicas meet the mixed bars against the oracle;
 * work shapes outside the committed table reproduce the oracle;
 * a list arena past 2^31 entries gives the replica above that line the same
   trajectory as alone.
@@ -51,6 +52,28 @@ def test_batch_fp64_accel_matches_oracle(mode):
             assert err <= 1e-12
             assert g["init"]["ev"] == pytest.approx(m_o["ev"], rel=1e-13)
             assert g["init"]["px"] == pytest.approx(m_o["px"], rel=1e-13)
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_mixed_replicas_against_oracle(mode):
+    """Mixed-precision replicas against the FP64 oracle: on cluster lists (the
+    FP64 near field) every particle's force within 1e-6 relative, on tile
+    items (every pair in FP32) within 1e-4; energies within 1e-6."""
+    reps = [W.replica(15, 2500, True, 175.0, 0.7, "mixed"),
+            W.replica(16, 1800, False, 200.0, 1.6, "mixed")]
+    with Ensemble(capacity=4608, max_replicas=2, tuning=MODES[mode]) as e:
+        ids = [e.add_workload(w, steps=0) for w in reps]
+        e.run(0)
+        for rid, w in zip(ids, reps):
+            g = e.get(rid)
+            p = w.params.replace(precision="fp64")
+            a_o, m_o = oracle.accel(p, w.x, w.Z, w.A, True, ref_f1=0)
+            m = masses(p, w.A)[:, None]
+            rel = (np.linalg.norm((g["a"] - a_o) * m, axis=1)
+                   / np.linalg.norm(a_o * m, axis=1)).max()
+            print(mode, w.name, w.n, "mixed replica: max |dF|/|F|", rel)
+            assert rel <= (1e-6 if mode == "lists" else 1e-4)
+            assert g["init"]["ev"] == pytest.approx(m_o["ev"], rel=1e-6)
 
 
 @pytest.mark.parametrize("mode", list(MODES))
